@@ -22,8 +22,31 @@ F64 = ctypes.c_double
 
 # name -> (restype, argtypes).  Mirrors include/gospl_hip.h.
 SIGNATURES = {
+    "gospl_last_error": (ctypes.c_char_p, []),
     "gospl_fv_geometry": (I32, [P, I64, P, P, I64, P, I64, P, I32, P, P, P, P, P]),
+    "gospl_mesh_create": (I32, [I64, I32, I32, P, P, P, P, P, P, P, ctypes.POINTER(P)]),
+    "gospl_mesh_destroy": (I32, [P]),
+    "gospl_mfdreceivers": (I32, [P, I32, F64, P, F64, P, P, P, P]),
+    "gospl_flow_direction": (I32, [P, I32, F64, P, F64, P, I32, P, P, P, P, P, P, P]),
+    "gospl_fill_depressions": (I32, [P, P, F64, F64, P, P]),
+    "gospl_pit_info": (I32, [P, P, P, F64, P, P, P, P, I64, P, P, P, P, ctypes.POINTER(I64), P]),
+    "gospl_flowacc": (I32, [P, I32, P, P, P, P, ctypes.POINTER(I32), ctypes.POINTER(I64), P]),
+    "gospl_spl": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P,
+                        ctypes.POINTER(I32), P]),
+    "gospl_hillslope": (I32, [P, F64, F64, F64, F64, I32, P, P, P, P]),
+    "gospl_rain_source": (I32, [P, P, P, F64, P, P]),
+    "gospl_pit_inflow": (I32, [P, P, P, P, I64, P, P]),
+    "gospl_pit_water": (I32, [P, P, P, P, P, P, P]),
+    "gospl_cascade_add": (I32, [P, P, P, P, F64, P, P, P]),
+    "gospl_flow_finalize": (I32, [P, P, P, P, I32, P, P, P]),
 }
+
+
+def check(rc: int, what: str = "") -> None:
+    """Raise RuntimeError with the library's message when rc != 0."""
+    if rc != 0:
+        msg = load().gospl_last_error()
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
 
 def _declare(lib):

@@ -1,0 +1,216 @@
+// Mesh handle, error reporting and the small per-step kernels of the
+// flow-accumulation driver (runoff source, pit-overflow cascade pieces,
+// discharge finalisation).
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace gospl {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// Runoff source b = rain * area, zero at sea (flowplex.py:794-795).
+__global__ void k_rain_source(int64_t N, const double* __restrict__ rain,
+                              const double* __restrict__ area, const double* __restrict__ lfill,
+                              double sl, double* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  b[i] = (lfill[i] <= sl) ? 0.0 : rain[i] * area[i];
+}
+
+// inV[p] += fa over lsink cells of pit p (flowplex.py:596-600).
+__global__ void k_pit_inflow(int64_t N, const double* __restrict__ fa,
+                             const int8_t* __restrict__ lsink, const int32_t* __restrict__ pit,
+                             int64_t npit, double* __restrict__ inv) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int32_t p = pit[i];
+  if (lsink[i] && p > -1 && p < npit) atomicAdd(&inv[p], fa[i]);
+}
+
+// Water level of pits (flowplex.py:624-653).
+__global__ void k_pit_water(int64_t N, const int32_t* __restrict__ pit,
+                            const int8_t* __restrict__ full, const double* __restrict__ level,
+                            const double* __restrict__ lfill, double* __restrict__ water) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int32_t p = pit[i];
+  if (p < 0) return;
+  if (full[p]) {
+    water[i] = lfill[i];
+  } else {
+    double l = level[p];
+    if (!isnan(l) && water[i] <= l) water[i] = l;
+  }
+}
+
+// nFA = q*dt ; FAL += (nFA masked where hl < water)/dt (flowplex.py:674-680).
+__global__ void k_cascade_add(int64_t N, const double* __restrict__ q, const double* __restrict__ hl,
+                              const double* __restrict__ water, double dt, double* __restrict__ nfa,
+                              double* __restrict__ fal) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double v = q[i] * dt;
+  nfa[i] = v;
+  double f = (hl[i] < water[i]) ? 0.0 : v;
+  fal[i] = fal[i] + f / dt;
+}
+
+// Discharge at filled pits and its maximum (flowplex.py:906-918).
+__global__ void k_finalize_a(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
+                             const int32_t* __restrict__ pit, int has_pits, double* __restrict__ fal,
+                             unsigned long long* __restrict__ maxkey) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = -INFINITY;
+  if (i < N) {
+    double f = fal[i];
+    if (has_pits && hl[i] <= water[i] && pit[i] > -1) f = 0.0;
+    fal[i] = f;
+    v = f;
+  }
+  // wave-level max then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o, 64);
+    v = v > w ? v : w;
+  }
+  if ((threadIdx.x & 63) == 0 && v > -INFINITY) atomicMax(maxkey, dkey(v));
+}
+
+__global__ void k_finalize_b(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
+                             const int32_t* __restrict__ pit, int has_pits,
+                             const unsigned long long* __restrict__ maxkey, double* __restrict__ fal,
+                             double* __restrict__ fillfal) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double f = fal[i];
+  double g = f;
+  if (has_pits && hl[i] <= water[i] && pit[i] > -1) g = dkey_inv(*maxkey) * 0.1;
+  fal[i] = f < 0.0 ? 0.0 : f;
+  fillfal[i] = g < 0.0 ? 0.0 : g;
+}
+
+}  // namespace gospl
+
+using namespace gospl;
+
+extern "C" {
+
+const char* gospl_last_error(void) { return g_err; }
+
+int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, const int32_t* deg,
+                      const double* elgt, const double* vdist, const double* area,
+                      const int32_t* gid, const int8_t* outlet, gospl_mesh_t** out) {
+  if (!out || N <= 0 || W <= 0 || W > GOSPL_MAXW || N >= (int64_t)INT32_MAX - 2) {
+    set_error("gospl_mesh_create: bad arguments (N=%lld, W=%d)", (long long)N, (int)W);
+    return 1;
+  }
+  gospl_mesh* m = new gospl_mesh();
+  std::memset(m, 0, sizeof(*m));
+  m->N = N;
+  m->W = W;
+  m->flat = flat;
+  m->nbr = nbr;
+  m->deg = deg;
+  m->elgt = elgt;
+  m->vdist = vdist;
+  m->area = area;
+  m->gid = gid;
+  m->outlet = outlet;
+  GOSPL_CHECK(hipGetDevice(&m->device));
+  hipDeviceProp_t prop;
+  GOSPL_CHECK(hipGetDeviceProperties(&prop, m->device));
+  m->num_cu = prop.multiProcessorCount;
+  for (int k = 0; k < 8; ++k) GOSPL_CHECK(hipMalloc(&m->wi[k], sizeof(int32_t) * (N + 64)));
+  for (int k = 0; k < 2; ++k) GOSPL_CHECK(hipMalloc(&m->wu[k], sizeof(unsigned long long) * (N + 64)));
+  for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
+  GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
+  GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
+  GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
+  // scan workspace sized for N ints
+  size_t bytes = 0;
+  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, m->wi[0], m->wi[1], (int)N));
+  m->cub_tmp_bytes = bytes + 256;
+  GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+
+  if (gospl_dag_init_occupancy(m)) return 1;
+  if (gospl_ensure_coef(m, 3)) return 1;
+  *out = m;
+  return 0;
+}
+
+int gospl_ensure_coef(gospl_mesh* m, int rows) {
+  if (m->wcoef_rows >= rows) return 0;
+  if (m->wcoef) GOSPL_CHECK(hipFree(m->wcoef));
+  m->wcoef = nullptr;
+  GOSPL_CHECK(hipMalloc(&m->wcoef, sizeof(double) * rows * (m->N + 64)));
+  m->wcoef_rows = rows;
+  return 0;
+}
+
+int gospl_mesh_destroy(gospl_mesh_t* m) {
+  if (!m) return 0;
+  for (int k = 0; k < 8; ++k) hipFree(m->wi[k]);
+  for (int k = 0; k < 2; ++k) hipFree(m->wu[k]);
+  for (int k = 0; k < 4; ++k) hipFree(m->wd[k]);
+  hipFree(m->counts);
+  hipFree(m->dflag);
+  hipHostFree(m->hflag);
+  hipFree(m->cub_tmp);
+  if (m->wcoef) hipFree(m->wcoef);
+  delete m;
+  return 0;
+}
+
+int gospl_rain_source(gospl_mesh_t* m, const double* rain, const double* lfill, double sl,
+                      double* b, gospl_stream_t stream) {
+  k_rain_source<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, rain, m->area, lfill, sl, b);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_pit_inflow(gospl_mesh_t* m, const double* fa, const int8_t* lsink, const int32_t* pitids,
+                     int64_t npit, double* inv, gospl_stream_t stream) {
+  GOSPL_CHECK(hipMemsetAsync(inv, 0, sizeof(double) * (npit > 0 ? npit : 1), stream));
+  k_pit_inflow<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, fa, lsink, pitids, npit, inv);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_pit_water(gospl_mesh_t* m, const int32_t* pitids, const int8_t* full, const double* level,
+                    const double* lfill, double* water, gospl_stream_t stream) {
+  k_pit_water<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, pitids, full, level, lfill, water);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_cascade_add(gospl_mesh_t* m, const double* q, const double* hl, const double* water,
+                      double dt, double* nfa, double* fal, gospl_stream_t stream) {
+  k_cascade_add<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, q, hl, water, dt, nfa, fal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_flow_finalize(gospl_mesh_t* m, const double* hl, const double* water,
+                        const int32_t* pitids, int32_t has_pits, double* fal, double* fillfal,
+                        gospl_stream_t stream) {
+  unsigned long long* key = m->wu[1];
+  GOSPL_CHECK(hipMemsetAsync(key, 0, sizeof(unsigned long long), stream));
+  k_finalize_a<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, fal, key);
+  k_finalize_b<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, key, fal,
+                                                           fillfal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Shared declarations of the gospl HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gospl_hip.h"
+
+#define GOSPL_MAXW 12        // neighbour cap of the FV stencil
+#define GOSPL_BLOCK 256
+
+// Mesh handle: device geometry owned by the caller + library workspace.
+struct gospl_mesh {
+  int64_t N;
+  int W;                      // ELL width (max degree)
+  int flat;
+  const int32_t* nbr;         // (W, N) device indices, -1 padded
+  const int32_t* deg;         // (N)
+  const double* elgt;         // (W, N)
+  const double* vdist;        // (W, N)
+  const double* area;         // (N)
+  const int32_t* gid;         // (N) input-mesh ids
+  const int8_t* outlet;       // (N) draining border flags (flat meshes)
+  int device;
+  int num_cu;
+  int coop_blocks;            // co-resident blocks for the persistent DAG kernels
+  // workspace
+  int32_t* wi[8];             // 8 x N int32
+  unsigned long long* wu[2];  // 2 x N uint64
+  double* wd[4];              // 4 x N double
+  int32_t* counts;            // N + 2 level counters
+  int32_t* dflag;             // small device flag block (64 ints)
+  int32_t* hflag;             // pinned host mirror (64 ints)
+  double* wcoef;              // (wcoef_rows, N) coefficient rows
+  int wcoef_rows;
+  void* cub_tmp;
+  size_t cub_tmp_bytes;
+};
+
+namespace gospl {
+
+void set_error(const char* fmt, ...);
+}  // namespace gospl
+
+extern "C" {
+int gospl_ensure_coef(gospl_mesh* m, int rows);        // internal (not exported in the header)
+int gospl_dag_init_occupancy(gospl_mesh* m);           // internal
+}
+
+namespace gospl {
+
+inline int nblocks(int64_t n, int b = GOSPL_BLOCK) {
+  int64_t g = (n + b - 1) / b;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// Total order on doubles as unsigned 64-bit keys (no NaNs expected).
+__device__ __forceinline__ unsigned long long dkey(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+}  // namespace gospl
+
+#define GOSPL_CHECK(expr)                                                       \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      gospl::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                       __FILE__, __LINE__);                                     \
+      return 1;                                                                 \
+    }                                                                           \
+  } while (0)

@@ -1,0 +1,578 @@
+// Depression filling and depression information.
+//
+// Fill.  goSPL floods from seed cells with a priority queue (Barnes'
+// priority-flood, functions.F90:2082-2178); the filled value of a reached
+// cell above the level is its minimax ("spill") elevation over paths from a
+// seed.  Here it is computed without a queue:
+//   1. every included cell points at its lowest included neighbour if that
+//      is lower in (elevation, index) order, else at itself (a pit bottom);
+//      seeds point at themselves;
+//   2. pointer jumping gives each cell the root of its descent tree (its
+//      basin); along a descent path elevations never increase, so inside a
+//      basin the minimax value of a cell is its own elevation joined with the
+//      basin's spill level;
+//   3. basin spill levels S (seed basins -inf) relax over the edges that
+//      cross basins: S[ru] = min(S[ru], max(z_u, z_v, S[rv])) until nothing
+//      changes (atomicMin on order-preserving 64-bit keys: exact, no
+//      rounding);
+//   4. fill = max(z, S[basin]) for reached cells above the level, z
+//      elsewhere.
+// Only comparisons and min/max touch the elevations, so the result is
+// bit-identical to the queue-based flood.
+//
+// Depression information (pitfilling.py:704-847): equal-fill connected
+// components with a local-minimum cell (union-find with atomic CAS hooking,
+// ECL-CC style), numbered by the smallest input id among their bottom cells
+// (label_pits' visit order) via a scan over input ids; spill points by
+// atomicMin of input ids; flat-routing distances by relaxation of the
+// breadth-first distance; pit volumes by FP64 atomics.
+#include <climits>
+
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace gospl {
+
+constexpr unsigned long long KEY_POS_INF = 0xFFF0000000000000ull;  // dkey(+inf)
+constexpr unsigned long long KEY_NEG_INF = 0x000FFFFFFFFFFFFFull;  // dkey(-inf)
+
+// state: 1 seed, 0 included, -1 excluded (edge_tile, functions.F90:2036-2080)
+__global__ void k_fill_init(int64_t N, int W, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const int8_t* __restrict__ outlet,
+                            int flat, const double* __restrict__ h, double level,
+                            double* __restrict__ dem, int32_t* __restrict__ state) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double z = h[i] - level;
+  dem[i] = z;
+  const bool border = flat && outlet[i];
+  int st = 0;
+  if (z < 0.0) {
+    bool up = false;
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      int n = nbr[(int64_t)p * N + i];
+      if (n >= 0 && h[n] - level >= 0.0) { up = true; break; }
+    }
+    st = up ? 1 : -1;
+  } else if (border) {
+    st = 1;
+  }
+  state[i] = st;
+}
+
+__global__ void k_fill_descent(int64_t N, int W, const int32_t* __restrict__ nbr,
+                               const int32_t* __restrict__ deg, const double* __restrict__ dem,
+                               const int32_t* __restrict__ state, int32_t* __restrict__ root,
+                               unsigned long long* __restrict__ S) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int st = state[i];
+  int best = (int)i;
+  if (st == 0) {
+    double bz = dem[i];
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      int n = nbr[(int64_t)p * N + i];
+      if (n < 0 || state[n] < 0) continue;
+      double zn = dem[n];
+      if (zn < bz || (zn == bz && n < best)) { bz = zn; best = n; }
+    }
+  }
+  root[i] = best;
+  S[i] = (st == 1) ? KEY_NEG_INF : KEY_POS_INF;
+}
+
+// In-place pointer jumping; a few hops per launch.
+__global__ void k_jump(int64_t N, int32_t* __restrict__ root, int32_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = root[i];
+  int r0 = r;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    int rr = root[r];
+    if (rr == r) break;
+    r = rr;
+  }
+  if (r != r0) {
+    root[i] = r;
+    if (root[r] != r) *flag = 1;
+  }
+}
+
+__global__ void k_jump_check(int64_t N, const int32_t* __restrict__ root, int32_t* __restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r = root[i];
+  if (root[r] != r) *flag = 1;
+}
+
+__global__ void k_fill_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ deg, const double* __restrict__ dem,
+                             const int32_t* __restrict__ state, const int32_t* __restrict__ root,
+                             unsigned long long* __restrict__ S, int32_t* __restrict__ flag) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= N) return;
+  if (state[u] < 0) return;
+  const int ru = root[u];
+  const double zu = dem[u];
+  unsigned long long cur = S[ru];
+  if (cur == KEY_NEG_INF) return;
+  unsigned long long best = cur;
+  const int dg = deg[u];
+  for (int p = 0; p < W && p < dg; ++p) {
+    int v = nbr[(int64_t)p * N + u];
+    if (v < 0 || state[v] < 0) continue;
+    int rv = root[v];
+    if (rv == ru) continue;
+    unsigned long long sv = S[rv];
+    if (sv == KEY_POS_INF) continue;
+    double w = dem[v] > zu ? dem[v] : zu;
+    unsigned long long kw = dkey(w);
+    unsigned long long c = kw > sv ? kw : sv;
+    if (c < best) best = c;
+  }
+  if (best < cur) {
+    unsigned long long old = atomicMin(&S[ru], best);
+    if (best < old) *flag = 1;
+  }
+}
+
+__global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
+                             const int32_t* __restrict__ state, const int32_t* __restrict__ root,
+                             const unsigned long long* __restrict__ S, double level, double sl,
+                             double* __restrict__ lfill) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double z = dem[i];
+  double f = z;
+  if (state[i] >= 0) {
+    unsigned long long s = S[root[i]];
+    if (s != KEY_POS_INF && z > 0.0) {
+      double sv = dkey_inv(s);
+      f = sv > z ? sv : z;
+    }
+  }
+  if (f < sl - level) f = z;
+  lfill[i] = f + level;
+}
+
+// ---------------------------------------------------------------------------
+// depression information
+// ---------------------------------------------------------------------------
+
+__global__ void k_pit_init(int64_t N, int W, const int32_t* __restrict__ nbr,
+                           const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                           double lvl, int32_t* __restrict__ parent, int8_t* __restrict__ noflow,
+                           int32_t* __restrict__ cmin, int32_t* __restrict__ cnon) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double f = fill[i];
+  cmin[i] = INT_MAX;
+  cnon[i] = 0;
+  if (!(f > lvl)) {
+    parent[i] = -1;
+    noflow[i] = 0;
+    return;
+  }
+  parent[i] = (int)i;
+  bool nf = true;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    int n = nbr[(int64_t)p * N + i];
+    if (n >= 0 && f > fill[n]) { nf = false; break; }
+  }
+  noflow[i] = nf ? 1 : 0;
+}
+
+__device__ __forceinline__ int uf_find(int32_t* parent, int x) {
+  int p = parent[x];
+  while (p != x) {
+    int gp = parent[p];
+    if (gp != p) parent[x] = gp;  // path halving (benign race: moves toward the root)
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__global__ void k_cc_hook(int64_t N, int W, const int32_t* __restrict__ nbr,
+                          const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                          int32_t* parent) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (parent[i] < 0) return;
+  const double f = fill[i];
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    int j = nbr[(int64_t)p * N + i];
+    if (j <= (int)i || fill[j] != f) continue;
+    int a = (int)i, b = j;
+    for (;;) {
+      a = uf_find(parent, a);
+      b = uf_find(parent, b);
+      if (a == b) break;
+      int hi = a > b ? a : b, lo = a > b ? b : a;
+      int old = atomicCAS(&parent[hi], hi, lo);
+      if (old == hi) break;
+      a = lo;
+      b = old;
+    }
+  }
+}
+
+__global__ void k_cc_compress(int64_t N, int32_t* parent, const int8_t* __restrict__ noflow,
+                              const int32_t* __restrict__ gid, const int8_t* __restrict__ outlet,
+                              int flat, int32_t* __restrict__ cmin, int32_t* __restrict__ cnon) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (parent[i] < 0) return;
+  int r = uf_find(parent, (int)i);
+  parent[i] = r;
+  if (noflow[i]) atomicMin(&cmin[r], gid[i]);
+  if (!(flat && outlet[i])) cnon[r] = 1;
+}
+
+__global__ void k_pit_mark(int64_t N, const int32_t* __restrict__ parent,
+                           const int32_t* __restrict__ cmin, const int32_t* __restrict__ cnon,
+                           int32_t* __restrict__ mark) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (parent[i] == (int)i && cmin[i] != INT_MAX && cnon[i]) mark[cmin[i]] = 1;
+}
+
+__global__ void k_pit_ids(int64_t N, const int32_t* __restrict__ parent,
+                          const int32_t* __restrict__ cmin, const int32_t* __restrict__ cnon,
+                          const int32_t* __restrict__ rank, const int8_t* __restrict__ outlet,
+                          int flat, int32_t* __restrict__ pitids) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int id = -1;
+  int r = parent[i];
+  if (r >= 0 && cmin[r] != INT_MAX && cnon[r] && !(flat && outlet[i])) id = rank[cmin[r]] + 1;
+  pitids[i] = id;
+}
+
+// spill candidates (pitfilling.py:741-801)
+__global__ void k_spill_best(int64_t N, int W, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                             const int32_t* __restrict__ pitids, const int8_t* __restrict__ outlet,
+                             int flat, const int32_t* __restrict__ gid, int32_t* __restrict__ best,
+                             int8_t* __restrict__ qual) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int8_t q = 0;
+  const int pid = pitids[i];
+  if (pid > -1) {
+    const double f = fill[i];
+    bool lower = false, eq = false;
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      int n = nbr[(int64_t)p * N + i];
+      if (n < 0) continue;
+      double fn = fill[n];
+      if (fn < f) lower = true;
+      else if (fn == f && ((flat && outlet[n]) || pitids[n] == -1)) eq = true;
+    }
+    if (lower) q = 2;
+    else if (eq) q = 1;
+    if (q) atomicMin(&best[pid], gid[i]);
+  }
+  qual[i] = q;
+}
+
+__global__ void k_spill_set(int64_t N, int W, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                            const int32_t* __restrict__ pitids, const int8_t* __restrict__ outlet,
+                            int flat, const int32_t* __restrict__ gid,
+                            const int32_t* __restrict__ best, const int8_t* __restrict__ qual,
+                            int32_t* __restrict__ spill, int8_t* __restrict__ lspill) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int8_t q = qual[i];
+  if (!q) return;
+  const int pid = pitids[i];
+  if (best[pid] != gid[i]) return;
+  if (q == 2) {
+    spill[pid] = (int)i;
+    lspill[i] = 1;
+  } else {
+    const double f = fill[i];
+    int s = -1, sg = INT_MAX;
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      int n = nbr[(int64_t)p * N + i];
+      if (n < 0) continue;
+      if (fill[n] == f && ((flat && outlet[n]) || pitids[n] == -1) && gid[n] < sg) {
+        sg = gid[n];
+        s = n;
+      }
+    }
+    spill[pid] = s;
+    lspill[s] = 1;
+  }
+}
+
+__global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* __restrict__ pd) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) pd[i] = ls[i] ? 1 : -1;
+}
+
+// Breadth-first distance from the spill cells over pit cells along
+// not-descending edges (fill_dir + nghb_dir, functions.F90:1845-1969).
+__global__ void k_ptdir_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
+                              const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                              const int32_t* __restrict__ pitids, int32_t* ptdir,
+                              int32_t* __restrict__ flag) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  if (pitids[c] < 0) return;
+  const int cur = ptdir[c];
+  if (cur == 1) return;
+  const double fc = fill[c];
+  int best = cur;
+  const int dg = deg[c];
+  for (int p = 0; p < W && p < dg; ++p) {
+    int i = nbr[(int64_t)p * N + c];
+    if (i < 0) continue;
+    int di = ptdir[i];
+    if (di > -1 && fc >= fill[i]) {
+      int cand = di + 1;
+      if (best < 0 || cand < best) best = cand;
+    }
+  }
+  if (best != cur) {
+    ptdir[c] = best;
+    *flag = 1;
+  }
+}
+
+// Flat receivers (pitfilling.py:479-517); spill cells and non-pit cells -1.
+__global__ void k_flatdirs(int64_t N, int W, const int32_t* __restrict__ nbr,
+                           const int32_t* __restrict__ deg, const double* __restrict__ fill,
+                           const int32_t* __restrict__ pitids, const int32_t* __restrict__ ptdir,
+                           const int32_t* __restrict__ gid, const int8_t* __restrict__ lspill,
+                           int32_t* __restrict__ flatdirs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int out = -1;
+  if (pitids[i] > -1) {
+    const int di = ptdir[i];
+    const double fi = fill[i];
+    long long bestk = LLONG_MAX;
+    int bestn = (int)i;
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      int c = nbr[(int64_t)p * N + i];
+      if (c < 0) continue;
+      int dc = ptdir[c];
+      if (dc > -1 && dc < di && fill[c] <= fi) {
+        long long k = ((long long)dc << 32) | (unsigned int)gid[c];
+        if (k < bestk) { bestk = k; bestn = c; }
+      }
+    }
+    out = bestn;
+  }
+  if (lspill[i]) out = -1;
+  flatdirs[i] = out;
+}
+
+// Marine restore + pit parameters (pitfilling.py:822-832, 521-555).
+__global__ void k_pit_params(int64_t N, const double* __restrict__ hl, double* __restrict__ fill,
+                             int32_t* __restrict__ flatdirs, const int32_t* __restrict__ pitids,
+                             const double* __restrict__ area, double sl, int64_t cap,
+                             double* __restrict__ vol, unsigned long long* __restrict__ hmaxk,
+                             unsigned long long* __restrict__ dhk) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double f = fill[i];
+  const double h = hl[i];
+  if (f <= sl) {
+    f = h;
+    fill[i] = f;
+    flatdirs[i] = -1;
+  }
+  const int p = pitids[i];
+  if (p < 0 || p >= cap) return;
+  const double hc = h < sl ? sl : h;
+  atomicAdd(&vol[p], (f - hc) * area[i]);
+  atomicMax(&hmaxk[p], dkey(f));
+  atomicMax(&dhk[p], dkey(f - hc));
+}
+
+// filled_lvl rows (pitfilling.py:877-881); converts the max keys.
+__global__ void k_pit_levels(int64_t npit, int64_t cap, double* __restrict__ params,
+                             const unsigned long long* __restrict__ hmaxk,
+                             const unsigned long long* __restrict__ dhk,
+                             double* __restrict__ lvl) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npit) return;
+  double hmax = (hmaxk[p] == 0ull) ? -1.0e8 : dkey_inv(hmaxk[p]);
+  double dh = (dhk[p] == 0ull) ? 0.0 : dkey_inv(dhk[p]);
+  params[cap + p] = hmax;
+  params[2 * cap + p] = dh;
+  double c[6] = {0, 0, 0, 0, 0, 0};
+  if (params[p] > 0.0) {
+    c[0] = hmax - dh;
+    double d5 = dh / 5.0;
+    for (int k = 1; k < 6; ++k) c[k] = c[k - 1] + d5;
+  }
+  for (int k = 0; k < 5; ++k) lvl[k * cap + p] = c[k + 1];
+}
+
+// getpitvol (functions.F90:1971-2003) on h clipped at sea level.
+__global__ void k_pit_vol(int64_t N, const double* __restrict__ hl, const int32_t* __restrict__ pitids,
+                          const double* __restrict__ area, double sl, int64_t cap,
+                          const double* __restrict__ lvl, double* __restrict__ fvol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int p = pitids[i];
+  if (p < 0 || p >= cap) return;
+  double e = hl[i];
+  if (e < sl) e = sl;
+  for (int k = 0; k < 4; ++k) {
+    double l = lvl[k * cap + p];
+    if (l > e) atomicAdd(&fvol[k * cap + p], (l - e) * area[i]);
+  }
+}
+
+__global__ void k_pit_vol_last(int64_t npit, int64_t cap, const double* __restrict__ params,
+                               double* __restrict__ fvol) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npit) return;
+  fvol[4 * cap + p] = params[p];
+}
+
+// host helper: run `launch` until the device flag stays 0
+template <typename F>
+static int until_stable(gospl_mesh* m, hipStream_t s, int max_iter, int check_every, F launch,
+                        int* iters) {
+  int it = 0;
+  for (;;) {
+    GOSPL_CHECK(hipMemsetAsync(m->dflag + 8, 0, sizeof(int32_t), s));
+    for (int k = 0; k < check_every; ++k) launch(m->dflag + 8);
+    it += check_every;
+    GOSPL_CHECK(hipGetLastError());
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 8, m->dflag + 8, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    if (m->hflag[8] == 0) break;
+    if (it >= max_iter) {
+      set_error("iteration did not converge within %d passes", max_iter);
+      return 1;
+    }
+  }
+  if (iters) *iters = it;
+  return 0;
+}
+
+}  // namespace gospl
+
+using namespace gospl;
+
+extern "C" {
+
+int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, double sealevel,
+                           double* lfill, gospl_stream_t s) {
+  const int64_t N = m->N;
+  double* dem = m->wd[0];
+  int32_t* state = m->wi[3];
+  int32_t* root = m->wi[4];
+  unsigned long long* S = m->wu[0];
+  const int g = nblocks(N);
+  k_fill_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->outlet, m->flat, h, level, dem,
+                                         state);
+  k_fill_descent<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S);
+  GOSPL_CHECK(hipGetLastError());
+  int it = 0;
+  if (until_stable(m, s, (int)N + 16, 3, [&](int32_t* f) {
+        k_jump<<<g, GOSPL_BLOCK, 0, s>>>(N, root, f);
+      }, &it))
+    return 1;
+  if (until_stable(m, s, (int)N + 16, 2, [&](int32_t* f) {
+        k_fill_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, f);
+      }, &it))
+    return 1;
+  k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, sealevel, lfill);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double sealevel,
+                   int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill, int64_t cap,
+                   int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
+                   int64_t* npits_out, gospl_stream_t s) {
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  int32_t* parent = m->wi[3];
+  int32_t* cmin = m->wi[4];
+  int32_t* cnon = m->wi[5];
+  int32_t* mark = m->wi[6];
+  int32_t* rank = m->wi[7];
+  int8_t* noflow = reinterpret_cast<int8_t*>(m->wi[0]);
+  int8_t* qual = reinterpret_cast<int8_t*>(m->wi[1]);
+  int32_t* best = m->wi[2];
+
+  k_pit_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, sealevel, parent, noflow, cmin,
+                                        cnon);
+  k_cc_hook<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, parent);
+  k_cc_compress<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, noflow, m->gid, m->outlet, m->flat, cmin, cnon);
+  GOSPL_CHECK(hipMemsetAsync(mark, 0, sizeof(int32_t) * N, s));
+  k_pit_mark<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, mark);
+  GOSPL_CHECK(hipGetLastError());
+  size_t bytes = m->cub_tmp_bytes;
+  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, mark, rank, (int)N, s));
+  // npits = rank[N-1] + mark[N-1]
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 16, rank + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 17, mark + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  k_pit_ids<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, rank, m->outlet, m->flat, pitids);
+  GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipStreamSynchronize(s));
+  const int64_t npits = (int64_t)m->hflag[16] + m->hflag[17];
+  *npits_out = npits;
+  const int64_t np1 = npits + 1;
+  if (np1 > cap) {
+    set_error("gospl_pit_info: %lld pits exceed the per-pit capacity %lld", (long long)npits,
+              (long long)cap);
+    return 1;
+  }
+  // spill points
+  GOSPL_CHECK(hipMemsetAsync(best, 0x7f, sizeof(int32_t) * np1, s));
+  GOSPL_CHECK(hipMemsetAsync(spill, 0xff, sizeof(int32_t) * np1, s));
+  GOSPL_CHECK(hipMemsetAsync(lspill, 0, N, s));
+  k_spill_best<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, m->outlet, m->flat,
+                                          m->gid, best, qual);
+  k_spill_set<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, m->outlet, m->flat,
+                                         m->gid, best, qual, spill, lspill);
+  GOSPL_CHECK(hipGetLastError());
+  // flat routing distances
+  k_ptdir_init<<<g, GOSPL_BLOCK, 0, s>>>(N, lspill, ptdir);
+  GOSPL_CHECK(hipGetLastError());
+  if (npits > 0) {
+    int it = 0;
+    if (until_stable(m, s, (int)N + 16, 4, [&](int32_t* f) {
+          k_ptdir_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, ptdir, f);
+        }, &it))
+      return 1;
+  }
+  k_flatdirs<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, ptdir, m->gid, lspill,
+                                        flatdirs);
+  // pit parameters
+  unsigned long long* hk = m->wu[0];
+  unsigned long long* dk = m->wu[1];
+  GOSPL_CHECK(hipMemsetAsync(pitparams, 0, sizeof(double) * cap * 3, s));
+  GOSPL_CHECK(hipMemsetAsync(hk, 0, sizeof(unsigned long long) * np1, s));
+  GOSPL_CHECK(hipMemsetAsync(dk, 0, sizeof(unsigned long long) * np1, s));
+  GOSPL_CHECK(hipMemsetAsync(filled_lvl, 0, sizeof(double) * cap * 5, s));
+  GOSPL_CHECK(hipMemsetAsync(filled_vol, 0, sizeof(double) * cap * 5, s));
+  k_pit_params<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, lfill, flatdirs, pitids, m->area, sealevel, cap,
+                                          pitparams, hk, dk);
+  k_pit_levels<<<nblocks(np1), GOSPL_BLOCK, 0, s>>>(np1, cap, pitparams, hk, dk, filled_lvl);
+  k_pit_vol<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, pitids, m->area, sealevel, cap, filled_lvl, filled_vol);
+  k_pit_vol_last<<<nblocks(np1), GOSPL_BLOCK, 0, s>>>(np1, cap, pitparams, filled_vol);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// Linear hillslope diffusion (gospl/sed/hillslope.py:171-233, 614-649).
+//
+// Operator rows from `sethillslopecoeff` (functions.F90:901-938):
+// v_kn = (0.5 * (Kd_k + Kd_n) / A_k) * vDist_kn / eLgt_kn, diag = 1 + sum v,
+// off = -v; flat-mesh outlets are identity rows (hillslope.py:253-255).
+// M h' = h is solved by Jacobi sweeps: M is strictly diagonally dominant with
+// contraction factor max_k sum v / (1 + sum v) (~1e-5 at the benchmark
+// parameters), so a handful of sweeps reach a fixed point in FP64.  Rows are
+// rebuilt every call from the sea mask of the current elevation.
+#include "common.h"
+
+namespace gospl {
+
+__global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const double* __restrict__ area,
+                            const double* __restrict__ elgt, const double* __restrict__ vdist,
+                            const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
+                            double sl, double cda, double cdm, double dt, double* __restrict__ coef,
+                            double* __restrict__ diag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double a = area[i];
+  const bool ident = flat && outlet[i];
+  const double ck = (h[i] <= sl ? cdm : cda) * dt;
+  double s1 = 0.0;
+  const int dg = deg[i];
+  for (int p = 0; p < W; ++p) {
+    double c_out = 0.0;
+    if (!ident && a > 0.0 && p < dg) {
+      const double vd = vdist[(int64_t)p * N + i];
+      if (vd > 0.0) {
+        const int n = nbr[(int64_t)p * N + i];
+        const double cn = (h[n] <= sl ? cdm : cda) * dt;
+        const double c = 0.5 * (ck + cn) / a;
+        const double v = c * vd / elgt[(int64_t)p * N + i];
+        s1 = s1 + v;
+        c_out = -v;
+      }
+    }
+    coef[(int64_t)p * N + i] = c_out;
+  }
+  diag[i] = ident ? 1.0 : (a > 0.0 ? 1.0 + s1 : 0.0);
+}
+
+__global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ deg, const double* __restrict__ coef,
+                             const double* __restrict__ diag, const double* __restrict__ rhs,
+                             const double* __restrict__ xin, double* __restrict__ xout) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double d = diag[i];
+  const double r = rhs[i];
+  if (d == 0.0) { xout[i] = r; return; }
+  double s = r;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    const double c = coef[(int64_t)p * N + i];
+    if (c != 0.0) s = s - c * xin[nbr[(int64_t)p * N + i]];
+  }
+  xout[i] = s / d;
+}
+
+__global__ void k_hill_update(int64_t N, const double* __restrict__ x, double dt, double* __restrict__ h,
+                              double* __restrict__ cumed, double* __restrict__ eblocal) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double t = x[i] - h[i];
+  h[i] = x[i];
+  cumed[i] = cumed[i] + t;
+  eblocal[i] = eblocal[i] + t / dt;
+}
+
+}  // namespace gospl
+
+using namespace gospl;
+
+extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
+                               int32_t sweeps, double* h, double* cumed, double* eblocal,
+                               gospl_stream_t s) {
+
+  if (gospl_ensure_coef(m, m->W)) return 1;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  double* coef = m->wcoef;
+  double* diag = m->wd[3];
+  double* xa = m->wd[1];
+  double* xb = m->wd[2];
+  k_hill_coef<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, m->outlet,
+                                         m->flat, h, sealevel, cda, cdm, dt, coef, diag);
+  if (sweeps < 1) sweeps = 1;
+  const double* xin = h;
+  double* xout = xa;
+  for (int k = 0; k < sweeps; ++k) {
+    k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout);
+    xin = xout;
+    xout = (xout == xa) ? xb : xa;
+  }
+  k_hill_update<<<g, GOSPL_BLOCK, 0, s>>>(N, xin, dt, h, cumed, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}

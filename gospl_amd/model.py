@@ -1,0 +1,377 @@
+"""goSPL's hot-path driver on one MI355X.
+
+``Model`` keeps the method names and state attributes of goSPL's mixin
+``Model`` (gospl/model.py:179-466) for the per-step surface-process path:
+
+* ``flowAccumulation``      gospl/flow/flowplex.py:752-945
+* ``fillElevation``         gospl/flow/pitfilling.py:849-903
+* ``_buildFlowDirection``   gospl/flow/flowplex.py:480-544
+* ``_distributeDownstream`` gospl/flow/flowplex.py:576-686
+* ``erodepSPL``             gospl/eroder/SPL.py:402-454
+* ``getHillslope``          gospl/sed/hillslope.py:614-649
+* ``applyForces``           gospl/mesher/unstructuredmesh.py:743-794
+* ``runProcesses``          gospl/model.py:331-466
+
+State lives in torch tensors on the GPU in device (Morton) order; the
+compute is the HIP library behind ``gospl_amd._lib`` (no CPU fallback: a
+missing library or GPU is an error).  Per-pit bookkeeping of the overflow
+cascade (a few thousand numbers) runs on the host as in goSPL.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mesh import Mesh
+
+MISSING_DATA_SENTINEL = -1.0e8   # gospl/tools/constants.py:34
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class DeviceMesh:
+    """A ``Mesh`` uploaded to the GPU plus its library handle."""
+
+    def __init__(self, mesh: Mesh, device="cuda"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("gospl_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.mesh = mesh
+        self.device = torch.device(device)
+        self.N = mesh.N
+        self.W = mesh.width
+        self.flat = mesh.flat
+        dev = self.device
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(dev)
+        self.nbr = t(mesh.nbr, torch.int32)
+        self.deg = t(mesh.deg, torch.int32)
+        self.elgt = t(mesh.elgt, torch.float64)
+        self.vdist = t(mesh.vdist, torch.float64)
+        self.area = t(mesh.area, torch.float64)
+        self.gid = t(mesh.gid, torch.int32)
+        self.outlet = t(mesh.outlet, torch.int8)
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.gospl_mesh_create(
+            self.N, self.W, 1 if self.flat else 0, _ptr(self.nbr), _ptr(self.deg), _ptr(self.elgt),
+            _ptr(self.vdist), _ptr(self.area), _ptr(self.gid), _ptr(self.outlet), ctypes.byref(h)),
+            "gospl_mesh_create")
+        self.handle = h
+        # contraction bound of the hillslope operator: g_i = sum_p vdist/elgt / area_i
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ratio = np.where(mesh.elgt > 0, mesh.vdist / np.where(mesh.elgt > 0, mesh.elgt, 1.0), 0.0)
+            g = np.where(mesh.area > 0, ratio.sum(axis=0) / np.where(mesh.area > 0, mesh.area, 1.0), 0.0)
+        self.gmax = float(np.max(g)) if len(g) else 0.0
+        self.outletIDs = torch.nonzero(self.outlet > 0).flatten() if self.flat else None
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.gospl_mesh_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """Per-step hot path of goSPL on one GPU (FP64 state).
+
+    ``mesh`` is a ``gospl_amd.mesh.Mesh`` (or ``DeviceMesh``), ``h0`` the
+    initial elevation in device order; keyword parameters follow goSPL's YAML
+    keys: ``K, m, d`` (spl), ``flowdir``, ``cda/cdm`` (hillslopeKa/Km),
+    ``dt``, ``rain`` (scalar or per-vertex m/yr), ``upsub`` (per-vertex m/yr
+    or None), ``sedfac`` (per-vertex K factor or None), ``sealevel``,
+    ``ofill`` (oFill), ``tstart``/``tend``.
+    """
+
+    def __init__(self, mesh, h0, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
+                 rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
+                 tend=None, device="cuda", verbose=False):
+        self.dm = mesh if isinstance(mesh, DeviceMesh) else DeviceMesh(mesh, device)
+        self.lib = self.dm.lib
+        self.H = self.dm.handle
+        dev = self.dm.device
+        self.device = dev
+        N = self.dm.N
+        self.lpoints = N
+        self.flatModel = self.dm.flat
+        self.K, self.spl_m, self.coeffd = float(K), float(m), float(d)
+        self.flowDir, self.flowExp = int(flowdir), 1.1
+        self.Cda, self.Cdm = float(cda), float(cdm)
+        self.dt = float(dt)
+        self.sealevel = float(sealevel)
+        self.oFill = float(ofill)
+        self.maxarea = np.array([float(self.dm.mesh.maxarea)])
+        self.verbose = verbose
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        i8 = dict(dtype=torch.int8, device=dev)
+        vec = lambda v: torch.as_tensor(np.broadcast_to(np.asarray(v, np.float64), (N,)).copy()).to(dev)
+        self.rainVal = torch.clamp(vec(rain), min=0.0)
+        self.upsub = None if upsub is None else vec(upsub)
+        self.sedfacVal = None if sedfac is None else vec(sedfac)
+        self._kfac = self._make_kfac()
+        self.hLocal = torch.as_tensor(np.asarray(h0, np.float64)).to(dev).contiguous()
+        self.cumED = torch.zeros(N, **f64)
+        self.EbLocal = torch.zeros(N, **f64)
+        self.Eb = torch.zeros(N, **f64)
+        self.FAL = torch.zeros(N, **f64)
+        self.fillFAL = torch.zeros(N, **f64)
+        self.bL = self.rainVal * self.dm.area
+        self.lFill = torch.zeros(N, **f64)
+        self.waterFilled = torch.zeros(N, **f64)
+        self.pitIDs = torch.full((N,), -1, **i32)
+        self.flatDirs = torch.full((N,), -1, **i32)
+        self.ptdir = torch.full((N,), -1, **i32)
+        self.lspill = torch.zeros(N, **i8)
+        K_ = self.flowDir
+        self.rcvID = torch.zeros((K_, N), **i32)
+        self.distRcv = torch.zeros((K_, N), **f64)
+        self.wghtVal = torch.zeros((K_, N), **f64)
+        self.lsink = torch.zeros(N, **i8)
+        self._b = torch.zeros(N, **f64)
+        self._q = torch.zeros(N, **f64)
+        self._nfa = torch.zeros(N, **f64)
+        cap = N + 1
+        self._cap = cap
+        self._spill = torch.zeros(cap, **i32)
+        self._pitparams = torch.zeros((3, cap), **f64)
+        self._flvl = torch.zeros((5, cap), **f64)
+        self._fvol = torch.zeros((5, cap), **f64)
+        self._inv = torch.zeros(cap, **f64)
+        self._full = torch.zeros(cap, **i8)
+        self._level = torch.zeros(cap, **f64)
+        self.tStart = float(tstart)
+        self.tNow = float(tstart)
+        self.tEnd = float(tend) if tend is not None else float(tstart)
+        self._cascade_max_steps = 100
+        self._cascade_patience = 3
+        self._cascade_rel_improve = 1.0e-3
+        self._cascade_resid = 0.0
+        self.cascade_passes = 0
+        self.fa_levels = 0
+        self.spl_levels = 0
+        self.timers = {}
+
+    # -------------------------------------------------------------- helpers
+    @property
+    def stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _call(self, name, *args):
+        _lib.check(getattr(self.lib, name)(*args), name)
+
+    def _make_kfac(self):
+        if self.sedfacVal is None and self.coeffd == 0.0:
+            return None
+        k = torch.ones_like(self.rainVal)
+        if self.sedfacVal is not None:
+            k = k * torch.clamp(self.sedfacVal, min=0.1)
+        return k * self.rainVal ** self.coeffd
+
+    def hillslope_sweeps(self):
+        kd = max(self.Cda, self.Cdm) * self.dt
+        s = kd * self.dm.gmax
+        if s <= 0.0:
+            return 1
+        rho = s / (1.0 + s)
+        n = int(math.ceil(math.log(2.0 ** -60) / math.log(rho))) + 1
+        return max(2, min(n, 500))
+
+    # -------------------------------------------------------------- pits
+    def fillElevation(self, sed=False):
+        """pitfilling.py:849-903 (water filling, sed=False)."""
+        minh = float(self.hLocal.min().item())
+        if not self.flatModel:
+            minh += 1.0e-3
+        level = max(minh, self.sealevel + self.oFill)
+        self.level = level
+        S = self.stream
+        self._call("gospl_fill_depressions", self.H, _ptr(self.hLocal), level, self.sealevel,
+                   _ptr(self.lFill), S)
+        npits = ctypes.c_int64(0)
+        self._call("gospl_pit_info", self.H, _ptr(self.hLocal), _ptr(self.lFill), self.sealevel,
+                   _ptr(self.pitIDs), _ptr(self.flatDirs), _ptr(self.ptdir), _ptr(self.lspill),
+                   self._cap, _ptr(self._spill), _ptr(self._pitparams), _ptr(self._flvl),
+                   _ptr(self._fvol), ctypes.byref(npits), S)
+        n1 = int(npits.value) + 1
+        self.npits = n1 - 1
+        self.pitParams = self._pitparams[:, :n1].T.cpu().numpy().copy()
+        spill = self._spill[:n1].cpu().numpy().astype(np.int64)
+        self.pitInfo = np.zeros((n1, 2), dtype=np.int64)
+        self.pitInfo[:, 0] = spill
+        self.pitInfo[:, 1] = np.where(spill >= 0, 0, -1)
+        self.filled_lvl = self._flvl[:, :n1].T.cpu().numpy().copy()
+        self.filled_vol = self._fvol[:, :n1].T.cpu().numpy().copy()
+
+    # -------------------------------------------------------------- flow
+    def _buildFlowDirection(self, h, down=True):
+        self._call("gospl_flow_direction", self.H, self.flowDir, self.flowExp, _ptr(h), self.sealevel,
+                   _ptr(self.lFill), 1 if down else 0, _ptr(self.pitIDs), _ptr(self.flatDirs),
+                   _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
+                   self.stream)
+
+    def _solve_flow(self, b, out):
+        lev = ctypes.c_int32(0)
+        bad = ctypes.c_int64(0)
+        self._call("gospl_flowacc", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal), _ptr(b),
+                   _ptr(out), ctypes.byref(lev), ctypes.byref(bad), self.stream)
+        if bad.value:
+            raise RuntimeError("flow receivers hold a cycle (%d vertices unsolved)" % bad.value)
+        return int(lev.value)
+
+    def _distributeDownstream(self, pitVol, FA, hl, step):
+        excess = False
+        npit = len(self.pitParams)
+        S = self.stream
+        self._call("gospl_pit_inflow", self.H, _ptr(FA), _ptr(self.lsink), _ptr(self.pitIDs), npit,
+                   _ptr(self._inv), S)
+        inV = self._inv[:npit].cpu().numpy()
+        eV = inV - pitVol
+        full = np.zeros(npit, np.int8)
+        level = np.full(npit, np.nan)
+        have_nfa = False
+        if (eV > 0.0).any():
+            eIDs = eV > 0.0
+            pitVol[eIDs] = 0.0
+            full[eIDs] = 1
+            spillIDs = self.pitInfo[eIDs, 0]
+            localSpill = np.where(self.pitInfo[eIDs, 1] == 0)[0]
+            localPts = spillIDs[localSpill]
+            vals = eV[eIDs][localSpill]
+            self._nfa.zero_()
+            if len(localPts):
+                # numpy fancy assignment: a repeated index keeps the last value
+                u, last = np.unique(localPts[::-1], return_index=True)
+                vv = vals[::-1][last]
+                self._nfa[torch.as_tensor(u, device=self.device)] = torch.as_tensor(vv, device=self.device)
+            have_nfa = True
+        if (eV < 0.0).any():
+            eIDs = np.where((eV < 0.0) & (inV > 0.0))[0]
+            pitVol[eIDs] += eV[eIDs]
+            if len(eIDs):
+                nid = np.absolute(self.filled_vol[eIDs] - pitVol[eIDs][:, None]).argmin(axis=1)
+                level[eIDs] = self.filled_lvl[eIDs, nid]
+        if full.any() or np.isfinite(level).any():
+            self._full[:npit] = torch.as_tensor(full, device=self.device)
+            self._level[:npit] = torch.as_tensor(level, device=self.device)
+            self._call("gospl_pit_water", self.H, _ptr(self.pitIDs), _ptr(self._full), _ptr(self._level),
+                       _ptr(self.lFill), _ptr(self.waterFilled), S)
+        nFA = None
+        if (eV > 1.0e-3).any():
+            if step == 100:
+                self._buildFlowDirection(self.lFill)
+            else:
+                self._buildFlowDirection(self.waterFilled)
+            rhs = self._nfa / self.dt if have_nfa else torch.zeros_like(self._nfa)
+            resid = float(rhs.sum().item())
+            self._cascade_resid = resid
+            if resid > self.maxarea[0]:
+                excess = True
+                self._solve_flow(rhs, self._q)
+                self._call("gospl_cascade_add", self.H, _ptr(self._q), _ptr(hl), _ptr(self.waterFilled),
+                           self.dt, _ptr(self._nfa), _ptr(self.FAL), S)
+                nFA = self._nfa
+        return excess, pitVol, nFA
+
+    def flowAccumulation(self):
+        t0 = time.perf_counter()
+        self.fillElevation(sed=False)
+        pitVol = self.pitParams[:, 0].copy()
+        hl = self.hLocal.clone()
+        self._buildFlowDirection(hl, False)
+        self.wghtVali = self.wghtVal.clone()
+        self.rcvIDi = self.rcvID.clone()
+        self.distRcvi = self.distRcv.clone()
+        self.lsinki = self.lsink.clone()
+        S = self.stream
+        self._call("gospl_rain_source", self.H, _ptr(self.rainVal), _ptr(self.lFill), self.sealevel,
+                   _ptr(self._b), S)
+        self.fa_levels = self._solve_flow(self._b, self.FAL)
+        self.waterFilled = hl.clone()
+        self.cascade_passes = 0
+        has_pits = bool((pitVol > 0.0).any())
+        if has_pits:
+            FA = self.FAL * self.dt
+            excess = True
+            step = 0
+            best_resid = None
+            stall = 0
+            while excess and step <= self._cascade_max_steps:
+                excess, pitVol, FA = self._distributeDownstream(pitVol, FA, hl, step)
+                step += 1
+                if excess:
+                    resid = self._cascade_resid
+                    if best_resid is None or resid < best_resid * (1.0 - self._cascade_rel_improve):
+                        best_resid = resid
+                        stall = 0
+                    else:
+                        stall += 1
+                        if stall >= self._cascade_patience:
+                            break
+            self.cascade_passes = step
+        self._call("gospl_flow_finalize", self.H, _ptr(hl), _ptr(self.waterFilled), _ptr(self.pitIDs),
+                   1 if has_pits else 0, _ptr(self.FAL), _ptr(self.fillFAL), S)
+        self.waterFilled = self.waterFilled - hl
+        self.timers["flow"] = time.perf_counter() - t0
+
+    # -------------------------------------------------------------- erosion
+    def erodepSPL(self):
+        lev = ctypes.c_int32(0)
+        self._call("gospl_spl", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
+                   _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.dt,
+                   self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),
+                   _ptr(self.Eb), _ptr(self.EbLocal), ctypes.byref(lev), self.stream)
+        self.spl_levels = int(lev.value)
+
+    # -------------------------------------------------------------- hillslope
+    def getHillslope(self):
+        if self.Cda == 0.0 and self.Cdm == 0.0:
+            # goSPL then adds the erosion step's thickness again (hillslope.py:638-643)
+            self.EbLocal += self.Eb * self.dt / self.dt
+            return
+        self._call("gospl_hillslope", self.H, self.Cda, self.Cdm, self.dt, self.sealevel,
+                   self.hillslope_sweeps(), _ptr(self.hLocal), _ptr(self.cumED), _ptr(self.EbLocal),
+                   self.stream)
+
+    # -------------------------------------------------------------- driver
+    def applyForces(self):
+        self.bL = self.rainVal * self.dm.area
+        if self.upsub is not None:
+            self.hLocal += self.upsub * self.dt
+
+    def initialise(self):
+        """model.py:284-295: forcing, then the init-time flow accumulation."""
+        self.applyForces()
+        self.flowAccumulation()
+
+    def step(self):
+        """Body of the runProcesses loop (model.py:379-453) for this path."""
+        self.flowAccumulation()
+        self.erodepSPL()
+        self.getHillslope()
+        if self.tNow < self.tEnd:
+            self.applyForces()
+        self.tNow += self.dt
+
+    def runProcesses(self):
+        while self.tNow <= self.tEnd:
+            if self.tNow == self.tEnd:
+                break
+            self.step()
+
+    # -------------------------------------------------------------- views
+    def input_order(self, t):
+        """Host copy of a per-vertex tensor in input-mesh order."""
+        a = t.detach().cpu().numpy()
+        return self.dm.mesh.to_input_order(a)

@@ -1,0 +1,181 @@
+/*
+ * gospl_hip.h — C ABI of the MI355X (gfx950) implementation of goSPL's
+ * per-timestep surface-process hot path.
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise every array pointer
+ * is a DEVICE pointer owned by the caller, and every per-vertex array is in
+ * the mesh's device (space-filling-curve) order.  Receiver tables are
+ * column-major (nrcv, N): entry k of vertex i is at [k * N + i].
+ * All functions return 0 on success and nonzero on error; the message is
+ * available from gospl_last_error().  Kernels are enqueued on `stream`;
+ * functions that must return a data-dependent count to the host
+ * (gospl_pit_info, gospl_flowacc's level count) synchronise that stream.
+ *
+ * Each declaration names the goSPL interface it replaces.  goSPL reaches
+ * these through the f2py module `gospl._fortran` (fortran/functions.pyf) and
+ * PETSc KSP solves inside its Python mixins.
+ */
+#ifndef GOSPL_HIP_H
+#define GOSPL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gospl_mesh gospl_mesh_t;
+typedef struct ihipStream_t* gospl_stream_t; /* a hipStream_t (NULL = default stream) */
+
+/* Message of the last failing call on this thread. */
+const char* gospl_last_error(void);
+
+/*
+ * Host function.  FV geometry of a triangulation: replaces `definetin`
+ * (fortran/functions.pyf:297-308, functions.F90:2715-2940) as called from
+ * UnstMesh._meshStructure (gospl/mesher/unstructuredmesh.py:150-160).
+ * HOST arrays: coords (nb,3), cells_nodes (n,3), cells_edges (n,3),
+ * edges_nodes (m,2), circumcenter (n,3), all row-major; outputs ngbID,
+ * eLgt, vDist (nb,12) row-major with -1 / 0 padding, nnb (nb), area (nb).
+ * curved = 1 for a sphere (setcurvedmesh, functions.pyf:44-46).
+ */
+int gospl_fv_geometry(const double* coords, int64_t nb, const int32_t* cells_nodes,
+                      const int32_t* cells_edges, int64_t n, const int32_t* edges_nodes,
+                      int64_t m, const double* circumcenter, int curved, int32_t* ngbID,
+                      int32_t* nnb, double* eLgt, double* vDist, double* area);
+
+/*
+ * Mesh handle: replaces the Fortran module state `meshparams`
+ * (functions.F90:10-36) that `definetin` fills.  The geometry arrays stay
+ * owned by the caller and must outlive the handle; the handle owns O(N)
+ * device workspace.  nbr/elgt/vdist are (W, N) column-major ELL tables.
+ */
+int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr,
+                      const int32_t* deg, const double* elgt, const double* vdist,
+                      const double* area, const int32_t* gid, const int8_t* outlet,
+                      gospl_mesh_t** out);
+int gospl_mesh_destroy(gospl_mesh_t* mesh);
+
+/*
+ * Multiple-flow-direction receivers: replaces `mfdreceivers`
+ * (functions.pyf:146-156, functions.F90:1544-1671).  elev (N) -> rcv, dist,
+ * wgt (nrcv, N).  nrcv <= 12.
+ */
+int gospl_mfdreceivers(gospl_mesh_t* mesh, int32_t nrcv, double flowexp, const double* elev,
+                       double sealevel, int32_t* rcv, double* dist, double* wgt,
+                       gospl_stream_t stream);
+
+/*
+ * FAMesh._buildFlowDirection (gospl/flow/flowplex.py:480-544): receivers on
+ * h, then (down != 0) flat-depression routing along flatdirs where
+ * h == lfill and the cell has no downslope neighbour, then flat-mesh outlets
+ * as self-receivers, then lsink = no receiver weight & not outlet & not sea
+ * (sea = lfill <= sealevel).  pitids/flatdirs are read only when down != 0.
+ */
+int gospl_flow_direction(gospl_mesh_t* mesh, int32_t nrcv, double flowexp, const double* h,
+                         double sealevel, const double* lfill, int32_t down,
+                         const int32_t* pitids, const int32_t* flatdirs, int32_t* rcv,
+                         double* dist, double* wgt, int8_t* lsink, gospl_stream_t stream);
+
+/*
+ * Depression filling: replaces PITFill._performFilling and the level shift
+ * of fillElevation (gospl/flow/pitfilling.py:557-702, 864-870), i.e. the
+ * Fortran `edge_tile`, `fill_tile`, `graph_nodes`, `combine_edges`,
+ * `fill_edges`, `fill_depressions` (functions.pyf:179-241).
+ * h (N) -> lfill (N) = ((filled h - level) + level) with cells whose fill is
+ * below sealevel reset to (h - level) + level.
+ */
+int gospl_fill_depressions(gospl_mesh_t* mesh, const double* h, double level,
+                           double sealevel, double* lfill, gospl_stream_t stream);
+
+/*
+ * Depression information: replaces PITFill._pitInformation and the
+ * depth-volume curves of fillElevation (gospl/flow/pitfilling.py:704-847,
+ * 872-891), i.e. the Fortran `label_pits`, `pits_cons`, `fill_dir`,
+ * `nghb_dir`, `getpitvol` (functions.pyf:172-177, 243-277) plus the
+ * numpy spill-point, flat-receiver and pit-parameter code.
+ * In: hl (N) unfilled elevation, lfill (N) from gospl_fill_depressions.
+ * Out: lfill (marine cells reset to hl), pitids (N, 1..npits or -1),
+ * flatdirs (N), ptdir (N), lspill (N int8).  Per-pit arrays have capacity
+ * `cap` rows (index 0 unused, pits 1..npits): spill (cap), pitparams
+ * (3, cap) = volume, max fill, max depth; filled_lvl (5, cap) and
+ * filled_vol (5, cap).  *npits (HOST) receives the pit count; the call
+ * fails if npits + 1 > cap.
+ */
+int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double sealevel,
+                   int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill,
+                   int64_t cap, int32_t* spill, double* pitparams, double* filled_lvl,
+                   double* filled_vol, int64_t* npits, gospl_stream_t stream);
+
+/*
+ * Flow accumulation solve: replaces FAMesh._solve_KSP on the flow matrix
+ * (I - W)^T built by matrixFlow (gospl/flow/flowplex.py:323-478, 843) with an
+ * exact sweep over the receiver DAG.  rcv/wgt (nrcv, N), b (N) -> q (N).
+ * *nlevels (HOST, may be NULL) receives the DAG depth; *unsolved (HOST, may be
+ * NULL) the number of vertices left on a cycle (their q is set to 0).
+ */
+int gospl_flowacc(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const double* wgt,
+                  const double* b, double* q, int32_t* nlevels, int64_t* unsolved,
+                  gospl_stream_t stream);
+
+/*
+ * Stream-power erosion: replaces SPL._eroMats + SPL._getEroDepRate and the
+ * state update of SPL.erodepSPL (gospl/eroder/SPL.py:45-114, 260-366,
+ * 402-438) for n = 1 and no deposition (G = 0).
+ * kfac (N, may be NULL) = per-vertex K multiplier (sedfactor * rain^d);
+ * sea = lfill <= sealevel.  Updates h, cumed; writes eb (N) =
+ * erosion/deposition rate with outlets (flat meshes) and lsink cells zeroed,
+ * and eblocal = (eb*dt)/dt.
+ */
+int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+              const double* disti, const double* fal, const double* kfac, double K, double m,
+              double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+              double* cumed, double* eb, double* eblocal, int32_t* nlevels,
+              gospl_stream_t stream);
+
+/*
+ * Linear hillslope diffusion: replaces hillSLP.getHillslope/_hillSlope(0)
+ * (gospl/sed/hillslope.py:171-233, 614-649) with `sethillslopecoeff`
+ * (functions.pyf:10-14, functions.F90:901-938) and its Richardson/ILU(0)
+ * solve.  Diffusivity cda on land, cdm at sea (h <= sealevel), times dt.
+ * Updates h, cumed and eblocal (+= dh/dt).  `sweeps` Jacobi sweeps (the
+ * Python host derives the count from the operator's contraction bound).
+ */
+int gospl_hillslope(gospl_mesh_t* mesh, double cda, double cdm, double dt, double sealevel,
+                    int32_t sweeps, double* h, double* cumed, double* eblocal,
+                    gospl_stream_t stream);
+
+/*
+ * Runoff source of flowAccumulation (gospl/flow/flowplex.py:794-795):
+ * b = rain * area with sea (lfill <= sealevel) zeroed.
+ */
+int gospl_rain_source(gospl_mesh_t* mesh, const double* rain, const double* lfill,
+                      double sealevel, double* b, gospl_stream_t stream);
+
+/*
+ * Pit-overflow cascade pieces (FAMesh._distributeDownstream,
+ * gospl/flow/flowplex.py:576-686):
+ *  - gospl_pit_inflow: inv[p] = sum of fa over lsink cells of pit p (npit rows).
+ *  - gospl_pit_water: per-pit water levels; full[p] != 0 sets water = lfill,
+ *    else if level[p] is not NaN and water <= level[p], water = level[p].
+ *  - gospl_cascade_add: nfa = q*dt; fal += (nfa masked where hl < water)/dt.
+ *  - gospl_flow_finalize (flowplex.py:906-933): zero fal at filled pit
+ *    cells, fillfal = fal with 0.1*max(fal) there (when has_pits), clamp >= 0.
+ */
+int gospl_pit_inflow(gospl_mesh_t* mesh, const double* fa, const int8_t* lsink,
+                     const int32_t* pitids, int64_t npit, double* inv, gospl_stream_t stream);
+int gospl_pit_water(gospl_mesh_t* mesh, const int32_t* pitids, const int8_t* full,
+                    const double* level, const double* lfill, double* water,
+                    gospl_stream_t stream);
+int gospl_cascade_add(gospl_mesh_t* mesh, const double* q, const double* hl,
+                      const double* water, double dt, double* nfa, double* fal,
+                      gospl_stream_t stream);
+int gospl_flow_finalize(gospl_mesh_t* mesh, const double* hl, const double* water,
+                        const int32_t* pitids, int32_t has_pits, double* fal,
+                        double* fillfal, gospl_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOSPL_HIP_H */

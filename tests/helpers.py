@@ -1,0 +1,51 @@
+"""Shared builders for the parity tests: the same seeded workload built once
+for the HIP path (device order) and once for the oracle (input order)."""
+
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from gospl_amd import _lib, mesh as gmesh, workloads
+from oracle import oracle as orc
+
+
+@functools.lru_cache(maxsize=None)
+def case(kind: str, size: int, seed: int = 0):
+    lib = _lib.load()
+    if kind == "planar":
+        m, h0, p = workloads.planar_case(lib, n=size, seed=seed)
+    elif kind == "sphere":
+        m, h0, p = workloads.sphere_case(lib, level=size, seed=seed, lmax=min(64, 4 * 2 ** size))
+    elif kind == "fixture":
+        d = np.load(size)
+        flat = not np.all(d["v"][:, 2] != 0) and np.allclose(d["v"][:, 2], 0)
+        m = gmesh.build_mesh(d["v"], d["c"], lib, flat=flat)
+        h0 = m.to_device_order(d["z"])
+        p = dict(K=4e-6, m=0.5, d=0.0, flowdir=3, cda=0.01, cdm=0.2, dt=100.0, rain=1.0,
+                 upsub=None, sealevel=-100.0)
+    else:
+        raise ValueError(kind)
+    return m, h0, p
+
+
+def oracle_model(m, h0, p, **over):
+    q = dict(p)
+    q.update(over)
+    conv = {k: (m.to_input_order(v) if isinstance(v, np.ndarray) else v) for k, v in q.items()}
+    return orc.OracleModel(workloads.oracle_geometry(m), m.to_input_order(h0), tend=1e12, **conv)
+
+
+def dev_index_to_input(m, idx):
+    """Map device indices (-1 kept) to input ids, then reorder rows to input order."""
+    idx = np.asarray(idx)
+    out = np.where(idx >= 0, m.perm[np.maximum(idx, 0)], -1)
+    return out
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)

@@ -1,0 +1,253 @@
+"""Pins of the FP64 oracle (and of the C-ABI geometry) — CPU only.
+
+1. Golden vectors written by goSPL's own Fortran (tests/golden/*.npz, see
+   make_golden.py): bit-for-bit equality.
+2. The live reference build (oracle/_ref) on random inputs, when present.
+3. Mathematics: brute-force minimax fill, dense solves of the flow, erosion
+   and diffusion systems, mass balance, the 1-D hillslope parabola.
+"""
+
+from __future__ import annotations
+
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from gospl_amd import _lib, mesh as gmesh
+from oracle import oracle as orc, ref
+
+GOLD = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def _load(path):
+    return dict(np.load(path, allow_pickle=False))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _lib.load()
+
+
+@pytest.mark.parametrize("path", GOLD, ids=lambda p: os.path.basename(p))
+def test_geometry_matches_definetin(path, lib):
+    g = _load(path)
+    flat = bool(g["flat"])
+    ngb, nnb, elgt, vdist, area, vi = gmesh.fv_geometry(g["v"], g["c"], flat, lib)
+    assert np.array_equal(ngb, g["ngb"])
+    assert np.array_equal(nnb, g["nnb"])
+    assert np.array_equal(elgt, g["elgt"])
+    assert np.array_equal(vdist, g["vdist"])
+    narea = g["narea"].copy()
+    narea[np.isnan(narea)] = 1.0
+    if not flat:
+        assert np.array_equal(area, narea)
+
+
+@pytest.mark.parametrize("path", GOLD, ids=lambda p: os.path.basename(p))
+def test_oracle_kernels_match_golden(path):
+    g = _load(path)
+    N = len(g["z"])
+    ngb, nnb, elgt = g["ngb"], g["nnb"], g["elgt"]
+    gid = np.arange(N, dtype=np.int32)
+    for t in range(2):
+        r, d, w = orc.mfdreceivers(3, 1.1, g["mfd%d_h" % t], 0.0, gid, ngb, nnb, elgt)
+        assert np.array_equal(r, g["mfd%d_rcv" % t])
+        assert np.array_equal(d, g["mfd%d_dist" % t])
+        assert np.array_equal(w, g["mfd%d_wgt" % t])
+    fill, reached, ledge = orc.fill(g["dem"], g["outlet"], ngb, nnb)
+    assert np.array_equal(fill, g["fill"])
+    lab = orc.label_pits(0.0, fill + g["level"], ngb, nnb)
+    assert np.array_equal(lab, g["labels"])
+    hc = orc.sethillslopecoeff(g["kd"], ngb, nnb, g["area"], elgt, g["vdist"])
+    assert np.array_equal(hc, g["hillcoeff"])
+    pd0 = orc.fill_dir(g["spill"], g["pitids"], g["lfill"], ngb, nnb) if (g["pitids"] > -1).any() \
+        else -np.ones(N, np.int32)
+    assert np.array_equal(pd0, g["ptdir_fill"])
+    pd1, _ = orc.nghb_dir(g["pitids"], g["lfill"], pd0, ngb, nnb)
+    assert np.array_equal(pd1, g["ptdir_relax"])
+    pv = orc.getpitvol(g["pitlvl"], np.maximum(g["z"], 0.0), g["pitids"], np.ones(N, np.int32), g["area"])
+    assert np.array_equal(pv, g["pitvol"])
+
+
+@pytest.mark.skipif(not ref.available(), reason="reference build oracle/_ref absent")
+def test_oracle_matches_live_reference_random(lib):
+    d = _load(GOLD[0]) if GOLD else None
+    rng = np.random.default_rng(7)
+    for path in GOLD:
+        g = _load(path)
+        v, c, flat = g["v"], g["c"], bool(g["flat"])
+        N = len(v)
+        vi = gmesh.voronoi_inputs(v, c, flat)
+        ref.fv_setup(v, c, vi["cells_edges"], vi["edges"], vi["circumcenters"], 0 if flat else 1)
+        gid = rng.permutation(N).astype(np.int32)
+        for trial in range(3):
+            h = rng.uniform(-50, 50, N) * (10 ** trial)
+            if trial == 2:
+                h = np.round(h / 1000.0) * 1000.0
+            a = orc.mfdreceivers(3, 1.1, h, 0.0, gid, g["ngb"], g["nnb"], g["elgt"])
+            b = ref.mfdreceivers(3, 1.1, h, 0.0, gid)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+            kd = rng.uniform(0, 3, N)
+            assert np.array_equal(orc.sethillslopecoeff(kd, g["ngb"], g["nnb"], g["narea"], g["elgt"],
+                                                        g["vdist"]), ref.sethillslopecoeff(kd))
+            lab_o = orc.label_pits(0.0, h, g["ngb"], g["nnb"])
+            assert np.array_equal(lab_o, ref.label_pits(0.0, h))
+    del d
+
+
+# --------------------------------------------------------------------------
+# mathematics
+# --------------------------------------------------------------------------
+
+def _small_plane(n=12, seed=0):
+    v, c = gmesh.planar_jittered(n, n, 100.0, 20.0, seed)
+    lib = _lib.load()
+    ngb, nnb, elgt, vdist, area, _ = gmesh.fv_geometry(v, c, True, lib)
+    bv = gmesh.boundary_vertices(c)
+    out = np.zeros(len(v), np.int8)
+    out[bv] = 1
+    return v, c, ngb, nnb, elgt, vdist, area, out
+
+
+def _minimax_bruteforce(z, seeds, incl, ngb, nnb):
+    """Dijkstra with max as path cost (the definition of the spill elevation)."""
+    import heapq
+
+    N = len(z)
+    best = np.full(N, np.inf)
+    pq = []
+    for s in seeds:
+        best[s] = z[s]
+        heapq.heappush(pq, (z[s], s))
+    while pq:
+        d, i = heapq.heappop(pq)
+        if d > best[i]:
+            continue
+        for k in range(nnb[i]):
+            j = ngb[i, k]
+            if j < 0 or not incl[j]:
+                continue
+            nd = max(d, z[j])
+            if nd < best[j]:
+                best[j] = nd
+                heapq.heappush(pq, (nd, j))
+    return best
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_fill_is_minimax_from_seeds(seed):
+    v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(14, seed)
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(0, 10, len(v))
+    z[out == 1] = rng.uniform(-1, 3, (out == 1).sum())
+    dem = z - 0.5
+    fill, reached, ledge = orc.fill(dem, out, ngb, nnb)
+    seeds = np.nonzero(ledge >= 0)[0]
+    incl = ledge != -2
+    mm = _minimax_bruteforce(dem, seeds, incl, ngb, nnb)
+    expect = np.where((dem > 0) & np.isfinite(mm), mm, dem)
+    assert np.array_equal(fill, expect)
+    assert np.all(fill >= dem)
+
+
+def _dense_flow(rcv, wgt, b):
+    N, K = rcv.shape
+    A = np.eye(N)
+    for i in range(N):
+        for k in range(K):
+            r = rcv[i, k]
+            if r != i and wgt[i, k] != 0:
+                A[i, r] -= wgt[i, k]
+    return np.linalg.solve(A.T, b)
+
+
+def test_flowacc_exact_and_mass_balance():
+    v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(12, 4)
+    rng = np.random.default_rng(4)
+    z = rng.uniform(0, 30, len(v)) + 0.05 * v[:, 0]
+    rcv, dist, wgt = orc.mfdreceivers(3, 1.1, z, -1e9, np.arange(len(v), dtype=np.int32), ngb, nnb, elgt)
+    b = rng.uniform(0.5, 2.0, len(v)) * area
+    q, bad = orc.flowacc(rcv, wgt, b)
+    assert bad == 0
+    qd = _dense_flow(rcv, wgt, b)
+    assert np.abs(q - qd).max() <= 1e-10 * np.abs(qd).max()
+    sinks = wgt.sum(axis=1) == 0
+    assert abs(q[sinks].sum() - b.sum()) <= 1e-10 * b.sum()
+
+
+def test_spl_solve_exact():
+    v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(12, 5)
+    rng = np.random.default_rng(5)
+    N = len(v)
+    z = rng.uniform(0, 30, N) + 0.05 * v[:, 0]
+    rcv, dist, wgt = orc.mfdreceivers(3, 1.1, z, -1e9, np.arange(N, dtype=np.int32), ngb, nnb, elgt)
+    off = -rng.uniform(0, 0.3, (N, 3)) * wgt
+    off[rcv == np.arange(N)[:, None]] = 0.0
+    diag = 1.0 - off.sum(axis=1)
+    x, bad = orc.spl_solve(rcv, off, diag, z)
+    A = np.diag(diag)
+    for i in range(N):
+        for k in range(3):
+            if rcv[i, k] != i:
+                A[i, rcv[i, k]] += off[i, k]
+    xd = np.linalg.solve(A, z)
+    assert bad == 0
+    assert np.abs(x - xd).max() <= 1e-11 * np.abs(xd).max()
+
+
+def test_hillslope_solve_exact_and_parabola():
+    v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(16, 6)
+    N = len(v)
+    rng = np.random.default_rng(6)
+    h = rng.uniform(0, 100, N)
+    kd = np.full(N, 50.0)
+    coeff = orc.sethillslopecoeff(kd, ngb, nnb, area, elgt, vdist)
+    x, it = orc.hillslope_solve(coeff, h, ngb, nnb)
+    A = np.diag(coeff[:, 0])
+    for i in range(N):
+        for p in range(nnb[i]):
+            A[i, ngb[i, p]] += coeff[i, p + 1]
+    xd = np.linalg.solve(A, h)
+    assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max()
+    # row sums: diag - 1 == -sum(off)
+    assert np.allclose(coeff[:, 0] - 1.0, -coeff[:, 1:].sum(axis=1), rtol=0, atol=1e-14)
+
+
+def test_hillslope_steady_parabola():
+    """Uplift U against fixed 0 m borders on a strip reaches z = U/(2k) x (L-x)
+    (benchmarks/test_hillslope.py:35-52) — the implicit operator is the FV
+    Laplacian."""
+    nx, ny, dx = 41, 6, 100.0
+    v, c = gmesh.planar_jittered(nx, ny, dx, 0.0, 0)
+    lib = _lib.load()
+    ngb, nnb, elgt, vdist, area, _ = gmesh.fv_geometry(v, c, True, lib)
+    N = len(v)
+    L = (nx - 1) * dx
+    x = v[:, 0]
+    fixed = (x == 0) | (x == L)
+    U, k, dt = 1e-3, 1.0, 5e4
+    h = np.zeros(N)
+    coeff = orc.sethillslopecoeff(np.full(N, k * dt), ngb, nnb, area, elgt, vdist)
+    coeff[fixed, 1:] = 0.0
+    coeff[fixed, 0] = 1.0
+    for _ in range(400):
+        rhs = h + np.where(fixed, 0.0, U * dt)
+        h, _ = orc.hillslope_solve(coeff, rhs, ngb, nnb, maxit=2000)
+    mid = np.abs(v[:, 1] - v[:, 1].mean()) < dx
+    exact = U / (2 * k) * x * (L - x)
+    err = np.abs(h[mid] - exact[mid]).max() / exact.max()
+    assert err < 0.05
+
+
+def test_cabi_exports_every_declared_symbol(lib):
+    import re
+
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "gospl_hip.h")).read()
+    names = set(re.findall(r"\b(gospl_[a-z0-9_]+)\s*\(", hdr))
+    assert names, "no declarations found"
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names <= set(_lib.SIGNATURES), sorted(names - set(_lib.SIGNATURES))

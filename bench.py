@@ -1,0 +1,170 @@
+"""Benchmark of the per-timestep surface-process hot path on one MI355X.
+
+Workload: the 2.6M-vertex icosahedral sphere (refinement level 9, 2,621,442
+vertices, radius 6371 km), SH initial surface (degree <= 64, spectrum l^-2,
+scaled to [-6000, 5000] m, sea level 0), lognormal rain, K=5e-6, m=0.5, n=1,
+3 MFD receivers, diffusivity 0.5 m2/yr, dt = 5000 yr.  One "step" is one pass
+of goSPL's runProcesses body over the whole mesh: flow accumulation (fill,
+pit information, receivers, flow solve, overflow cascade), stream-power
+erosion, hillslope diffusion and forcing.  Inputs are resident in HBM before
+the timed region.
+
+Prints one JSON line (the driver's contract) with the roofline of the
+dominant kernel and the CPU oracle timed on a bounded sample beside it.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic (compulsory, single-pass) bytes per vertex per step of each
+# phase at FP64 state, int32 indices, 6 neighbours, 3 receivers.
+ALG_BYTES = {
+    "gospl_fill_depressions": 44,
+    "gospl_pit_info": 64,
+    "gospl_flow_direction": 156,
+    "gospl_flowacc": 68,
+    "gospl_spl": 116,
+    "gospl_hillslope": 116,
+}
+METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel err"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(level, seed, device):
+    from gospl_amd import _lib, workloads
+
+    t0 = time.time()
+    m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device)
+    log("[bench] mesh level %d: N=%d built in %.1fs" % (level, m.N, time.time() - t0))
+    return m, h0, p
+
+
+def cpu_baseline(m, h0, p, steps):
+    """The FP64 oracle (serial C + numpy, exact solves) timed on this host."""
+    from oracle import oracle as orc
+    from gospl_amd import workloads
+
+    conv = {k: (m.to_input_order(v) if isinstance(v, np.ndarray) else v) for k, v in p.items()}
+    o = orc.OracleModel(workloads.oracle_geometry(m), m.to_input_order(h0), tend=1e12, **conv)
+    o.initialise()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step()
+    dt = time.perf_counter() - t0
+    return steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--level", type=int, default=9)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the whole mesh lives on one MI355X and this build has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gospl_amd.model import Model
+
+    dev = torch.device("cuda:0")
+    m, h0, p = build_workload(args.level, args.seed, dev)
+    g = Model(m, h0, tend=1e12, device=dev, **p)
+    t0 = time.time()
+    g.initialise()
+    torch.cuda.synchronize()
+    log("[bench] initialise %.2fs" % (time.time() - t0))
+    for i in range(args.warmup):
+        g.step()
+        torch.cuda.synchronize()
+        log("[bench] warmup %d done (fa levels %d, spl levels %d, cascade %d, pits %d)" %
+            (i, g.fa_levels, g.spl_levels, g.cascade_passes, g.npits))
+
+    # timed region: K steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sps = args.steps / elapsed
+    log("[bench] %d steps in %.3fs -> %.2f steps/s" % (args.steps, elapsed, sps))
+
+    # per-entry-point durations (HIP events on the launch stream) over a second
+    # timed pass of the same length
+    g.enable_profiling(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        g.step()
+    torch.cuda.synchronize()
+    prof_elapsed = time.perf_counter() - t1
+    ph = g.phase_times()
+    g.enable_profiling(False)
+    N = m.N
+    phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
+    dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms_total"])
+    bytes_per_launch = ALG_BYTES[dom] * N
+    achieved = bytes_per_launch / (phases[dom]["ms_per_call"] * 1e-3) / 1e9
+    log("[bench] phases (ms per step): " + ", ".join(
+        "%s=%.3f" % (k, v["ms_total"] / args.steps) for k, v in sorted(phases.items(), key=lambda x: -x[1]["ms_total"])))
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cps, cdt = cpu_baseline(m, h0, p, args.cpu_steps)
+        log("[bench] cpu oracle %d steps in %.1fs (%.3f steps/s), total %.1fs" %
+            (args.cpu_steps, cdt, cps, time.time() - t0))
+        cpu = {"value": cps, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": "FP64 oracle (C + numpy, exact DAG solves), %d steps of the same %d-vertex "
+                         "workload after its init flow accumulation, one host core" % (args.cpu_steps, N)}
+
+    out = {
+        "metric": METRIC,
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SH surface, lognormal rain)",
+        "config": {"workload": "icosahedral sphere level %d, %d vertices, FP64 plain path" % (args.level, N),
+                   "vertices": N, "dt_yr": p["dt"], "flowdir": p["flowdir"],
+                   "vertex_steps_per_s": sps * N,
+                   "fp32_refine": "not implemented",
+                   "phases_ms_per_step": {k: v["ms_total"] / args.steps for k, v in phases.items()},
+                   "profiled_pass_s": prof_elapsed},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "alg_bytes_per_vertex": ALG_BYTES[dom]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -223,13 +223,25 @@ __global__ void k_cc_hook(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
 }
 
+// Read-only find: after hooking no pointer may move any more, and a halving
+// store here could overwrite a neighbour's final root with an intermediate
+// ancestor.
+__device__ __forceinline__ int uf_root(const int32_t* parent, int x) {
+  int p = parent[x];
+  while (p != x) {
+    x = p;
+    p = parent[x];
+  }
+  return x;
+}
+
 __global__ void k_cc_compress(int64_t N, int32_t* parent, const int8_t* __restrict__ noflow,
                               const int32_t* __restrict__ gid, const int8_t* __restrict__ outlet,
                               int flat, int32_t* __restrict__ cmin, int32_t* __restrict__ cnon) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   if (parent[i] < 0) return;
-  int r = uf_find(parent, (int)i);
+  int r = uf_root(parent, (int)i);
   parent[i] = r;
   if (noflow[i]) atomicMin(&cmin[r], gid[i]);
   if (!(flat && outlet[i])) cnon[r] = 1;

@@ -285,7 +285,8 @@ def build_mesh(coords, cells, lib, flat=None, order="morton", outlet_mask=None) 
     coords = np.ascontiguousarray(coords, dtype=np.float64)
     cells = np.ascontiguousarray(cells, dtype=np.int32)
     N = len(coords)
-    bverts = boundary_vertices(cells)
+    # a closed sphere has no boundary edges: skip the (large) edge count
+    bverts = boundary_vertices(cells) if flat is not False else np.zeros(0, np.int64)
     if flat is None:
         flat = len(bverts) > 0
     ngb, nnb, elgt, vdist, area, vi = fv_geometry(coords, cells, flat, lib)

@@ -162,6 +162,8 @@ class Model:
         self.fa_levels = 0
         self.spl_levels = 0
         self.timers = {}
+        self.profile = False
+        self._events = []
 
     # -------------------------------------------------------------- helpers
     @property
@@ -169,7 +171,33 @@ class Model:
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _call(self, name, *args):
-        _lib.check(getattr(self.lib, name)(*args), name)
+        if self.profile:
+            s = torch.cuda.current_stream(self.device)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            _lib.check(getattr(self.lib, name)(*args), name)
+            e1.record(s)
+            self._events.append((name, e0, e1))
+        else:
+            _lib.check(getattr(self.lib, name)(*args), name)
+
+    def enable_profiling(self, on=True):
+        """Record HIP events (on the launch stream) around every library call."""
+        self.profile = on
+        self._events = []
+
+    def phase_times(self, reset=True):
+        """Milliseconds per library entry point since the last reset."""
+        torch.cuda.synchronize(self.device)
+        out = {}
+        for name, e0, e1 in self._events:
+            d = out.setdefault(name, [0.0, 0])
+            d[0] += e0.elapsed_time(e1)
+            d[1] += 1
+        if reset:
+            self._events = []
+        return out
 
     def _make_kfac(self):
         if self.sedfacVal is None and self.coeffd == 0.0:

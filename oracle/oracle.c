@@ -419,9 +419,10 @@ void or_sethillslopecoeff(int64_t nb, const int32_t* ngb, const int32_t* nnb,
 /* vertices left unsolved (nonzero only if the receivers hold a cycle).      */
 /* ------------------------------------------------------------------------ */
 int64_t or_flowacc(int64_t nb, int K, const int32_t* rcv, const double* wgt,
-                   const double* b, double* q) {
+                   const double* b, double* q, const int32_t* ngb, const int32_t* nnb) {
+  /* Receivers are mesh neighbours, so the donors of i are found among its
+     FV neighbours; each solved vertex sums its donors in neighbour order. */
   int32_t* pend = (int32_t*)calloc((size_t)nb, sizeof(int32_t));
-  double* acc = (double*)calloc((size_t)nb, sizeof(double));
   int64_t* st = (int64_t*)malloc((size_t)(nb + 1) * sizeof(int64_t));
   for (int64_t j = 0; j < nb; ++j)
     for (int k = 0; k < K; ++k) {
@@ -432,19 +433,28 @@ int64_t or_flowacc(int64_t nb, int K, const int32_t* rcv, const double* wgt,
   for (int64_t j = 0; j < nb; ++j) if (pend[j] == 0) st[ns++] = j;
   while (ns > 0) {
     int64_t i = st[--ns];
-    q[i] = b[i] + acc[i];
+    double s = b[i];
+    for (int p = 0; p < nnb[i]; ++p) {
+      int32_t j = ngb[i * MAXNB + p];
+      if (j < 0) continue;
+      for (int k = 0; k < K; ++k) {
+        if (rcv[j * K + k] == i) {
+          double w = wgt[j * K + k];
+          if (w != 0.0) s = s + w * q[j];
+        }
+      }
+    }
+    q[i] = s;
     done++;
     for (int k = 0; k < K; ++k) {
       int32_t r = rcv[i * K + k];
-      double w = wgt[i * K + k];
-      if (r >= 0 && r != i && w != 0.0) {
-        acc[r] += w * q[i];
+      if (r >= 0 && r != i && wgt[i * K + k] != 0.0) {
         if (--pend[r] == 0) st[ns++] = r;
       }
     }
   }
   for (int64_t j = 0; j < nb; ++j) if (pend[j] > 0) q[j] = NAN;
-  free(pend); free(acc); free(st);
+  free(pend); free(st);
   return nb - done;
 }
 

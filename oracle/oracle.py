@@ -58,7 +58,7 @@ def lib():
         L.or_nghb_dir.restype = I32
         L.or_getpitvol.argtypes = [P, P, P, P, P, I64, I64, P]
         L.or_sethillslopecoeff.argtypes = [I64, P, P, P, P, P, P, P]
-        L.or_flowacc.argtypes = [I64, I32, P, P, P, P]
+        L.or_flowacc.argtypes = [I64, I32, P, P, P, P, P, P]
         L.or_flowacc.restype = I64
         L.or_spl_solve.argtypes = [I64, I32, P, P, P, P, P]
         L.or_spl_solve.restype = I64
@@ -145,10 +145,11 @@ def sethillslopecoeff(kd, ngb, nnb, area, elgt, vdist):
     return out
 
 
-def flowacc(rcv, wgt, b):
+def flowacc(rcv, wgt, b, ngb, nnb):
     N, K = rcv.shape
     q = np.empty(N)
-    bad = lib().or_flowacc(N, K, _p(_c(rcv, np.int32)), _p(_c(wgt, np.float64)), _p(_c(b, np.float64)), _p(q))
+    bad = lib().or_flowacc(N, K, _p(_c(rcv, np.int32)), _p(_c(wgt, np.float64)), _p(_c(b, np.float64)), _p(q),
+                           _p(_c(ngb, np.int32)), _p(_c(nnb, np.int32)))
     return q, int(bad)
 
 
@@ -380,7 +381,7 @@ class OracleModel:
         self.lsink = (lsink.astype(int) * self.inIDs) == 1
 
     def _solve_flow(self, b):
-        q, bad = flowacc(self.rcvID, self.wghtVal, b)
+        q, bad = flowacc(self.rcvID, self.wghtVal, b, self.ngb, self.nnb)
         if bad:
             raise RuntimeError("receiver graph has a cycle (%d vertices unsolved)" % bad)
         return q

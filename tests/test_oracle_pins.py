@@ -170,7 +170,7 @@ def test_flowacc_exact_and_mass_balance():
     z = rng.uniform(0, 30, len(v)) + 0.05 * v[:, 0]
     rcv, dist, wgt = orc.mfdreceivers(3, 1.1, z, -1e9, np.arange(len(v), dtype=np.int32), ngb, nnb, elgt)
     b = rng.uniform(0.5, 2.0, len(v)) * area
-    q, bad = orc.flowacc(rcv, wgt, b)
+    q, bad = orc.flowacc(rcv, wgt, b, ngb, nnb)
     assert bad == 0
     qd = _dense_flow(rcv, wgt, b)
     assert np.abs(q - qd).max() <= 1e-10 * np.abs(qd).max()

@@ -139,7 +139,7 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
   GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
   // scan workspace sized for N ints
   size_t bytes = 0;
-  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, m->wi[0], m->wi[1], (int)N));
+  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, m->wi[0], m->wi[1], (int)(N + 1)));
   m->cub_tmp_bytes = bytes + 256;
   GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
 
@@ -168,6 +168,8 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   hipHostFree(m->hflag);
   hipFree(m->cub_tmp);
   if (m->wcoef) hipFree(m->wcoef);
+  if (m->wdon) hipFree(m->wdon);
+  if (m->wdval) hipFree(m->wdval);
   delete m;
   return 0;
 }

@@ -33,6 +33,11 @@ struct gospl_mesh {
   int32_t* hflag;             // pinned host mirror (64 ints)
   double* wcoef;              // (wcoef_rows, N) coefficient rows
   int wcoef_rows;
+  int32_t* wdon;              // donor rows of the receiver DAG
+  double* wdval;
+  int64_t wdon_cap;
+  int dag_chain;              // chaining in the DAG sweep (GOSPL_DAG_CHAIN)
+  int dag_small;              // frontier size handled by one block (GOSPL_DAG_SMALL)
   void* cub_tmp;
   size_t cub_tmp_bytes;
 };

@@ -77,7 +77,20 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 5 copies of %d MiB" % args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the whole mesh lives on one MI355X and this build has no "
               "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)

@@ -79,12 +79,19 @@ __global__ void k_finalize_a(int64_t N, const double* __restrict__ hl, const dou
     fal[i] = f;
     v = f;
   }
-  // wave-level max then one atomic per wave
+  // wave-level max, block-level max, then one atomic per block
   for (int o = 32; o > 0; o >>= 1) {
     double w = __shfl_down(v, o, 64);
     v = v > w ? v : w;
   }
-  if ((threadIdx.x & 63) == 0 && v > -INFINITY) atomicMax(maxkey, dkey(v));
+  __shared__ double s_max[GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_max[0];
+    for (int k = 1; k < GOSPL_BLOCK / 64; ++k) b = b > s_max[k] ? b : s_max[k];
+    if (b > -INFINITY) atomicMax(maxkey, dkey(b));
+  }
 }
 
 __global__ void k_finalize_b(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,

@@ -69,9 +69,22 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k) {
   return __longlong_as_double((long long)b);
 }
 
+// Append `value` for every lane with `pred` set, one atomicAdd per wavefront.
+// Must be reached by all lanes of the wavefront.
+__device__ __forceinline__ void wave_append(bool pred, int value, int32_t* counter, int32_t* out) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) out[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 }  // namespace gospl
 
-#define GOSPL_CHECK(expr)                                                       \
+#define GOSPL_CHECK(expr)                                                     \
   do {                                                                          \
     hipError_t e_ = (expr);                                                     \
     if (e_ != hipSuccess) {                                                     \

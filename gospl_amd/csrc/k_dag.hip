@@ -9,12 +9,11 @@
 //   * a donor list (reverse receiver graph, in FV neighbour order) is built
 //     first so each vertex pulls its dependencies from a contiguous row;
 //   * pending counters hold the number of unsolved dependencies; a vertex
-//     whose counter reaches zero joins the next frontier;
+//     whose counter reaches zero joins the next frontier (wavefront-
+//     aggregated appends: one atomic per wavefront, not per vertex);
 //   * one light grid barrier (atomic arrival counter + generation word)
-//     separates levels;
-//   * optional chaining (GOSPL_DAG_CHAIN=1): the thread that releases a
-//     vertex whose other dependencies were published in earlier levels
-//     computes it at once.
+//     separates levels; the grid is small because the sweep is bound by the
+//     latency of each level, not by bandwidth.
 // Every value is a fixed-order sum over its own row, so results do not depend
 // on the schedule (no floating-point atomics).
 #include <cstdlib>
@@ -70,22 +69,23 @@ __global__ void k_dep_fill(int64_t N, int W, int NR, const int32_t* __restrict__
 // pending counters + frontier 0
 __global__ void k_pending(int mode, int64_t N, int NR, const int32_t* __restrict__ rcv,
                           const double* __restrict__ coef, const int32_t* __restrict__ doff,
-                          int32_t* __restrict__ pend, int32_t* __restrict__ lv,
-                          int32_t* __restrict__ front, int32_t* __restrict__ counts) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+                          int32_t* __restrict__ pend, int32_t* __restrict__ front,
+                          int32_t* __restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < N;
   int c = 0;
-  if (mode == 0) {
-    c = doff[i + 1] - doff[i];
-  } else {
-    for (int k = 0; k < NR; ++k) {
-      int r = rcv[(int64_t)k * N + i];
-      if (r != (int)i && coef[(int64_t)k * N + i] != 0.0) c++;
+  if (valid) {
+    if (mode == 0) {
+      c = doff[i + 1] - doff[i];
+    } else {
+      for (int k = 0; k < NR; ++k) {
+        int r = rcv[(int64_t)k * N + i];
+        if (r != (int)i && coef[(int64_t)k * N + i] != 0.0) c++;
+      }
     }
+    pend[i] = c;
   }
-  pend[i] = c;
-  lv[i] = -1;
-  if (c == 0) front[atomicAdd(&counts[0], 1)] = (int)i;
+  wave_append(valid && c == 0, (int)i, &counts[0], front);
 }
 
 // Grid barrier (all blocks co-resident: cooperative launch).  bar[0]
@@ -124,144 +124,67 @@ __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int nbl
 }
 
 template <int MODE>
-__device__ __forceinline__ double solve_one(int v, int64_t N, int NR, const int32_t* __restrict__ rcv,
-                                            const double* __restrict__ coef,
-                                            const double* __restrict__ diag,
-                                            const double* __restrict__ rhs, const double* x,
-                                            const int32_t* __restrict__ doff,
-                                            const int32_t* __restrict__ don,
-                                            const double* __restrict__ dval) {
-  double s = rhs[v];
-  if (MODE == 0) {
-    const int e = doff[v + 1];
-    for (int c = doff[v]; c < e; ++c) s = s + dval[c] * x[don[c]];
-    return s;
-  } else {
-    for (int k = 0; k < NR; ++k) {
-      int r = rcv[(int64_t)k * N + v];
-      double o = coef[(int64_t)k * N + v];
-      if (r != v && o != 0.0) s = s - o * x[r];
-    }
-    return s / diag[v];
-  }
-}
-
-// Chaining test: every dependency of r other than v was solved in an earlier
-// level (so published by a barrier).
-template <int MODE>
-__device__ __forceinline__ bool ready_now(int r, int v, int level, int64_t N, int NR,
-                                          const int32_t* __restrict__ rcv,
-                                          const double* __restrict__ coef,
-                                          const int32_t* __restrict__ doff,
-                                          const int32_t* __restrict__ don, const int32_t* lv) {
-  if (MODE == 0) {
-    const int e = doff[r + 1];
-    for (int c = doff[r]; c < e; ++c) {
-      int j = don[c];
-      if (j == v) continue;
-      int l = __hip_atomic_load(&lv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (l < 0 || l >= level) return false;
-    }
-  } else {
-    for (int k = 0; k < NR; ++k) {
-      int q = rcv[(int64_t)k * N + r];
-      if (q == r || q == v || coef[(int64_t)k * N + r] == 0.0) continue;
-      int l = __hip_atomic_load(&lv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (l < 0 || l >= level) return false;
-    }
-  }
-  return true;
-}
-
-template <int MODE, int CHAIN>
 __global__ __launch_bounds__(GOSPL_BLOCK) void k_dag_level(
-    int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ coef,
+    int64_t N, int W, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ coef,
     const double* __restrict__ diag, const double* __restrict__ rhs, double* x,
     const int32_t* __restrict__ doff, const int32_t* __restrict__ don,
-    const double* __restrict__ dval, int32_t* __restrict__ pend, int32_t* lv,
-    int32_t* __restrict__ fa, int32_t* __restrict__ fb, int32_t* __restrict__ counts,
-    unsigned int* __restrict__ bar, int32_t* __restrict__ nlev, int small) {
+    const double* __restrict__ dval, int32_t* __restrict__ pend, int32_t* __restrict__ fa,
+    int32_t* __restrict__ fb, int32_t* __restrict__ counts, unsigned int* __restrict__ bar,
+    int32_t* __restrict__ nlev) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nth = (int64_t)gridDim.x * blockDim.x;
   int level = 0;
-  __shared__ int s_n;
   for (;;) {
     const int n = load_count(&counts[level]);
     if (n == 0) break;
-    if (n <= small) {
-      // Narrow frontier: block 0 runs consecutive narrow levels on its own,
-      // separated by block barriers only; the other blocks wait at the next
-      // grid barrier and resume at the level block 0 publishes.
-      if (blockIdx.x == 0) {
-        int ln = n;
-        for (;;) {
-          int32_t* cur = (level & 1) ? fb : fa;
-          int32_t* nxt = (level & 1) ? fa : fb;
-          for (int t = threadIdx.x; t < ln; t += blockDim.x) {
-            const int v = cur[t];
-            x[v] = solve_one<MODE>(v, N, NR, rcv, coef, diag, rhs, x, doff, don, dval);
-            if (MODE == 0) {
-              for (int k = 0; k < NR; ++k) {
-                int r = rcv[(int64_t)k * N + v];
-                if (r != v && r >= 0 && coef[(int64_t)k * N + v] != 0.0)
-                  if (atomicSub(&pend[r], 1) == 1) nxt[atomicAdd(&counts[level + 1], 1)] = r;
-              }
-            } else {
-              const int e = doff[v + 1];
-              for (int c = doff[v]; c < e; ++c) {
-                int j = don[c];
-                if (atomicSub(&pend[j], 1) == 1) nxt[atomicAdd(&counts[level + 1], 1)] = j;
-              }
-            }
-          }
-          __threadfence_block();
-          __syncthreads();
-          level++;
-          if (threadIdx.x == 0) s_n = load_count(&counts[level]);
-          __syncthreads();
-          ln = s_n;
-          if (ln == 0 || ln > small) break;
-        }
-        if (threadIdx.x == 0) __hip_atomic_store(&bar[3], (unsigned int)level, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (!grid_barrier(bar, gridDim.x)) break;
-      level = (int)__hip_atomic_load(&bar[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      continue;
-    }
     int32_t* cur = (level & 1) ? fb : fa;
     int32_t* nxt = (level & 1) ? fa : fb;
-    for (int64_t t = tid; t < n; t += nth) {
-      int v = cur[t];
-      while (v >= 0) {
-        x[v] = solve_one<MODE>(v, N, NR, rcv, coef, diag, rhs, x, doff, don, dval);
-        if (CHAIN) __hip_atomic_store(&lv[v], level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int next = -1;
-        if (MODE == 0) {
+    int32_t* cnt = &counts[level + 1];
+    // uniform trip count per wavefront so the aggregated appends see all lanes
+    for (int64_t t0 = tid - (tid & 63); t0 < n; t0 += nth) {
+      const int64_t t = t0 + (tid & 63);
+      const bool valid = t < n;
+      const int v = valid ? cur[t] : 0;
+      if (MODE == 0) {
+        int r[GOSPL_MAXW];
+        double w[GOSPL_MAXW];
+        if (valid) {
           for (int k = 0; k < NR; ++k) {
-            int r = rcv[(int64_t)k * N + v];
-            if (r != v && r >= 0 && coef[(int64_t)k * N + v] != 0.0) {
-              if (atomicSub(&pend[r], 1) == 1) {
-                if (CHAIN && next < 0 && ready_now<0>(r, v, level, N, NR, rcv, coef, doff, don, lv))
-                  next = r;
-                else
-                  nxt[atomicAdd(&counts[level + 1], 1)] = r;
-              }
-            }
+            r[k] = rcv[(int64_t)k * N + v];
+            w[k] = coef[(int64_t)k * N + v];
           }
-        } else {
+          double s = rhs[v];
           const int e = doff[v + 1];
-          for (int c = doff[v]; c < e; ++c) {
-            int j = don[c];
-            if (atomicSub(&pend[j], 1) == 1) {
-              if (CHAIN && next < 0 && ready_now<1>(j, v, level, N, NR, rcv, coef, doff, don, lv))
-                next = j;
-              else
-                nxt[atomicAdd(&counts[level + 1], 1)] = j;
-            }
-          }
+          for (int c = doff[v]; c < e; ++c) s = s + dval[c] * x[don[c]];
+          x[v] = s;
         }
-        v = next;
+        for (int k = 0; k < NR; ++k) {
+          bool rel = false;
+          if (valid && r[k] != v && r[k] >= 0 && w[k] != 0.0) rel = atomicSub(&pend[r[k]], 1) == 1;
+          wave_append(rel, valid ? r[k] : 0, cnt, nxt);
+        }
+      } else {
+        int b = 0, e = 0;
+        if (valid) {
+          double s = rhs[v];
+          for (int k = 0; k < NR; ++k) {
+            int rr = rcv[(int64_t)k * N + v];
+            double o = coef[(int64_t)k * N + v];
+            if (rr != v && o != 0.0) s = s - o * x[rr];
+          }
+          x[v] = s / diag[v];
+          b = doff[v];
+          e = doff[v + 1];
+        }
+        for (int c = 0; c < W; ++c) {
+          bool rel = false;
+          int j = 0;
+          if (valid && b + c < e) {
+            j = don[b + c];
+            rel = atomicSub(&pend[j], 1) == 1;
+          }
+          wave_append(rel, j, cnt, nxt);
+        }
       }
     }
     if (!grid_barrier(bar, gridDim.x)) break;
@@ -342,11 +265,6 @@ static int ensure_donors(gospl_mesh* m, int64_t cap) {
   return 0;
 }
 
-template <int MODE>
-static const void* level_kernel(int chain) {
-  return chain ? (const void*)k_dag_level<MODE, 1> : (const void*)k_dag_level<MODE, 0>;
-}
-
 static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
                      const double* diag, const double* rhs, double* x, int32_t* nlevels,
                      int64_t* unsolved, hipStream_t s) {
@@ -355,7 +273,6 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* pend = m->wi[0];
   int32_t* fa = m->wi[1];
   int32_t* fb = m->wi[2];
-  int32_t* lv = m->wi[3];
   int32_t* dcnt = m->wi[4];
   int32_t* doff = m->wi[5];
   int32_t* counts = m->counts;
@@ -370,17 +287,17 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   size_t bytes = m->cub_tmp_bytes;
   GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
   k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, m->wdval);
-  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, lv, fa, counts);
+  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, fa, counts);
   GOSPL_CHECK(hipGetLastError());
   int64_t NN = N;
+  int W = m->W;
   int NR = nrcv;
   int32_t* nlev = dev;
   int32_t* don = m->wdon;
   double* dval = m->wdval;
-  int small = m->dag_small;
-  void* args[] = {&NN, &NR, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff, &don,
-                  &dval, &pend, &lv, &fa, &fb, &counts, &bar, &nlev, &small};
-  const void* kern = mode == 0 ? level_kernel<0>(m->dag_chain) : level_kernel<1>(m->dag_chain);
+  void* args[] = {&NN, &W, &NR, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
+                  &don, &dval, &pend, &fa, &fb, &counts, &bar, &nlev};
+  const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
   GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(GOSPL_BLOCK), args, 0, s));
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1);
   GOSPL_CHECK(hipGetLastError());
@@ -402,9 +319,8 @@ using namespace gospl;
 
 extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   int per_cu = 1 << 30;
-  const void* ks[4] = {(const void*)k_dag_level<0, 0>, (const void*)k_dag_level<0, 1>,
-                       (const void*)k_dag_level<1, 0>, (const void*)k_dag_level<1, 1>};
-  for (int i = 0; i < 4; ++i) {
+  const void* ks[2] = {(const void*)k_dag_level<0>, (const void*)k_dag_level<1>};
+  for (int i = 0; i < 2; ++i) {
     int b = 0;
     GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ks[i], GOSPL_BLOCK, 0));
     per_cu = b < per_cu ? b : per_cu;
@@ -418,10 +334,6 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* env = getenv("GOSPL_DAG_BLOCKS");
   if (env && atoi(env) > 0) want = atoi(env);
   m->coop_blocks = want < maxb ? want : maxb;
-  const char* ch = getenv("GOSPL_DAG_CHAIN");
-  m->dag_chain = (ch && atoi(ch) > 0) ? 1 : 0;
-  const char* sm = getenv("GOSPL_DAG_SMALL");
-  m->dag_small = (sm && atoi(sm) >= 0) ? atoi(sm) : 0;
   return 0;
 }
 

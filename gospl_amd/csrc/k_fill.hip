@@ -258,13 +258,17 @@ __global__ void k_pit_mark(int64_t N, const int32_t* __restrict__ parent,
 __global__ void k_pit_ids(int64_t N, const int32_t* __restrict__ parent,
                           const int32_t* __restrict__ cmin, const int32_t* __restrict__ cnon,
                           const int32_t* __restrict__ rank, const int8_t* __restrict__ outlet,
-                          int flat, int32_t* __restrict__ pitids) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+                          int flat, int32_t* __restrict__ pitids, int32_t* __restrict__ ncells,
+                          int32_t* __restrict__ cells) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < N;
   int id = -1;
-  int r = parent[i];
-  if (r >= 0 && cmin[r] != INT_MAX && cnon[r] && !(flat && outlet[i])) id = rank[cmin[r]] + 1;
-  pitids[i] = id;
+  if (valid) {
+    int r = parent[i];
+    if (r >= 0 && cmin[r] != INT_MAX && cnon[r] && !(flat && outlet[i])) id = rank[cmin[r]] + 1;
+    pitids[i] = id;
+  }
+  wave_append(valid && id > 0, (int)i, ncells, cells);  // compact list of pit cells
 }
 
 // spill candidates (pitfilling.py:741-801)
@@ -336,11 +340,11 @@ __global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* 
 // not-descending edges (fill_dir + nghb_dir, functions.F90:1845-1969).
 __global__ void k_ptdir_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
                               const int32_t* __restrict__ deg, const double* __restrict__ fill,
-                              const int32_t* __restrict__ pitids, int32_t* ptdir,
-                              int32_t* __restrict__ flag) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  if (pitids[c] < 0) return;
+                              const int32_t* __restrict__ ncells, const int32_t* __restrict__ cells,
+                              int32_t* ptdir, int32_t* __restrict__ flag) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= *ncells) return;
+  const int c = cells[t];
   const int cur = ptdir[c];
   if (cur == 1) return;
   const double fc = fill[c];
@@ -538,10 +542,16 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   // npits = rank[N-1] + mark[N-1]
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 16, rank + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 17, mark + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  k_pit_ids<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, rank, m->outlet, m->flat, pitids);
+  int32_t* ncells = m->dflag + 40;
+  int32_t* cells = mark;  // the marks are consumed by the scan above
+  GOSPL_CHECK(hipMemsetAsync(ncells, 0, sizeof(int32_t), s));
+  k_pit_ids<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, rank, m->outlet, m->flat, pitids, ncells,
+                                       cells);
   GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 18, ncells, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
   const int64_t npits = (int64_t)m->hflag[16] + m->hflag[17];
+  const int npc = m->hflag[18];
   *npits_out = npits;
   const int64_t np1 = npits + 1;
   if (np1 > cap) {
@@ -561,10 +571,11 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   // flat routing distances
   k_ptdir_init<<<g, GOSPL_BLOCK, 0, s>>>(N, lspill, ptdir);
   GOSPL_CHECK(hipGetLastError());
-  if (npits > 0) {
+  if (npits > 0 && npc > 0) {
     int it = 0;
-    if (until_stable(m, s, (int)N + 16, 4, [&](int32_t* f) {
-          k_ptdir_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, ptdir, f);
+    if (until_stable(m, s, npc + 32, 16, [&](int32_t* f) {
+          k_ptdir_relax<<<nblocks(npc), GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, ncells,
+                                                             cells, ptdir, f);
         }, &it))
       return 1;
   }

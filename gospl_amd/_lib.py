@@ -39,6 +39,8 @@ SIGNATURES = {
     "gospl_pit_water": (I32, [P, P, P, P, P, P, P]),
     "gospl_cascade_add": (I32, [P, P, P, P, F64, P, P, P]),
     "gospl_flow_finalize": (I32, [P, P, P, P, I32, P, P, P]),
+    "gospl_sweep_timing": (I32, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64),
+                                 ctypes.POINTER(F64), ctypes.POINTER(I64)]),
 }
 
 

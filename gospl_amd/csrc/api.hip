@@ -175,6 +175,8 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   hipHostFree(m->hflag);
   hipFree(m->cub_tmp);
   if (m->wcoef) hipFree(m->wcoef);
+  if (m->tev[0]) hipEventDestroy(m->tev[0]);
+  if (m->tev[1]) hipEventDestroy(m->tev[1]);
   if (m->wdon) hipFree(m->wdon);
   if (m->wdval) hipFree(m->wdval);
   delete m;

@@ -37,7 +37,10 @@ struct gospl_mesh {
   double* wdval;
   int64_t wdon_cap;
   int dag_chain;              // chaining in the DAG sweep (GOSPL_DAG_CHAIN)
-  int dag_small;              // frontier size handled by one block (GOSPL_DAG_SMALL)
+  int timing;                 // record HIP events around the DAG sweep kernels
+  hipEvent_t tev[2];
+  double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep
+  int64_t t_calls[2];
   void* cub_tmp;
   size_t cub_tmp_bytes;
 };

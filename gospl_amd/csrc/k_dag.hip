@@ -298,7 +298,9 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   void* args[] = {&NN, &W, &NR, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
                   &don, &dval, &pend, &fa, &fb, &counts, &bar, &nlev};
   const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
+  if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(GOSPL_BLOCK), args, 0, s));
+  if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
@@ -307,6 +309,12 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   if (m->hflag[34]) {
     set_error("receiver-DAG sweep aborted: a grid barrier did not complete");
     return 1;
+  }
+  if (m->timing) {
+    float ms = 0.0f;
+    GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
+    m->t_ms[mode] += ms;
+    m->t_calls[mode] += 1;
   }
   if (nlevels) *nlevels = m->hflag[0];
   if (unsolved) *unsolved = m->hflag[1];
@@ -338,6 +346,22 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
 }
 
 extern "C" {
+
+int gospl_sweep_timing(gospl_mesh_t* m, int32_t enable, double* fa_ms, int64_t* fa_calls,
+                       double* spl_ms, int64_t* spl_calls) {
+  if (enable && !m->tev[0]) {
+    GOSPL_CHECK(hipEventCreate(&m->tev[0]));
+    GOSPL_CHECK(hipEventCreate(&m->tev[1]));
+  }
+  if (fa_ms) *fa_ms = m->t_ms[0];
+  if (fa_calls) *fa_calls = m->t_calls[0];
+  if (spl_ms) *spl_ms = m->t_ms[1];
+  if (spl_calls) *spl_calls = m->t_calls[1];
+  m->t_ms[0] = m->t_ms[1] = 0.0;
+  m->t_calls[0] = m->t_calls[1] = 0;
+  m->timing = enable ? 1 : 0;
+  return 0;
+}
 
 int gospl_flowacc(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const double* wgt,
                   const double* b, double* q, int32_t* nlevels, int64_t* unsolved,

@@ -174,6 +174,15 @@ int gospl_flow_finalize(gospl_mesh_t* mesh, const double* hl, const double* wate
                         const int32_t* pitids, int32_t has_pits, double* fal,
                         double* fillfal, gospl_stream_t stream);
 
+/*
+ * Instrumentation (no counterpart in goSPL): when enabled, HIP events on the
+ * launch stream bracket every DAG sweep kernel (flow accumulation and SPL).
+ * Returns the accumulated milliseconds and call counts since the previous
+ * call (HOST outputs, may be NULL), resets them, and sets the enable flag.
+ */
+int gospl_sweep_timing(gospl_mesh_t* mesh, int32_t enable, double* fa_ms, int64_t* fa_calls,
+                       double* spl_ms, int64_t* spl_calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ struct gospl_mesh {
   double* wdval;
   int64_t wdon_cap;
   int dag_chain;              // chaining in the DAG sweep (GOSPL_DAG_CHAIN)
+  int tile_S;                 // tile size of the tile-resident DAG sweep (0 = level sweep only)
   int timing;                 // record HIP events around the DAG sweep kernels
   hipEvent_t tev[2];
   double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep

@@ -253,6 +253,171 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
+// ---------------------------------------------------------------------------
+// Tile-resident asynchronous sweep.
+//
+// Block b owns the Morton tile [b*S, min((b+1)*S, N)).  Dependencies inside
+// the tile are counted in LDS and released with LDS atomics; a vertex whose
+// in-tile counter reaches zero is queued in LDS if its out-of-tile counter
+// (global, `ext`) is already zero.  Dependencies on other tiles are released
+// with a global atomic carrying release semantics; the owning block finds
+// such vertices by polling its boundary list (vertices that had out-of-tile
+// dependencies) with acquire loads.  No grid barrier: a hop inside a tile
+// costs a block barrier and a few L2 hits.  All blocks are co-resident
+// (cooperative launch); a block that sees no global progress for a long
+// time raises `abort` (then every block leaves and the host falls back to the
+// level-synchronous sweep, which also handles cycles).
+// ---------------------------------------------------------------------------
+
+template <int MODE>
+__global__ void k_tile_ext(int64_t N, int NR, int S, const int32_t* __restrict__ rcv,
+                           const double* __restrict__ coef, const int32_t* __restrict__ doff,
+                           const int32_t* __restrict__ don, int32_t* __restrict__ ext) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const int tv = (int)(v / S);
+  int c = 0;
+  if (MODE == 0) {
+    for (int e = doff[v]; e < doff[v + 1]; ++e)
+      if (don[e] / S != tv) c++;
+  } else {
+    for (int k = 0; k < NR; ++k) {
+      int r = rcv[(int64_t)k * N + v];
+      if (r != (int)v && coef[(int64_t)k * N + v] != 0.0 && r / S != tv) c++;
+    }
+  }
+  ext[v] = c;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_dag_tile(
+    int64_t N, int NR, int S, const int32_t* __restrict__ rcv, const double* __restrict__ coef,
+    const double* __restrict__ diag, const double* __restrict__ rhs, double* x,
+    const int32_t* __restrict__ doff, const int32_t* __restrict__ don,
+    const double* __restrict__ dval, int32_t* ext, unsigned int* __restrict__ ctl) {
+  // ctl[0] global progress (vertices done), ctl[1] abort
+  extern __shared__ int smem[];
+  int* pend = smem;                                           // S ints
+  short* q0 = reinterpret_cast<short*>(pend + S);             // S shorts
+  short* q1 = q0 + S;                                         // S shorts
+  short* bnd = q1 + S;                                        // S shorts (boundary list)
+  __shared__ int s_qn[2], s_nb, s_done, s_stop;
+  const int t0 = blockIdx.x * S;
+  const int n = (int)((int64_t)t0 + S <= N ? S : N - t0);
+  if (threadIdx.x == 0) { s_qn[0] = 0; s_qn[1] = 0; s_nb = 0; s_done = 0; s_stop = 0; }
+  __syncthreads();
+  // in-tile dependency counts, initial queue, boundary list
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    const int v = t0 + l;
+    int c = 0;
+    if (MODE == 0) {
+      for (int e = doff[v]; e < doff[v + 1]; ++e) {
+        int j = don[e];
+        if (j >= t0 && j < t0 + n) c++;
+      }
+    } else {
+      for (int k = 0; k < NR; ++k) {
+        int r = rcv[(int64_t)k * N + v];
+        if (r != v && coef[(int64_t)k * N + v] != 0.0 && r >= t0 && r < t0 + n) c++;
+      }
+    }
+    const int e = ext[v];
+    pend[l] = c;
+    if (e > 0) bnd[atomicAdd(&s_nb, 1)] = (short)l;
+    if (c == 0 && e == 0) {
+      pend[l] = -1000000;
+      q0[atomicAdd(&s_qn[0], 1)] = (short)l;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  unsigned int last_global = 0;
+  int idle = 0;
+  while (true) {
+    __syncthreads();
+    if (s_done >= n || s_stop) break;
+    short* qc = cur ? q1 : q0;
+    short* qn = cur ? q0 : q1;
+    if (s_qn[cur] == 0) {
+      // poll the boundary list for vertices released by other tiles
+      for (int b = threadIdx.x; b < s_nb; b += blockDim.x) {
+        const int l = bnd[b];
+        if (l < 0) continue;
+        if (pend[l] == 0 &&
+            __hip_atomic_load(&ext[t0 + l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+          if (atomicCAS(&pend[l], 0, -1000000) == 0) {
+            qc[atomicAdd(&s_qn[cur], 1)] = (short)l;
+            bnd[b] = -1;
+          }
+        }
+      }
+      __syncthreads();
+      if (s_qn[cur] == 0) {
+        if (threadIdx.x == 0) {
+          const unsigned int gp = __hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (gp != last_global) { last_global = gp; idle = 0; } else { ++idle; }
+          if (idle > (1 << 18) || __hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(&ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_stop = 1;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        continue;
+      }
+    }
+    const int m = s_qn[cur];
+    for (int it = threadIdx.x; it < m; it += blockDim.x) {
+      const int l = qc[it];
+      const int v = t0 + l;
+      double s = rhs[v];
+      if (MODE == 0) {
+        const int e = doff[v + 1];
+        for (int c = doff[v]; c < e; ++c) s = s + dval[c] * x[don[c]];
+        x[v] = s;
+        for (int k = 0; k < NR; ++k) {
+          const int r = rcv[(int64_t)k * N + v];
+          if (r == v || r < 0 || coef[(int64_t)k * N + v] == 0.0) continue;
+          if (r >= t0 && r < t0 + n) {
+            if (atomicSub(&pend[r - t0], 1) == 1 &&
+                __hip_atomic_load(&ext[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                atomicCAS(&pend[r - t0], 0, -1000000) == 0)
+              qn[atomicAdd(&s_qn[cur ^ 1], 1)] = (short)(r - t0);
+          } else {
+            __hip_atomic_fetch_add(&ext[r], -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      } else {
+        for (int k = 0; k < NR; ++k) {
+          const int r = rcv[(int64_t)k * N + v];
+          const double o = coef[(int64_t)k * N + v];
+          if (r != v && o != 0.0) s = s - o * x[r];
+        }
+        x[v] = s / diag[v];
+        const int e = doff[v + 1];
+        for (int c = doff[v]; c < e; ++c) {
+          const int j = don[c];
+          if (j >= t0 && j < t0 + n) {
+            if (atomicSub(&pend[j - t0], 1) == 1 &&
+                __hip_atomic_load(&ext[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                atomicCAS(&pend[j - t0], 0, -1000000) == 0)
+              qn[atomicAdd(&s_qn[cur ^ 1], 1)] = (short)(j - t0);
+          } else {
+            __hip_atomic_fetch_add(&ext[j], -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_done += m;
+      s_qn[cur] = 0;
+      __hip_atomic_fetch_add(&ctl[0], (unsigned int)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      idle = 0;
+    }
+    cur ^= 1;
+  }
+}
+
 static int ensure_donors(gospl_mesh* m, int64_t cap) {
   if (m->wdon_cap >= cap) return 0;
   if (m->wdon) GOSPL_CHECK(hipFree(m->wdon));
@@ -287,14 +452,48 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   size_t bytes = m->cub_tmp_bytes;
   GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
   k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, m->wdval);
-  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, fa, counts);
-  GOSPL_CHECK(hipGetLastError());
   int64_t NN = N;
   int W = m->W;
   int NR = nrcv;
   int32_t* nlev = dev;
   int32_t* don = m->wdon;
   double* dval = m->wdval;
+  if (m->tile_S > 0) {
+    // tile-resident asynchronous sweep
+    int S = m->tile_S;
+    int32_t* ext = pend;
+    unsigned int* ctl = reinterpret_cast<unsigned int*>(dev + 48);
+    GOSPL_CHECK(hipMemsetAsync(ctl, 0, sizeof(unsigned int) * 4, s));
+    if (mode == 0)
+      k_tile_ext<0><<<g, GOSPL_BLOCK, 0, s>>>(N, NR, S, rcv, coef, doff, don, ext);
+    else
+      k_tile_ext<1><<<g, GOSPL_BLOCK, 0, s>>>(N, NR, S, rcv, coef, doff, don, ext);
+    GOSPL_CHECK(hipGetLastError());
+    void* targs[] = {&NN, &NR, &S, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
+                     &don, &dval, &ext, &ctl};
+    const void* tk = mode == 0 ? (const void*)k_dag_tile<0> : (const void*)k_dag_tile<1>;
+    const int nt = (int)((N + S - 1) / S);
+    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
+    GOSPL_CHECK(hipLaunchCooperativeKernel(tk, dim3(nt), dim3(GOSPL_BLOCK), targs,
+                                           (size_t)10 * S, s));
+    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 40, ctl, sizeof(unsigned int) * 2, hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    if (m->hflag[41] == 0 && m->hflag[40] == (int32_t)N) {
+      if (m->timing) {
+        float ms = 0.0f;
+        GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
+        m->t_ms[mode] += ms;
+        m->t_calls[mode] += 1;
+      }
+      if (nlevels) *nlevels = -1;
+      if (unsolved) *unsolved = 0;
+      return 0;
+    }
+    // no progress (a cycle in the receivers): fall through to the level sweep
+  }
+  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, fa, counts);
+  GOSPL_CHECK(hipGetLastError());
   void* args[] = {&NN, &W, &NR, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
                   &don, &dval, &pend, &fa, &fb, &counts, &bar, &nlev};
   const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
@@ -342,6 +541,30 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* env = getenv("GOSPL_DAG_BLOCKS");
   if (env && atoi(env) > 0) want = atoi(env);
   m->coop_blocks = want < maxb ? want : maxb;
+
+  // tile size for the tile-resident sweep: one tile per co-resident block,
+  // LDS = 10 bytes per tile vertex, tile-local indices in 16 bits
+  m->tile_S = 0;
+  const char* te = getenv("GOSPL_DAG_TILE");
+  if (!(te && atoi(te) == 0)) {
+    int64_t S = (m->N + m->num_cu - 1) / m->num_cu;
+    if (S < 1024) S = 1024;
+    if (S <= 16000) {
+      const size_t lds = (size_t)10 * S;
+      GOSPL_CHECK(hipFuncSetAttribute((const void*)k_dag_tile<0>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      GOSPL_CHECK(hipFuncSetAttribute((const void*)k_dag_tile<1>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      int b0 = 0, b1 = 0;
+      GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, (const void*)k_dag_tile<0>,
+                                                               GOSPL_BLOCK, lds));
+      GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, (const void*)k_dag_tile<1>,
+                                                               GOSPL_BLOCK, lds));
+      const int64_t nt = (m->N + S - 1) / S;
+      const int64_t cap = (int64_t)(b0 < b1 ? b0 : b1) * m->num_cu;
+      if (nt <= cap) m->tile_S = (int)S;
+    }
+  }
   return 0;
 }
 

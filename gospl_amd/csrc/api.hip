@@ -167,18 +167,24 @@ int gospl_ensure_coef(gospl_mesh* m, int rows) {
 
 int gospl_mesh_destroy(gospl_mesh_t* m) {
   if (!m) return 0;
-  for (int k = 0; k < 8; ++k) hipFree(m->wi[k]);
-  for (int k = 0; k < 2; ++k) hipFree(m->wu[k]);
-  for (int k = 0; k < 4; ++k) hipFree(m->wd[k]);
-  hipFree(m->counts);
-  hipFree(m->dflag);
-  hipHostFree(m->hflag);
-  hipFree(m->cub_tmp);
-  if (m->wcoef) hipFree(m->wcoef);
-  if (m->tev[0]) hipEventDestroy(m->tev[0]);
-  if (m->tev[1]) hipEventDestroy(m->tev[1]);
-  if (m->wdon) hipFree(m->wdon);
-  if (m->wdval) hipFree(m->wdval);
+  // best effort: release everything even if one call fails
+  auto dfree = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (int k = 0; k < 8; ++k) dfree(m->wi[k]);
+  for (int k = 0; k < 2; ++k) dfree(m->wu[k]);
+  for (int k = 0; k < 4; ++k) dfree(m->wd[k]);
+  dfree(m->counts);
+  dfree(m->dflag);
+  if (m->hflag) (void)hipHostFree(m->hflag);
+  dfree(m->cub_tmp);
+  dfree(m->wcoef);
+  if (m->tev[0]) (void)hipEventDestroy(m->tev[0]);
+  if (m->tev[1]) (void)hipEventDestroy(m->tev[1]);
+  dfree(m->wdon);
+  dfree(m->wacc);
+  dfree(m->wslot);
+  dfree(m->trace_buf);
   delete m;
   return 0;
 }

@@ -24,6 +24,9 @@ struct gospl_mesh {
   int device;
   int num_cu;
   int coop_blocks;            // co-resident blocks for the persistent DAG kernels
+  int dag_threads;            // threads per block of the persistent DAG kernel
+  int dag_interleave;         // frontier slices interleaved over blocks
+  int dag_tail;               // frontier size below which one workgroup finishes the sweep
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64
@@ -34,10 +37,10 @@ struct gospl_mesh {
   double* wcoef;              // (wcoef_rows, N) coefficient rows
   int wcoef_rows;
   int32_t* wdon;              // donor rows of the receiver DAG
-  double* wdval;
+  double* wacc;               // weighted donor values, one per donor entry
+  int32_t* wslot;             // (nrcv, N) slot of each (vertex, receiver) in its receiver's row
   int64_t wdon_cap;
-  int dag_chain;              // chaining in the DAG sweep (GOSPL_DAG_CHAIN)
-  int tile_S;                 // tile size of the tile-resident DAG sweep (0 = level sweep only)
+  long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
   int timing;                 // record HIP events around the DAG sweep kernels
   hipEvent_t tev[2];
   double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep

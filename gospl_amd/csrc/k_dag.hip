@@ -4,16 +4,21 @@
 //
 // Both matrices are triangular in a topological order of the receiver graph
 // (receivers are strictly lower, or closer to a spill point on flats), so
-// they are solved exactly by one sweep.  The sweep is level-synchronous in a
-// persistent cooperative kernel:
+// they are solved exactly by one sweep over its levels:
 //   * a donor list (reverse receiver graph, in FV neighbour order) is built
-//     first so each vertex pulls its dependencies from a contiguous row;
+//     first; for flow accumulation every donor entry has a slot, and a solved
+//     donor writes its weighted value w*q into its slot of each receiver's
+//     row, so a vertex finds all its inputs in one contiguous row;
 //   * pending counters hold the number of unsolved dependencies; a vertex
 //     whose counter reaches zero joins the next frontier (wavefront-
 //     aggregated appends: one atomic per wavefront, not per vertex);
-//   * one light grid barrier (atomic arrival counter + generation word)
-//     separates levels; the grid is small because the sweep is bound by the
-//     latency of each level, not by bandwidth.
+//   * level 0 (no dependencies) runs on the whole GPU; the following levels
+//     run in a small persistent cooperative kernel separated by a light grid
+//     barrier, and once the frontier is small, in one workgroup with LDS
+//     frontiers and workgroup barriers.
+// Past level 0 the sweep is bound by the latency of each level: the loads of a
+// vertex are issued together (unrolled over the stencil width) so one level
+// costs about three dependent memory round trips plus the barrier.
 // Every value is a fixed-order sum over its own row, so results do not depend
 // on the schedule (no floating-point atomics).
 #include <cstdlib>
@@ -46,10 +51,12 @@ __global__ void k_dep_count(int64_t N, int W, int NR, const int32_t* __restrict_
   dcnt[i] = c;
 }
 
+// Donor rows in neighbour order; slot[k][j] = position of (donor j, its
+// receiver k) in the receiver's row (-1 where there is none).
 __global__ void k_dep_fill(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                            const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
                            const double* __restrict__ coef, const int32_t* __restrict__ doff,
-                           int32_t* __restrict__ don, double* __restrict__ dval) {
+                           int32_t* __restrict__ don, int32_t* __restrict__ slot) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int c = doff[i];
@@ -58,9 +65,10 @@ __global__ void k_dep_fill(int64_t N, int W, int NR, const int32_t* __restrict__
     int j = nbr[(int64_t)p * N + i];
     if (j < 0) continue;
     for (int k = 0; k < NR; ++k) {
-      if (rcv[(int64_t)k * N + j] == (int)i) {
-        double w = coef[(int64_t)k * N + j];
-        if (w != 0.0) { don[c] = j; dval[c] = w; c++; }
+      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
+        don[c] = j;
+        if (slot) slot[(int64_t)k * N + j] = c;
+        c++;
       }
     }
   }
@@ -123,20 +131,174 @@ __device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int nbl
   return s_abort == 0;
 }
 
+#define DAG_BLOCK 1024
+
+// Operands of one sweep.
+struct DagArgs {
+  int64_t N;
+  int NR;
+  const int32_t* rcv;   // (NR, N)
+  const double* coef;   // (NR, N) weights (flow) or off-diagonal coefficients (SPL)
+  const double* diag;   // (N) SPL diagonal
+  const double* rhs;    // (N)
+  double* x;            // (N) solution
+  const int32_t* doff;  // (N + 1) donor row offsets
+  const int32_t* don;   // donor rows
+  double* acc;          // flow: weighted donor values, one per donor entry
+  const int32_t* slot;  // flow: (NR, N) slot of (vertex, receiver k) in acc
+  int32_t* pend;        // (N) unsolved dependencies
+};
+
+// Solve vertex v (if valid) and release its dependents into the next
+// frontier.  Reached by all lanes of a wavefront (the appends are
+// wavefront-aggregated).
+template <int MODE, class Append>
+__device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs& a,
+                                               Append append) {
+  const int64_t N = a.N;
+  const int NR = a.NR;
+  if (MODE == 0) {
+    // q_v = b_v + sum over donor entries (row order) of w * q_donor
+    int r[GOSPL_MAXW];
+    bool rel[GOSPL_MAXW];
+#pragma unroll
+    for (int k = 0; k < GOSPL_MAXW; ++k) {
+      r[k] = 0;
+      rel[k] = false;
+    }
+    if (valid) {
+      double w[GOSPL_MAXW];
+      int sl[GOSPL_MAXW];
+      const int b = a.doff[v];
+      const int e = a.doff[v + 1];
+      double s = a.rhs[v];
+#pragma unroll
+      for (int k = 0; k < GOSPL_MAXW; ++k) {
+        if (k < NR) {
+          r[k] = a.rcv[(int64_t)k * N + v];
+          w[k] = a.coef[(int64_t)k * N + v];
+          sl[k] = a.slot[(int64_t)k * N + v];
+        }
+      }
+      double cv[GOSPL_MAXW];
+#pragma unroll
+      for (int c = 0; c < GOSPL_MAXW; ++c)
+        if (b + c < e) cv[c] = a.acc[b + c];
+#pragma unroll
+      for (int c = 0; c < GOSPL_MAXW; ++c)
+        if (b + c < e) s = s + cv[c];
+      for (int c = b + GOSPL_MAXW; c < e; ++c) s = s + a.acc[c];
+      a.x[v] = s;
+#pragma unroll
+      for (int k = 0; k < GOSPL_MAXW; ++k) {
+        if (k < NR && r[k] != v && r[k] >= 0 && w[k] != 0.0) {
+          if (sl[k] >= 0) a.acc[sl[k]] = w[k] * s;
+          rel[k] = atomicSub(&a.pend[r[k]], 1) == 1;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < GOSPL_MAXW; ++k)
+      if (k < NR) append(rel[k], r[k]);
+  } else {
+    // x_v = (h_v - sum_k off_k x_{rcv_k}) / diag_v, then release the donors
+    int j[GOSPL_MAXW];
+    bool rel[GOSPL_MAXW];
+#pragma unroll
+    for (int c = 0; c < GOSPL_MAXW; ++c) {
+      j[c] = 0;
+      rel[c] = false;
+    }
+    int b = 0, e = 0;
+    if (valid) {
+      int rr[GOSPL_MAXW];
+      double o[GOSPL_MAXW], xr[GOSPL_MAXW];
+      b = a.doff[v];
+      e = a.doff[v + 1];
+      double s = a.rhs[v];
+      const double dg = a.diag[v];
+#pragma unroll
+      for (int k = 0; k < GOSPL_MAXW; ++k) {
+        if (k < NR) {
+          rr[k] = a.rcv[(int64_t)k * N + v];
+          o[k] = a.coef[(int64_t)k * N + v];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < GOSPL_MAXW; ++c)
+        if (b + c < e) j[c] = a.don[b + c];
+#pragma unroll
+      for (int k = 0; k < GOSPL_MAXW; ++k)
+        if (k < NR && rr[k] != v && o[k] != 0.0) xr[k] = a.x[rr[k]];
+#pragma unroll
+      for (int k = 0; k < GOSPL_MAXW; ++k)
+        if (k < NR && rr[k] != v && o[k] != 0.0) s = s - o[k] * xr[k];
+      a.x[v] = s / dg;
+#pragma unroll
+      for (int c = 0; c < GOSPL_MAXW; ++c)
+        if (b + c < e) rel[c] = atomicSub(&a.pend[j[c]], 1) == 1;
+    }
+#pragma unroll
+    for (int c = 0; c < GOSPL_MAXW; ++c) append(rel[c], j[c]);
+    // rows longer than the stencil width (not produced by an FV mesh)
+    for (int c = GOSPL_MAXW; __ballot(b + c < e) != 0ull; ++c) {
+      bool rl = false;
+      int jj = 0;
+      if (b + c < e) {
+        jj = a.don[b + c];
+        rl = atomicSub(&a.pend[jj], 1) == 1;
+      }
+      append(rl, jj);
+    }
+  }
+}
+
+// Appends into a frontier list in global memory.
+struct GlobalAppend {
+  int32_t* cnt;
+  int32_t* out;
+  __device__ __forceinline__ void operator()(bool pred, int value) const {
+    wave_append(pred, value, cnt, out);
+  }
+};
+
+// Level 0 (vertices without dependencies, typically a third of the mesh) on
+// the whole GPU before the persistent sweep takes over at level 1.
 template <int MODE>
-__global__ __launch_bounds__(GOSPL_BLOCK) void k_dag_level(
-    int64_t N, int W, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-    const double* __restrict__ diag, const double* __restrict__ rhs, double* x,
-    const int32_t* __restrict__ doff, const int32_t* __restrict__ don,
-    const double* __restrict__ dval, int32_t* __restrict__ pend, int32_t* __restrict__ fa,
-    int32_t* __restrict__ fb, int32_t* __restrict__ counts, unsigned int* __restrict__ bar,
-    int32_t* __restrict__ nlev) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_dag_level0(DagArgs a, const int32_t* __restrict__ fa, int32_t* __restrict__ fb,
+                             int32_t* __restrict__ counts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = counts[0];
+  if (t - (t & 63) >= n) return;  // whole wavefront beyond the frontier
+  const bool valid = t < n;
+  const int v = valid ? fa[t] : 0;
+  process_vertex<MODE>(valid, v, a, GlobalAppend{&counts[1], fb});
+}
+
+// Levels 1.. on a small co-resident grid.  Stops at the first level whose
+// frontier is empty or smaller than tail_n (k_dag_tail continues there);
+// *nlev = that level.
+template <int MODE>
+__global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
+    DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb, int32_t* __restrict__ counts,
+    unsigned int* __restrict__ bar, int32_t* __restrict__ nlev, long long* __restrict__ trace,
+    int interleave, int tail_n) {
+  // Wavefront w of block b takes frontier slice (w * gridDim.x + b) when
+  // interleaving, so a small frontier spreads over all CUs instead of filling
+  // the first blocks.
+  const int64_t wave = interleave
+                           ? (int64_t)(threadIdx.x >> 6) * gridDim.x + blockIdx.x
+                           : (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t tid = wave * 64 + (threadIdx.x & 63);
   const int64_t nth = (int64_t)gridDim.x * blockDim.x;
-  int level = 0;
+  int level = 1;  // level 0 ran in k_dag_level0
   for (;;) {
     const int n = load_count(&counts[level]);
-    if (n == 0) break;
+    if (trace && tid == 0 && level < 8192) {
+      trace[2 * level] = (long long)wall_clock64();
+      trace[2 * level + 1] = n;
+    }
+    if (n < tail_n || n == 0) break;
     int32_t* cur = (level & 1) ? fb : fa;
     int32_t* nxt = (level & 1) ? fa : fb;
     int32_t* cnt = &counts[level + 1];
@@ -145,52 +307,88 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_dag_level(
       const int64_t t = t0 + (tid & 63);
       const bool valid = t < n;
       const int v = valid ? cur[t] : 0;
-      if (MODE == 0) {
-        int r[GOSPL_MAXW];
-        double w[GOSPL_MAXW];
-        if (valid) {
-          for (int k = 0; k < NR; ++k) {
-            r[k] = rcv[(int64_t)k * N + v];
-            w[k] = coef[(int64_t)k * N + v];
-          }
-          double s = rhs[v];
-          const int e = doff[v + 1];
-          for (int c = doff[v]; c < e; ++c) s = s + dval[c] * x[don[c]];
-          x[v] = s;
-        }
-        for (int k = 0; k < NR; ++k) {
-          bool rel = false;
-          if (valid && r[k] != v && r[k] >= 0 && w[k] != 0.0) rel = atomicSub(&pend[r[k]], 1) == 1;
-          wave_append(rel, valid ? r[k] : 0, cnt, nxt);
-        }
-      } else {
-        int b = 0, e = 0;
-        if (valid) {
-          double s = rhs[v];
-          for (int k = 0; k < NR; ++k) {
-            int rr = rcv[(int64_t)k * N + v];
-            double o = coef[(int64_t)k * N + v];
-            if (rr != v && o != 0.0) s = s - o * x[rr];
-          }
-          x[v] = s / diag[v];
-          b = doff[v];
-          e = doff[v + 1];
-        }
-        for (int c = 0; c < W; ++c) {
-          bool rel = false;
-          int j = 0;
-          if (valid && b + c < e) {
-            j = don[b + c];
-            rel = atomicSub(&pend[j], 1) == 1;
-          }
-          wave_append(rel, j, cnt, nxt);
-        }
-      }
+      process_vertex<MODE>(valid, v, a, GlobalAppend{cnt, nxt});
     }
     if (!grid_barrier(bar, gridDim.x)) break;
     level++;
   }
   if (tid == 0) *nlev = level;
+}
+
+#define TAIL_THREADS 1024
+#define TAIL_Q 8192
+
+// Appends into the tail's next frontier: slots below TAIL_Q in LDS, the rest
+// in the global list at the same index.
+struct TailAppend {
+  int32_t* cnt;  // LDS counter
+  int32_t* lq;   // LDS list
+  int32_t* gq;   // global list (capacity N)
+  __device__ __forceinline__ void operator()(bool pred, int value) const {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) {
+      const int idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+      if (idx < TAIL_Q) lq[idx] = value;
+      else gq[idx] = value;
+    }
+  }
+};
+
+// The remaining levels of the sweep on one workgroup, from the level where
+// k_dag_level stopped (*nlev).  Frontiers live in LDS and levels are separated
+// by workgroup barriers instead of a device-wide barrier with its cache
+// write-back/invalidate.
+template <int MODE>
+__global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
+    DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb,
+    const int32_t* __restrict__ counts, const unsigned int* __restrict__ bar,
+    int32_t* __restrict__ nlev, long long* __restrict__ trace) {
+  __shared__ int32_t sq[2][TAIL_Q];
+  __shared__ int32_t sc[3];
+  if (bar[2]) return;  // the multi-block phase aborted
+  int level = *nlev;
+  int n = counts[level];
+  if (threadIdx.x < 3) sc[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  bool first = true;  // the first frontier is entirely in the global list
+  while (n > 0) {
+    if (trace && threadIdx.x == 0 && level < 8192) {
+      trace[2 * level] = (long long)wall_clock64();
+      trace[2 * level + 1] = n;
+    }
+    const int p = level & 1;
+    const int32_t* gcur = p ? fb : fa;
+    int32_t* gnxt = p ? fa : fb;
+    // counters rotate over three slots: this level appends to (level+1)%3 and
+    // clears (level+2)%3, which every thread read before the previous barrier
+    if (threadIdx.x == 0) sc[(level + 2) % 3] = 0;
+    const TailAppend app{&sc[(level + 1) % 3], sq[p ^ 1], gnxt};
+    for (int t0 = threadIdx.x - lane; t0 < n; t0 += blockDim.x) {
+      const int t = t0 + lane;
+      const bool valid = t < n;
+      int v = 0;
+      if (valid) v = (first || t >= TAIL_Q) ? gcur[t] : sq[p][t];
+      process_vertex<MODE>(valid, v, a, app);
+    }
+    __syncthreads();
+    n = sc[(level + 1) % 3];
+    first = false;
+    level++;
+  }
+  if (threadIdx.x == 0) {
+    if (trace && level < 8192) {
+      trace[2 * level] = (long long)wall_clock64();
+      trace[2 * level + 1] = 0;
+    }
+    *nlev = level;
+  }
 }
 
 // Vertices never reached (a cycle in the receivers) -> 0, counted.
@@ -253,179 +451,17 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
-// ---------------------------------------------------------------------------
-// Tile-resident asynchronous sweep.
-//
-// Block b owns the Morton tile [b*S, min((b+1)*S, N)).  Dependencies inside
-// the tile are counted in LDS and released with LDS atomics; a vertex whose
-// in-tile counter reaches zero is queued in LDS if its out-of-tile counter
-// (global, `ext`) is already zero.  Dependencies on other tiles are released
-// with a global atomic carrying release semantics; the owning block finds
-// such vertices by polling its boundary list (vertices that had out-of-tile
-// dependencies) with acquire loads.  No grid barrier: a hop inside a tile
-// costs a block barrier and a few L2 hits.  All blocks are co-resident
-// (cooperative launch); a block that sees no global progress for a long
-// time raises `abort` (then every block leaves and the host falls back to the
-// level-synchronous sweep, which also handles cycles).
-// ---------------------------------------------------------------------------
-
-template <int MODE>
-__global__ void k_tile_ext(int64_t N, int NR, int S, const int32_t* __restrict__ rcv,
-                           const double* __restrict__ coef, const int32_t* __restrict__ doff,
-                           const int32_t* __restrict__ don, int32_t* __restrict__ ext) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= N) return;
-  const int tv = (int)(v / S);
-  int c = 0;
-  if (MODE == 0) {
-    for (int e = doff[v]; e < doff[v + 1]; ++e)
-      if (don[e] / S != tv) c++;
-  } else {
-    for (int k = 0; k < NR; ++k) {
-      int r = rcv[(int64_t)k * N + v];
-      if (r != (int)v && coef[(int64_t)k * N + v] != 0.0 && r / S != tv) c++;
-    }
-  }
-  ext[v] = c;
-}
-
-template <int MODE>
-__global__ __launch_bounds__(GOSPL_BLOCK) void k_dag_tile(
-    int64_t N, int NR, int S, const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-    const double* __restrict__ diag, const double* __restrict__ rhs, double* x,
-    const int32_t* __restrict__ doff, const int32_t* __restrict__ don,
-    const double* __restrict__ dval, int32_t* ext, unsigned int* __restrict__ ctl) {
-  // ctl[0] global progress (vertices done), ctl[1] abort
-  extern __shared__ int smem[];
-  int* pend = smem;                                           // S ints
-  short* q0 = reinterpret_cast<short*>(pend + S);             // S shorts
-  short* q1 = q0 + S;                                         // S shorts
-  short* bnd = q1 + S;                                        // S shorts (boundary list)
-  __shared__ int s_qn[2], s_nb, s_done, s_stop;
-  const int t0 = blockIdx.x * S;
-  const int n = (int)((int64_t)t0 + S <= N ? S : N - t0);
-  if (threadIdx.x == 0) { s_qn[0] = 0; s_qn[1] = 0; s_nb = 0; s_done = 0; s_stop = 0; }
-  __syncthreads();
-  // in-tile dependency counts, initial queue, boundary list
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
-    const int v = t0 + l;
-    int c = 0;
-    if (MODE == 0) {
-      for (int e = doff[v]; e < doff[v + 1]; ++e) {
-        int j = don[e];
-        if (j >= t0 && j < t0 + n) c++;
-      }
-    } else {
-      for (int k = 0; k < NR; ++k) {
-        int r = rcv[(int64_t)k * N + v];
-        if (r != v && coef[(int64_t)k * N + v] != 0.0 && r >= t0 && r < t0 + n) c++;
-      }
-    }
-    const int e = ext[v];
-    pend[l] = c;
-    if (e > 0) bnd[atomicAdd(&s_nb, 1)] = (short)l;
-    if (c == 0 && e == 0) {
-      pend[l] = -1000000;
-      q0[atomicAdd(&s_qn[0], 1)] = (short)l;
-    }
-  }
-  __syncthreads();
-  int cur = 0;
-  unsigned int last_global = 0;
-  int idle = 0;
-  while (true) {
-    __syncthreads();
-    if (s_done >= n || s_stop) break;
-    short* qc = cur ? q1 : q0;
-    short* qn = cur ? q0 : q1;
-    if (s_qn[cur] == 0) {
-      // poll the boundary list for vertices released by other tiles
-      for (int b = threadIdx.x; b < s_nb; b += blockDim.x) {
-        const int l = bnd[b];
-        if (l < 0) continue;
-        if (pend[l] == 0 &&
-            __hip_atomic_load(&ext[t0 + l], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          if (atomicCAS(&pend[l], 0, -1000000) == 0) {
-            qc[atomicAdd(&s_qn[cur], 1)] = (short)l;
-            bnd[b] = -1;
-          }
-        }
-      }
-      __syncthreads();
-      if (s_qn[cur] == 0) {
-        if (threadIdx.x == 0) {
-          const unsigned int gp = __hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (gp != last_global) { last_global = gp; idle = 0; } else { ++idle; }
-          if (idle > (1 << 18) || __hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            __hip_atomic_store(&ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_stop = 1;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-        continue;
-      }
-    }
-    const int m = s_qn[cur];
-    for (int it = threadIdx.x; it < m; it += blockDim.x) {
-      const int l = qc[it];
-      const int v = t0 + l;
-      double s = rhs[v];
-      if (MODE == 0) {
-        const int e = doff[v + 1];
-        for (int c = doff[v]; c < e; ++c) s = s + dval[c] * x[don[c]];
-        x[v] = s;
-        for (int k = 0; k < NR; ++k) {
-          const int r = rcv[(int64_t)k * N + v];
-          if (r == v || r < 0 || coef[(int64_t)k * N + v] == 0.0) continue;
-          if (r >= t0 && r < t0 + n) {
-            if (atomicSub(&pend[r - t0], 1) == 1 &&
-                __hip_atomic_load(&ext[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-                atomicCAS(&pend[r - t0], 0, -1000000) == 0)
-              qn[atomicAdd(&s_qn[cur ^ 1], 1)] = (short)(r - t0);
-          } else {
-            __hip_atomic_fetch_add(&ext[r], -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
-      } else {
-        for (int k = 0; k < NR; ++k) {
-          const int r = rcv[(int64_t)k * N + v];
-          const double o = coef[(int64_t)k * N + v];
-          if (r != v && o != 0.0) s = s - o * x[r];
-        }
-        x[v] = s / diag[v];
-        const int e = doff[v + 1];
-        for (int c = doff[v]; c < e; ++c) {
-          const int j = don[c];
-          if (j >= t0 && j < t0 + n) {
-            if (atomicSub(&pend[j - t0], 1) == 1 &&
-                __hip_atomic_load(&ext[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-                atomicCAS(&pend[j - t0], 0, -1000000) == 0)
-              qn[atomicAdd(&s_qn[cur ^ 1], 1)] = (short)(j - t0);
-          } else {
-            __hip_atomic_fetch_add(&ext[j], -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      s_done += m;
-      s_qn[cur] = 0;
-      __hip_atomic_fetch_add(&ctl[0], (unsigned int)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      idle = 0;
-    }
-    cur ^= 1;
-  }
-}
-
 static int ensure_donors(gospl_mesh* m, int64_t cap) {
   if (m->wdon_cap >= cap) return 0;
   if (m->wdon) GOSPL_CHECK(hipFree(m->wdon));
-  if (m->wdval) GOSPL_CHECK(hipFree(m->wdval));
+  if (m->wacc) GOSPL_CHECK(hipFree(m->wacc));
+  if (m->wslot) GOSPL_CHECK(hipFree(m->wslot));
   m->wdon = nullptr;
-  m->wdval = nullptr;
+  m->wacc = nullptr;
+  m->wslot = nullptr;
   GOSPL_CHECK(hipMalloc(&m->wdon, sizeof(int32_t) * (cap + 64)));
-  GOSPL_CHECK(hipMalloc(&m->wdval, sizeof(double) * (cap + 64)));
+  GOSPL_CHECK(hipMalloc(&m->wacc, sizeof(double) * (cap + 64)));
+  GOSPL_CHECK(hipMalloc(&m->wslot, sizeof(int32_t) * (cap + 64)));
   m->wdon_cap = cap;
   return 0;
 }
@@ -447,58 +483,41 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
   unsigned int* bar = reinterpret_cast<unsigned int*>(dev + 32);
   GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned int) * 4, s));
+  int32_t* slot = mode == 0 ? m->wslot : nullptr;
+  if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
   // donor rows
   k_dep_count<<<nblocks(N + 1), GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, dcnt);
   size_t bytes = m->cub_tmp_bytes;
   GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
-  k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, m->wdval);
-  int64_t NN = N;
-  int W = m->W;
-  int NR = nrcv;
-  int32_t* nlev = dev;
-  int32_t* don = m->wdon;
-  double* dval = m->wdval;
-  if (m->tile_S > 0) {
-    // tile-resident asynchronous sweep
-    int S = m->tile_S;
-    int32_t* ext = pend;
-    unsigned int* ctl = reinterpret_cast<unsigned int*>(dev + 48);
-    GOSPL_CHECK(hipMemsetAsync(ctl, 0, sizeof(unsigned int) * 4, s));
-    if (mode == 0)
-      k_tile_ext<0><<<g, GOSPL_BLOCK, 0, s>>>(N, NR, S, rcv, coef, doff, don, ext);
-    else
-      k_tile_ext<1><<<g, GOSPL_BLOCK, 0, s>>>(N, NR, S, rcv, coef, doff, don, ext);
-    GOSPL_CHECK(hipGetLastError());
-    void* targs[] = {&NN, &NR, &S, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
-                     &don, &dval, &ext, &ctl};
-    const void* tk = mode == 0 ? (const void*)k_dag_tile<0> : (const void*)k_dag_tile<1>;
-    const int nt = (int)((N + S - 1) / S);
-    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
-    GOSPL_CHECK(hipLaunchCooperativeKernel(tk, dim3(nt), dim3(GOSPL_BLOCK), targs,
-                                           (size_t)10 * S, s));
-    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
-    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 40, ctl, sizeof(unsigned int) * 2, hipMemcpyDeviceToHost, s));
-    GOSPL_CHECK(hipStreamSynchronize(s));
-    if (m->hflag[41] == 0 && m->hflag[40] == (int32_t)N) {
-      if (m->timing) {
-        float ms = 0.0f;
-        GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
-        m->t_ms[mode] += ms;
-        m->t_calls[mode] += 1;
-      }
-      if (nlevels) *nlevels = -1;
-      if (unsolved) *unsolved = 0;
-      return 0;
-    }
-    // no progress (a cycle in the receivers): fall through to the level sweep
-  }
+  k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
   k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, fa, counts);
   GOSPL_CHECK(hipGetLastError());
-  void* args[] = {&NN, &W, &NR, (void*)&rcv, (void*)&coef, (void*)&diag, (void*)&rhs, &x, &doff,
-                  &don, &dval, &pend, &fa, &fb, &counts, &bar, &nlev};
+  DagArgs a{N, nrcv, rcv, coef, diag, rhs, x, doff, m->wdon, m->wacc, slot, pend};
+  int32_t* nlev = dev;
+  // GOSPL_DAG_TRACE=1: per-level wall-clock stamps and frontier sizes, summarised on stderr
+  static const bool do_trace = getenv("GOSPL_DAG_TRACE") && atoi(getenv("GOSPL_DAG_TRACE")) > 0;
+  long long* trace = nullptr;
+  if (do_trace) {
+    if (!m->trace_buf) GOSPL_CHECK(hipMalloc(&m->trace_buf, sizeof(long long) * 2 * 8192));
+    trace = m->trace_buf;
+    GOSPL_CHECK(hipMemsetAsync(trace, 0, sizeof(long long) * 2 * 8192, s));
+  }
+  void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
   const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
-  GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(GOSPL_BLOCK), args, 0, s));
+  if (mode == 0)
+    k_dag_level0<0><<<g, GOSPL_BLOCK, 0, s>>>(a, fa, fb, counts);
+  else
+    k_dag_level0<1><<<g, GOSPL_BLOCK, 0, s>>>(a, fa, fb, counts);
+  GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
+  if (m->dag_tail > 0) {
+    if (mode == 0)
+      k_dag_tail<0><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+    else
+      k_dag_tail<1><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+    GOSPL_CHECK(hipGetLastError());
+  }
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1);
   GOSPL_CHECK(hipGetLastError());
@@ -515,6 +534,23 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     m->t_ms[mode] += ms;
     m->t_calls[mode] += 1;
   }
+  if (trace) {
+    static long long h[2 * 8192];
+    GOSPL_CHECK(hipMemcpy(h, trace, sizeof(h), hipMemcpyDeviceToHost));
+    const int L = m->hflag[0] < 8191 ? m->hflag[0] : 8191;
+    double tb[4] = {0, 0, 0, 0};
+    long long nb[4] = {0, 0, 0, 0};
+    for (int l = 1; l < L; ++l) {  // level 0 runs in its own kernel
+      const double us = (h[2 * (l + 1)] - h[2 * l]) * 0.01;  // 100 MHz counter
+      const long long n = h[2 * l + 1];
+      const int b = n >= 100000 ? 0 : n >= 10000 ? 1 : n >= 1000 ? 2 : 3;
+      tb[b] += us;
+      nb[b] += 1;
+    }
+    fprintf(stderr, "[dag mode %d] levels %d: >=1e5: %lld lv %.0f us | 1e4-1e5: %lld lv %.0f us | "
+            "1e3-1e4: %lld lv %.0f us | <1e3: %lld lv %.0f us\n", mode, L, nb[0], tb[0], nb[1], tb[1],
+            nb[2], tb[2], nb[3], tb[3]);
+  }
   if (nlevels) *nlevels = m->hflag[0];
   if (unsolved) *unsolved = m->hflag[1];
   return 0;
@@ -525,46 +561,36 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
 using namespace gospl;
 
 extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
+  // Launch shape of the sweep past level 0 (env overrides for tuning):
+  // GOSPL_DAG_BLOCKS x GOSPL_DAG_THREADS co-resident threads, frontier slices
+  // interleaved over blocks, frontiers below GOSPL_DAG_TAIL on one workgroup.
+  m->dag_threads = 256;
+  const char* te_threads = getenv("GOSPL_DAG_THREADS");
+  if (te_threads) {
+    const int t = atoi(te_threads);
+    if (t >= 64 && t <= DAG_BLOCK && t % 64 == 0) m->dag_threads = t;
+  }
+  m->dag_interleave = 1;
+  const char* te_il = getenv("GOSPL_DAG_INTERLEAVE");
+  if (te_il) m->dag_interleave = atoi(te_il) > 0;
+  m->dag_tail = 256;
+  const char* te_tail = getenv("GOSPL_DAG_TAIL");
+  if (te_tail && atoi(te_tail) >= 0) m->dag_tail = atoi(te_tail);
   int per_cu = 1 << 30;
   const void* ks[2] = {(const void*)k_dag_level<0>, (const void*)k_dag_level<1>};
   for (int i = 0; i < 2; ++i) {
     int b = 0;
-    GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ks[i], GOSPL_BLOCK, 0));
+    GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ks[i], m->dag_threads, 0));
     per_cu = b < per_cu ? b : per_cu;
   }
   if (per_cu < 1) per_cu = 1;
   const int maxb = per_cu * m->num_cu;
-  // Few blocks: the sweep is bound by per-level latency (dependent loads and
-  // the grid barrier, ~3 us at 32 blocks vs ~18 us at 256 on MI355X), not by
-  // bandwidth.  Measured best on the 2.6M-vertex sphere: 32.
+  // Few blocks: the grid barrier costs ~3 us at 32 blocks and ~18 us at 256 on
+  // MI355X, and the frontiers past level 0 hold a few thousand vertices.
   int want = 32;
   const char* env = getenv("GOSPL_DAG_BLOCKS");
   if (env && atoi(env) > 0) want = atoi(env);
   m->coop_blocks = want < maxb ? want : maxb;
-
-  // tile size for the tile-resident sweep: one tile per co-resident block,
-  // LDS = 10 bytes per tile vertex, tile-local indices in 16 bits
-  m->tile_S = 0;
-  const char* te = getenv("GOSPL_DAG_TILE");
-  if (!(te && atoi(te) == 0)) {
-    int64_t S = (m->N + m->num_cu - 1) / m->num_cu;
-    if (S < 1024) S = 1024;
-    if (S <= 16000) {
-      const size_t lds = (size_t)10 * S;
-      GOSPL_CHECK(hipFuncSetAttribute((const void*)k_dag_tile<0>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      GOSPL_CHECK(hipFuncSetAttribute((const void*)k_dag_tile<1>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      int b0 = 0, b1 = 0;
-      GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, (const void*)k_dag_tile<0>,
-                                                               GOSPL_BLOCK, lds));
-      GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, (const void*)k_dag_tile<1>,
-                                                               GOSPL_BLOCK, lds));
-      const int64_t nt = (m->N + S - 1) / S;
-      const int64_t cap = (int64_t)(b0 < b1 ? b0 : b1) * m->num_cu;
-      if (nt <= cap) m->tile_S = (int)S;
-    }
-  }
   return 0;
 }
 

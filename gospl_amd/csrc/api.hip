@@ -29,16 +29,6 @@ __global__ void k_rain_source(int64_t N, const double* __restrict__ rain,
   b[i] = (lfill[i] <= sl) ? 0.0 : rain[i] * area[i];
 }
 
-// inV[p] += fa over lsink cells of pit p (flowplex.py:596-600).
-__global__ void k_pit_inflow(int64_t N, const double* __restrict__ fa,
-                             const int8_t* __restrict__ lsink, const int32_t* __restrict__ pit,
-                             int64_t npit, double* __restrict__ inv) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int32_t p = pit[i];
-  if (lsink[i] && p > -1 && p < npit) atomicAdd(&inv[p], fa[i]);
-}
-
 // Water level of pits (flowplex.py:624-653).
 __global__ void k_pit_water(int64_t N, const int32_t* __restrict__ pit,
                             const int8_t* __restrict__ full, const double* __restrict__ level,
@@ -144,9 +134,11 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
   GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
   GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
-  // scan workspace sized for N ints
-  size_t bytes = 0;
+  // hipcub workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
+  size_t bytes = 0, b2 = 0;
   GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, m->wi[0], m->wi[1], (int)(N + 1)));
+  if (pit_tmp_bytes(N, &b2)) return 1;
+  if (b2 > bytes) bytes = b2;
   m->cub_tmp_bytes = bytes + 256;
   GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
 
@@ -185,6 +177,9 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->wacc);
   dfree(m->wslot);
   dfree(m->trace_buf);
+  dfree(m->pg_keys[0]);
+  dfree(m->pg_keys[1]);
+  dfree(m->pg_seg);
   delete m;
   return 0;
 }
@@ -192,14 +187,6 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
 int gospl_rain_source(gospl_mesh_t* m, const double* rain, const double* lfill, double sl,
                       double* b, gospl_stream_t stream) {
   k_rain_source<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, rain, m->area, lfill, sl, b);
-  GOSPL_CHECK(hipGetLastError());
-  return 0;
-}
-
-int gospl_pit_inflow(gospl_mesh_t* m, const double* fa, const int8_t* lsink, const int32_t* pitids,
-                     int64_t npit, double* inv, gospl_stream_t stream) {
-  GOSPL_CHECK(hipMemsetAsync(inv, 0, sizeof(double) * (npit > 0 ? npit : 1), stream));
-  k_pit_inflow<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, fa, lsink, pitids, npit, inv);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

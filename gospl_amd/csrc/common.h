@@ -27,6 +27,7 @@ struct gospl_mesh {
   int dag_threads;            // threads per block of the persistent DAG kernel
   int dag_interleave;         // frontier slices interleaved over blocks
   int dag_tail;               // frontier size below which one workgroup finishes the sweep
+  int dag_coop;               // launch the persistent DAG kernel with the cooperative API
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64
@@ -41,6 +42,13 @@ struct gospl_mesh {
   int32_t* wslot;             // (nrcv, N) slot of each (vertex, receiver) in its receiver's row
   int64_t wdon_cap;
   long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
+  // cells grouped by depression (sorted (pit id << 32 | cell) keys + runs)
+  unsigned long long* pg_keys[2];
+  int32_t* pg_seg;            // start (pg_cap + 64) then end (pg_cap + 64)
+  int64_t pg_cap;
+  const int32_t* pg_ids;      // pit-id buffer the grouping was built from
+  int64_t pg_npits;
+  int64_t pg_n;               // grouped cells
   int timing;                 // record HIP events around the DAG sweep kernels
   hipEvent_t tev[2];
   double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep
@@ -52,6 +60,14 @@ struct gospl_mesh {
 namespace gospl {
 
 void set_error(const char* fmt, ...);
+
+// depression grouping (k_pits.hip)
+int pit_tmp_bytes(int64_t N, size_t* bytes);  // hipcub workspace of the grouping
+int pit_cells(gospl_mesh* m, const int32_t* pitids, int32_t* cells, int32_t* ncells, hipStream_t s);
+int pit_group(gospl_mesh* m, const int32_t* pitids, int64_t npits, const int32_t* cells,
+              const int32_t* ncells_dev, int64_t ncells, hipStream_t s);
+int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double sl, int64_t cap,
+                     double* pitparams, double* filled_lvl, double* filled_vol, hipStream_t s);
 }  // namespace gospl
 
 extern "C" {

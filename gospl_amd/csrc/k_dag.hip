@@ -74,26 +74,23 @@ __global__ void k_dep_fill(int64_t N, int W, int NR, const int32_t* __restrict__
   }
 }
 
-// pending counters + frontier 0
+// Pending counters.  Level 0 (pend == 0) is not listed: k_dag_level0 scans
+// all vertices, which avoids a million appends on one counter.
 __global__ void k_pending(int mode, int64_t N, int NR, const int32_t* __restrict__ rcv,
                           const double* __restrict__ coef, const int32_t* __restrict__ doff,
-                          int32_t* __restrict__ pend, int32_t* __restrict__ front,
-                          int32_t* __restrict__ counts) {
+                          int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < N;
+  if (i >= N) return;
   int c = 0;
-  if (valid) {
-    if (mode == 0) {
-      c = doff[i + 1] - doff[i];
-    } else {
-      for (int k = 0; k < NR; ++k) {
-        int r = rcv[(int64_t)k * N + i];
-        if (r != (int)i && coef[(int64_t)k * N + i] != 0.0) c++;
-      }
+  if (mode == 0) {
+    c = doff[i + 1] - doff[i];
+  } else {
+    for (int k = 0; k < NR; ++k) {
+      int r = rcv[(int64_t)k * N + i];
+      if (r != (int)i && coef[(int64_t)k * N + i] != 0.0) c++;
     }
-    pend[i] = c;
   }
-  wave_append(valid && c == 0, (int)i, &counts[0], front);
+  pend[i] = c;
 }
 
 // Grid barrier (all blocks co-resident: cooperative launch).  bar[0]
@@ -263,15 +260,26 @@ struct GlobalAppend {
 };
 
 // Level 0 (vertices without dependencies, typically a third of the mesh) on
-// the whole GPU before the persistent sweep takes over at level 1.
+// the whole GPU before the persistent sweep takes over at level 1.  The
+// dependency count is recomputed from the rows (pend[] is being decremented
+// by other threads).  Same-address atomics serialise (~25 ns each on MI355X),
+// so the level-0 size is not counted.
 template <int MODE>
-__global__ void k_dag_level0(DagArgs a, const int32_t* __restrict__ fa, int32_t* __restrict__ fb,
-                             int32_t* __restrict__ counts) {
+__global__ void k_dag_level0(DagArgs a, int32_t* __restrict__ fb, int32_t* __restrict__ counts) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = counts[0];
-  if (t - (t & 63) >= n) return;  // whole wavefront beyond the frontier
-  const bool valid = t < n;
-  const int v = valid ? fa[t] : 0;
+  if (t - (t & 63) >= a.N) return;  // whole wavefront beyond the mesh
+  bool valid = t < a.N;
+  const int v = (int)t;
+  if (valid) {
+    if (MODE == 0) {
+      valid = a.doff[v + 1] == a.doff[v];
+    } else {
+      for (int k = 0; k < a.NR && valid; ++k) {
+        const int r = a.rcv[(int64_t)k * a.N + v];
+        if (r != v && a.coef[(int64_t)k * a.N + v] != 0.0) valid = false;
+      }
+    }
+  }
   process_vertex<MODE>(valid, v, a, GlobalAppend{&counts[1], fb});
 }
 
@@ -391,15 +399,88 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
   }
 }
 
-// Vertices never reached (a cycle in the receivers) -> 0, counted.
+// Vertices never reached (a cycle in the receivers and everything below it):
+// x = 0, listed.
 __global__ void k_dag_unsolved(int64_t N, const int32_t* __restrict__ pend, double* __restrict__ x,
-                               int32_t* __restrict__ cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  if (pend[i] > 0) {
-    x[i] = 0.0;
-    atomicAdd(cnt, 1);
+                               int32_t* __restrict__ cnt, int32_t* __restrict__ list) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool left = i < N && pend[i] > 0;
+  if (left) x[i] = 0.0;
+  wave_append(left, (int)i, cnt, list);
+}
+
+#define CYCLE_THREADS 1024
+#define CYCLE_MAXIT 100000
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmax(r, sh[w]);
+  return r;
+}
+
+// Flow accumulation on the left-over vertices U of a sweep (a receiver cycle
+// and everything below it).  goSPL's iterative solve converges there when
+// the cycle drains and gives up otherwise (flowplex.py:178-322): here the
+// fixed-point iteration q <- b + W^T q over U in Jacobi form, from q = 0 with
+// the solved vertices fixed, in one workgroup, until the largest change is at
+// most 1e-14 of the largest value.  Each pass sums a row in its fixed order,
+// so the result does not depend on scheduling.  res[0] = 1 when converged.
+__global__ __launch_bounds__(CYCLE_THREADS) void k_flow_cycles(DagArgs a, const int32_t* __restrict__ U,
+                                                             const int32_t* __restrict__ nU,
+                                                             double* __restrict__ xn,
+                                                             int32_t* __restrict__ res) {
+  __shared__ double sh[3][CYCLE_THREADS / 64];
+  const int n = *nU;
+  const int64_t N = a.N;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int v = U[t];
+    a.x[v] = 0.0;
+    for (int k = 0; k < a.NR; ++k) {
+      const int r = a.rcv[(int64_t)k * N + v];
+      const int sl = a.slot[(int64_t)k * N + v];
+      if (r != v && r >= 0 && a.coef[(int64_t)k * N + v] != 0.0 && sl >= 0) a.acc[sl] = 0.0;
+    }
   }
+  __syncthreads();
+  int conv = 0;
+  for (int it = 0; it < CYCLE_MAXIT; ++it) {
+    double dmax = 0.0, xmax = 0.0, bad = 0.0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int v = U[t];
+      double s = a.rhs[v];
+      const int e = a.doff[v + 1];
+      for (int c = a.doff[v]; c < e; ++c) s = s + a.acc[c];
+      xn[t] = s;
+      if (!isfinite(s)) bad = 1.0;
+      dmax = fmax(dmax, fabs(s - a.x[v]));
+      xmax = fmax(xmax, fabs(s));
+    }
+    dmax = block_max(dmax, sh[0]);
+    xmax = block_max(xmax, sh[1]);
+    bad = block_max(bad, sh[2]);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int v = U[t];
+      const double s = xn[t];
+      a.x[v] = s;
+      for (int k = 0; k < a.NR; ++k) {
+        const int r = a.rcv[(int64_t)k * N + v];
+        const double w = a.coef[(int64_t)k * N + v];
+        const int sl = a.slot[(int64_t)k * N + v];
+        if (r != v && r >= 0 && w != 0.0 && sl >= 0) a.acc[sl] = w * s;
+      }
+    }
+    __syncthreads();
+    if (bad != 0.0) break;
+    if (dmax <= 1.0e-14 * xmax) {
+      conv = 1;
+      break;
+    }
+  }
+  if (threadIdx.x == 0) res[0] = conv;
 }
 
 // SPL coefficients (SPL.py:56-108): off[k] = -(Kbr*limiter/dist)*w (0 for
@@ -490,7 +571,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   size_t bytes = m->cub_tmp_bytes;
   GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
   k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
-  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend, fa, counts);
+  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend);
   GOSPL_CHECK(hipGetLastError());
   DagArgs a{N, nrcv, rcv, coef, diag, rhs, x, doff, m->wdon, m->wacc, slot, pend};
   int32_t* nlev = dev;
@@ -506,11 +587,14 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   if (mode == 0)
-    k_dag_level0<0><<<g, GOSPL_BLOCK, 0, s>>>(a, fa, fb, counts);
+    k_dag_level0<0><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
   else
-    k_dag_level0<1><<<g, GOSPL_BLOCK, 0, s>>>(a, fa, fb, counts);
+    k_dag_level0<1><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
   GOSPL_CHECK(hipGetLastError());
-  GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
+  if (m->dag_coop)
+    GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
+  else  // plain launch (counter collection); the grid fits the GPU many times over
+    GOSPL_CHECK(hipLaunchKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
   if (m->dag_tail > 0) {
     if (mode == 0)
       k_dag_tail<0><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
@@ -519,7 +603,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     GOSPL_CHECK(hipGetLastError());
   }
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
-  k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1);
+  int32_t* left = fa;  // the frontier lists are free again
+  k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, bar + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -527,6 +612,14 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   if (m->hflag[34]) {
     set_error("receiver-DAG sweep aborted: a grid barrier did not complete");
     return 1;
+  }
+  if (mode == 0 && m->hflag[1] > 0) {
+    // a receiver cycle: iterate on what the sweep left over
+    k_flow_cycles<<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
+    GOSPL_CHECK(hipGetLastError());
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 2, dev + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    if (m->hflag[2]) m->hflag[1] = 0;
   }
   if (m->timing) {
     float ms = 0.0f;
@@ -574,6 +667,9 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* te_il = getenv("GOSPL_DAG_INTERLEAVE");
   if (te_il) m->dag_interleave = atoi(te_il) > 0;
   m->dag_tail = 256;
+  m->dag_coop = 1;
+  const char* te_coop = getenv("GOSPL_DAG_COOP");
+  if (te_coop) m->dag_coop = atoi(te_coop) != 0;
   const char* te_tail = getenv("GOSPL_DAG_TAIL");
   if (te_tail && atoi(te_tail) >= 0) m->dag_tail = atoi(te_tail);
   int per_cu = 1 << 30;

@@ -25,7 +25,7 @@
 // ECL-CC style), numbered by the smallest input id among their bottom cells
 // (label_pits' visit order) via a scan over input ids; spill points by
 // atomicMin of input ids; flat-routing distances by relaxation of the
-// breadth-first distance; pit volumes by FP64 atomics.
+// breadth-first distance; per-pit sums in a fixed order (k_pits.hip).
 #include <climits>
 
 #include <hipcub/hipcub.hpp>
@@ -258,17 +258,13 @@ __global__ void k_pit_mark(int64_t N, const int32_t* __restrict__ parent,
 __global__ void k_pit_ids(int64_t N, const int32_t* __restrict__ parent,
                           const int32_t* __restrict__ cmin, const int32_t* __restrict__ cnon,
                           const int32_t* __restrict__ rank, const int8_t* __restrict__ outlet,
-                          int flat, int32_t* __restrict__ pitids, int32_t* __restrict__ ncells,
-                          int32_t* __restrict__ cells) {
+                          int flat, int32_t* __restrict__ pitids) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < N;
+  if (i >= N) return;
   int id = -1;
-  if (valid) {
-    int r = parent[i];
-    if (r >= 0 && cmin[r] != INT_MAX && cnon[r] && !(flat && outlet[i])) id = rank[cmin[r]] + 1;
-    pitids[i] = id;
-  }
-  wave_append(valid && id > 0, (int)i, ncells, cells);  // compact list of pit cells
+  const int r = parent[i];
+  if (r >= 0 && cmin[r] != INT_MAX && cnon[r] && !(flat && outlet[i])) id = rank[cmin[r]] + 1;
+  pitids[i] = id;
 }
 
 // spill candidates (pitfilling.py:741-801)
@@ -395,70 +391,16 @@ __global__ void k_flatdirs(int64_t N, int W, const int32_t* __restrict__ nbr,
   flatdirs[i] = out;
 }
 
-// Marine restore + pit parameters (pitfilling.py:822-832, 521-555).
-__global__ void k_pit_params(int64_t N, const double* __restrict__ hl, double* __restrict__ fill,
-                             int32_t* __restrict__ flatdirs, const int32_t* __restrict__ pitids,
-                             const double* __restrict__ area, double sl, int64_t cap,
-                             double* __restrict__ vol, unsigned long long* __restrict__ hmaxk,
-                             unsigned long long* __restrict__ dhk) {
+// Marine restore (pitfilling.py:822-832): cells filled to sea level or below
+// take their unfilled elevation and have no flat receiver.
+__global__ void k_pit_marine(int64_t N, const double* __restrict__ hl, double* __restrict__ fill,
+                             int32_t* __restrict__ flatdirs, double sl) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double f = fill[i];
-  const double h = hl[i];
-  if (f <= sl) {
-    f = h;
-    fill[i] = f;
+  if (fill[i] <= sl) {
+    fill[i] = hl[i];
     flatdirs[i] = -1;
   }
-  const int p = pitids[i];
-  if (p < 0 || p >= cap) return;
-  const double hc = h < sl ? sl : h;
-  atomicAdd(&vol[p], (f - hc) * area[i]);
-  atomicMax(&hmaxk[p], dkey(f));
-  atomicMax(&dhk[p], dkey(f - hc));
-}
-
-// filled_lvl rows (pitfilling.py:877-881); converts the max keys.
-__global__ void k_pit_levels(int64_t npit, int64_t cap, double* __restrict__ params,
-                             const unsigned long long* __restrict__ hmaxk,
-                             const unsigned long long* __restrict__ dhk,
-                             double* __restrict__ lvl) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npit) return;
-  double hmax = (hmaxk[p] == 0ull) ? -1.0e8 : dkey_inv(hmaxk[p]);
-  double dh = (dhk[p] == 0ull) ? 0.0 : dkey_inv(dhk[p]);
-  params[cap + p] = hmax;
-  params[2 * cap + p] = dh;
-  double c[6] = {0, 0, 0, 0, 0, 0};
-  if (params[p] > 0.0) {
-    c[0] = hmax - dh;
-    double d5 = dh / 5.0;
-    for (int k = 1; k < 6; ++k) c[k] = c[k - 1] + d5;
-  }
-  for (int k = 0; k < 5; ++k) lvl[k * cap + p] = c[k + 1];
-}
-
-// getpitvol (functions.F90:1971-2003) on h clipped at sea level.
-__global__ void k_pit_vol(int64_t N, const double* __restrict__ hl, const int32_t* __restrict__ pitids,
-                          const double* __restrict__ area, double sl, int64_t cap,
-                          const double* __restrict__ lvl, double* __restrict__ fvol) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int p = pitids[i];
-  if (p < 0 || p >= cap) return;
-  double e = hl[i];
-  if (e < sl) e = sl;
-  for (int k = 0; k < 4; ++k) {
-    double l = lvl[k * cap + p];
-    if (l > e) atomicAdd(&fvol[k * cap + p], (l - e) * area[i]);
-  }
-}
-
-__global__ void k_pit_vol_last(int64_t npit, int64_t cap, const double* __restrict__ params,
-                               double* __restrict__ fvol) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npit) return;
-  fvol[4 * cap + p] = params[p];
 }
 
 // host helper: run `launch` until the device flag stays 0
@@ -544,10 +486,10 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 17, mark + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
   int32_t* ncells = m->dflag + 40;
   int32_t* cells = mark;  // the marks are consumed by the scan above
-  GOSPL_CHECK(hipMemsetAsync(ncells, 0, sizeof(int32_t), s));
-  k_pit_ids<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, rank, m->outlet, m->flat, pitids, ncells,
-                                       cells);
+  k_pit_ids<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, rank, m->outlet, m->flat, pitids);
   GOSPL_CHECK(hipGetLastError());
+  // compact list of pit cells in ascending order
+  if (pit_cells(m, pitids, cells, ncells, s)) return 1;
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 18, ncells, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
   const int64_t npits = (int64_t)m->hflag[16] + m->hflag[17];
@@ -559,6 +501,8 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
               (long long)cap);
     return 1;
   }
+  // cells grouped by pit for the per-pit reductions
+  if (pit_group(m, pitids, npits, cells, ncells, npc, s)) return 1;
   // spill points
   GOSPL_CHECK(hipMemsetAsync(best, 0x7f, sizeof(int32_t) * np1, s));
   GOSPL_CHECK(hipMemsetAsync(spill, 0xff, sizeof(int32_t) * np1, s));
@@ -581,21 +525,10 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   }
   k_flatdirs<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, ptdir, m->gid, lspill,
                                         flatdirs);
-  // pit parameters
-  unsigned long long* hk = m->wu[0];
-  unsigned long long* dk = m->wu[1];
-  GOSPL_CHECK(hipMemsetAsync(pitparams, 0, sizeof(double) * cap * 3, s));
-  GOSPL_CHECK(hipMemsetAsync(hk, 0, sizeof(unsigned long long) * np1, s));
-  GOSPL_CHECK(hipMemsetAsync(dk, 0, sizeof(unsigned long long) * np1, s));
-  GOSPL_CHECK(hipMemsetAsync(filled_lvl, 0, sizeof(double) * cap * 5, s));
-  GOSPL_CHECK(hipMemsetAsync(filled_vol, 0, sizeof(double) * cap * 5, s));
-  k_pit_params<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, lfill, flatdirs, pitids, m->area, sealevel, cap,
-                                          pitparams, hk, dk);
-  k_pit_levels<<<nblocks(np1), GOSPL_BLOCK, 0, s>>>(np1, cap, pitparams, hk, dk, filled_lvl);
-  k_pit_vol<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, pitids, m->area, sealevel, cap, filled_lvl, filled_vol);
-  k_pit_vol_last<<<nblocks(np1), GOSPL_BLOCK, 0, s>>>(np1, cap, pitparams, filled_vol);
+  // marine restore, then the pit parameters and depth-volume curves
+  k_pit_marine<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, lfill, flatdirs, sealevel);
   GOSPL_CHECK(hipGetLastError());
-  return 0;
+  return pit_group_params(m, hl, lfill, sealevel, cap, pitparams, filled_lvl, filled_vol, s);
 }
 
 }  // extern "C"

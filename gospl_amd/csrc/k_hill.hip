@@ -70,9 +70,46 @@ __global__ void k_hill_update(int64_t N, const double* __restrict__ x, double dt
   eblocal[i] = eblocal[i] + t / dt;
 }
 
+// sethillslopecoeff (functions.F90:901-938) for a per-vertex Kd: out is
+// (13, N), row 0 the diagonal and row 1 + p the coefficient of neighbour p.
+__global__ void k_hill_coef_kd(int64_t N, int W, const int32_t* __restrict__ nbr,
+                               const int32_t* __restrict__ deg, const double* __restrict__ area,
+                               const double* __restrict__ elgt, const double* __restrict__ vdist,
+                               const double* __restrict__ kd, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double a = area[i];
+  const double ck = kd[i];
+  double s1 = 0.0;
+  const int dg = deg[i];
+  for (int p = 0; p < GOSPL_MAXW; ++p) {
+    double c_out = 0.0;
+    if (a > 0.0 && p < dg && p < W) {
+      const double vd = vdist[(int64_t)p * N + i];
+      if (vd > 0.0) {
+        const int n = nbr[(int64_t)p * N + i];
+        const double c = 0.5 * (ck + kd[n]) / a;
+        const double v = c * vd / elgt[(int64_t)p * N + i];
+        s1 = s1 + v;
+        c_out = -v;
+      }
+    }
+    out[(int64_t)(p + 1) * N + i] = c_out;
+  }
+  out[i] = a > 0.0 ? 1.0 + s1 : 0.0;
+}
+
 }  // namespace gospl
 
 using namespace gospl;
+
+extern "C" int gospl_hillslope_coeff(gospl_mesh_t* m, const double* kd, double* coeff,
+                                     gospl_stream_t s) {
+  k_hill_coef_kd<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, m->W, m->nbr, m->deg, m->area, m->elgt,
+                                                       m->vdist, kd, coeff);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
 
 extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
                                int32_t sweeps, double* h, double* cumed, double* eblocal,

@@ -249,13 +249,19 @@ class Model:
                    _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
                    self.stream)
 
-    def _solve_flow(self, b, out):
+    def _solve_flow(self, b, out, fatal=True):
+        """_solve_KSP (flowplex.py:323-427) on the flow matrix.  A receiver
+        cycle that does not drain makes goSPL's solvers give up: the main
+        solve (fatal) raises, a cascade solve is zeroed (flowplex.py:304-320)."""
         lev = ctypes.c_int32(0)
         bad = ctypes.c_int64(0)
         self._call("gospl_flowacc", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal), _ptr(b),
                    _ptr(out), ctypes.byref(lev), ctypes.byref(bad), self.stream)
+        self.undrained = int(bad.value)
         if bad.value:
-            raise RuntimeError("flow receivers hold a cycle (%d vertices unsolved)" % bad.value)
+            if fatal:
+                raise RuntimeError("flow accumulation did not converge (%d undrained vertices)" % bad.value)
+            out.zero_()
         return int(lev.value)
 
     def _distributeDownstream(self, pitVol, FA, hl, step):
@@ -306,7 +312,7 @@ class Model:
             self._cascade_resid = resid
             if resid > self.maxarea[0]:
                 excess = True
-                self._solve_flow(rhs, self._q)
+                self._solve_flow(rhs, self._q, fatal=False)
                 self._call("gospl_cascade_add", self.H, _ptr(self._q), _ptr(hl), _ptr(self.waterFilled),
                            self.dt, _ptr(self._nfa), _ptr(self.FAL), S)
                 nFA = self._nfa

@@ -100,7 +100,9 @@ int gospl_fill_depressions(gospl_mesh_t* mesh, const double* h, double level,
  * `cap` rows (index 0 unused, pits 1..npits): spill (cap), pitparams
  * (3, cap) = volume, max fill, max depth; filled_lvl (5, cap) and
  * filled_vol (5, cap).  *npits (HOST) receives the pit count; the call
- * fails if npits + 1 > cap.
+ * fails if npits + 1 > cap.  Per-pit sums are reduced in a fixed order, so
+ * results are reproducible run to run (their last bits differ from numpy's
+ * group-by order).
  */
 int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double sealevel,
                    int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill,
@@ -111,8 +113,14 @@ int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double s
  * Flow accumulation solve: replaces FAMesh._solve_KSP on the flow matrix
  * (I - W)^T built by matrixFlow (gospl/flow/flowplex.py:323-478, 843) with an
  * exact sweep over the receiver DAG.  rcv/wgt (nrcv, N), b (N) -> q (N).
- * *nlevels (HOST, may be NULL) receives the DAG depth; *unsolved (HOST, may be
- * NULL) the number of vertices left on a cycle (their q is set to 0).
+ * *nlevels (HOST, may be NULL) receives the DAG depth.  Receivers built with
+ * flat routing on a partly filled surface can hold a cycle; the vertices the
+ * sweep cannot reach (the cycle and everything below it) are then solved by
+ * the fixed-point iteration q <- b + W^T q restricted to them, to a relative
+ * change of 1e-14, as goSPL's iterative solve converges on a draining cycle.
+ * *unsolved (HOST, may be NULL) receives 0, or the number of those vertices
+ * when the iteration does not converge (an undrained cycle, on which goSPL's
+ * solvers give up, flowplex.py:178-322; q is then not meaningful there).
  */
 int gospl_flowacc(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const double* wgt,
                   const double* b, double* q, int32_t* nlevels, int64_t* unsolved,
@@ -146,6 +154,18 @@ int gospl_hillslope(gospl_mesh_t* mesh, double cda, double cdm, double dt, doubl
                     gospl_stream_t stream);
 
 /*
+ * Hillslope operator coefficients: replaces `sethillslopecoeff`
+ * (functions.pyf:10-14, functions.F90:901-938).  kd (N) = diffusivity * dt
+ * per vertex; coeff (13, N): row 0 the diagonal 1 + sum v, row 1 + p the
+ * coefficient -v of neighbour p (0 past the vertex's degree), with
+ * v = (0.5 * (kd_i + kd_n) / area_i) * vDist / eLgt.  Rows of zero-area
+ * vertices are all zero.  Flat-mesh outlets are not special-cased here (that
+ * is the operator assembly's job, hillslope.py:253-255).
+ */
+int gospl_hillslope_coeff(gospl_mesh_t* mesh, const double* kd, double* coeff,
+                          gospl_stream_t stream);
+
+/*
  * Runoff source of flowAccumulation (gospl/flow/flowplex.py:794-795):
  * b = rain * area with sea (lfill <= sealevel) zeroed.
  */
@@ -155,7 +175,11 @@ int gospl_rain_source(gospl_mesh_t* mesh, const double* rain, const double* lfil
 /*
  * Pit-overflow cascade pieces (FAMesh._distributeDownstream,
  * gospl/flow/flowplex.py:576-686):
- *  - gospl_pit_inflow: inv[p] = sum of fa over lsink cells of pit p (npit rows).
+ *  - gospl_pit_inflow: inv[p] = sum of fa over lsink cells of pit p (npit rows),
+ *    in a fixed order.  It reuses the grouping of cells by pit that the last
+ *    gospl_pit_info built for the same `pitids` buffer and pit count (so
+ *    `pitids` must not be changed in between), else it builds one, which
+ *    synchronises `stream`.
  *  - gospl_pit_water: per-pit water levels; full[p] != 0 sets water = lfill,
  *    else if level[p] is not NaN and water <= level[p], water = level[p].
  *  - gospl_cascade_add: nfa = q*dt; fal += (nfa masked where hl < water)/dt.

@@ -411,12 +411,15 @@ void or_sethillslopecoeff(int64_t nb, const int32_t* ngb, const int32_t* nnb,
   }
 }
 
+#define OR_CYCLE_MAXIT 100000
+
 /* ------------------------------------------------------------------------ */
 /* Exact solve of (I - W)^T q = b for a receiver DAG (flow accumulation,     */
 /* flowplex.py:429-478 matrix, solved by _solve_KSP :323-427 to rtol 1e-8;   */
 /* here by a topological sweep).  rcv/wgt are (nb, K) row-major; a receiver  */
 /* equal to the row or a zero weight adds nothing.  Returns the number of    */
-/* vertices left unsolved (nonzero only if the receivers hold a cycle).      */
+/* vertices left unsolved (nonzero only if the receivers hold a cycle that   */
+/* does not drain; see the fixed-point pass at the end).                     */
 /* ------------------------------------------------------------------------ */
 int64_t or_flowacc(int64_t nb, int K, const int32_t* rcv, const double* wgt,
                    const double* b, double* q, const int32_t* ngb, const int32_t* nnb) {
@@ -453,9 +456,55 @@ int64_t or_flowacc(int64_t nb, int K, const int32_t* rcv, const double* wgt,
       }
     }
   }
-  for (int64_t j = 0; j < nb; ++j) if (pend[j] > 0) q[j] = NAN;
+  int64_t left = nb - done;
+  if (left > 0) {
+    /* Vertices left over: a cycle in the receivers (cascade passes route on
+       waterFilled with flat receivers, flowplex.py:655-661) and everything
+       below it.  goSPL's iterative solve converges there when the cycle
+       drains and gives up otherwise, zeroing a non-fatal solve
+       (flowplex.py:178-322, 410-424).  Restated as the fixed-point
+       iteration q <- b + W^T q on the left-over vertices (Jacobi form, from
+       q = 0, solved vertices fixed) until the largest change is at most
+       1e-14 of the largest value; 0 is returned when it converges, the
+       number of left-over vertices when it does not within
+       OR_CYCLE_MAXIT passes or leaves the finite range. */
+    int64_t* U = (int64_t*)malloc((size_t)left * sizeof(int64_t));
+    double* xn = (double*)malloc((size_t)left * sizeof(double));
+    int64_t nu = 0;
+    for (int64_t j = 0; j < nb; ++j)
+      if (pend[j] > 0) { U[nu++] = j; q[j] = 0.0; }
+    int conv = 0;
+    for (int it = 0; it < OR_CYCLE_MAXIT; ++it) {
+      double dmax = 0.0, xmax = 0.0;
+      int bad = 0;
+      for (int64_t t = 0; t < nu; ++t) {
+        const int64_t i = U[t];
+        double s = b[i];
+        for (int p = 0; p < nnb[i]; ++p) {
+          int32_t j = ngb[i * MAXNB + p];
+          if (j < 0) continue;
+          for (int k = 0; k < K; ++k) {
+            if (rcv[j * K + k] == i) {
+              double w = wgt[j * K + k];
+              if (w != 0.0) s = s + w * q[j];
+            }
+          }
+        }
+        xn[t] = s;
+        if (!isfinite(s)) bad = 1;
+        const double d = fabs(s - q[i]);
+        if (d > dmax) dmax = d;
+        if (fabs(s) > xmax) xmax = fabs(s);
+      }
+      for (int64_t t = 0; t < nu; ++t) q[U[t]] = xn[t];
+      if (bad) break;
+      if (dmax <= 1.0e-14 * xmax) { conv = 1; break; }
+    }
+    free(U); free(xn);
+    left = conv ? 0 : nu;
+  }
   free(pend); free(st);
-  return nb - done;
+  return left;
 }
 
 /* ------------------------------------------------------------------------ */

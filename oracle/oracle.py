@@ -380,10 +380,16 @@ class OracleModel:
         lsink[self.seaID] = False
         self.lsink = (lsink.astype(int) * self.inIDs) == 1
 
-    def _solve_flow(self, b):
+    def _solve_flow(self, b, fatal=True):
+        """_solve_KSP (flowplex.py:323-427) on the flow matrix.  A receiver
+        cycle that does not drain makes goSPL's solvers fail: the main solve
+        (fatal) raises, a cascade solve is zeroed (flowplex.py:304-320)."""
         q, bad = flowacc(self.rcvID, self.wghtVal, b, self.ngb, self.nnb)
+        self.undrained = bad
         if bad:
-            raise RuntimeError("receiver graph has a cycle (%d vertices unsolved)" % bad)
+            if fatal:
+                raise RuntimeError("flow accumulation did not converge (%d undrained vertices)" % bad)
+            q = np.zeros_like(q)
         return q
 
     def _distributeDownstream(self, pitVol, FA, hl, step):
@@ -425,7 +431,7 @@ class OracleModel:
             self._cascade_resid = float(resid)
             if resid > self.maxarea[0]:
                 excess = True
-                q = self._solve_flow(rhs)
+                q = self._solve_flow(rhs, fatal=False)
                 nFA = q * self.dt
                 FA = nFA.copy()
                 FA[hl < self.waterFilled] = 0.0

@@ -49,3 +49,36 @@ def rel_err(a, b):
     b = np.asarray(b, np.float64)
     scale = max(np.abs(b).max(), 1e-300)
     return float(np.abs(a - b).max() / scale)
+
+
+def make_cycle(rcv, wgt, drain=True):
+    """Receivers (N, K) in input order with a two-vertex cycle added: a vertex
+    A with a positive-weight receiver B that itself drains elsewhere gets B
+    routing half its flow back to A (a cycle of gain < 1 that drains), or,
+    with drain=False, A and B sending everything to each other (a cycle that
+    never drains).  Returns new (rcv, wgt) and (A, B)."""
+    rcv = rcv.copy()
+    wgt = wgt.copy()
+    N, K = rcv.shape
+    for a in range(N):
+        for k in range(K):
+            b = rcv[a, k]
+            if b == a or wgt[a, k] <= 0.0:
+                continue
+            out_b = [(rcv[b, j], wgt[b, j]) for j in range(K) if rcv[b, j] != b and wgt[b, j] > 0 and rcv[b, j] != a]
+            if not out_b:
+                continue
+            if drain:
+                rcv[b, :] = b
+                wgt[b, :] = 0.0
+                rcv[b, 0], wgt[b, 0] = a, 0.5
+                rcv[b, 1], wgt[b, 1] = out_b[0][0], 0.5
+            else:
+                rcv[a, :] = a
+                wgt[a, :] = 0.0
+                rcv[b, :] = b
+                wgt[b, :] = 0.0
+                rcv[a, 0], wgt[a, 0] = b, 1.0
+                rcv[b, 0], wgt[b, 0] = a, 1.0
+            return rcv, wgt, (a, b)
+    raise ValueError("no vertex pair for a cycle")

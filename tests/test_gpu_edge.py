@@ -1,0 +1,145 @@
+"""Edge cases of the hot path, HIP vs the oracle from identical state.
+
+The surfaces here stress the discrete parts of goSPL's step: a mesh that is
+entirely below sea level (no runoff, no stream power, marine diffusion
+only), a flat plateau (every cell is on a flat: routing is the flat-receiver
+field of _dirFlats), one closed bowl (a single depression that fills and
+overflows through the cascade), integer elevations (receiver ties on every
+slope) and the smallest planar mesh with interior cells.  Each case runs a
+few steps; before every step the oracle takes the GPU's state, and the
+outputs of the step must agree to 1e-9 relative (pit ids, filled
+elevations and receivers bit for bit).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.helpers import case, dev_index_to_input, oracle_model, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def _surface(kind, m, h0):
+    xy = m.coords
+    if kind == "all_sea":
+        rng = np.random.default_rng(3)
+        return -1000.0 + rng.uniform(0.0, 50.0, m.N)
+    if kind == "plateau":
+        # flat interior at 10 m, borders at 0: one flat drained through the borders
+        h = np.full(m.N, 10.0)
+        h[m.outlet > 0] = 0.0
+        return h
+    if kind == "bowl":
+        c = xy[:, :2].mean(axis=0)
+        r = np.hypot(xy[:, 0] - c[0], xy[:, 1] - c[1])
+        R = r.max()
+        h = 200.0 - 150.0 * np.exp(-(r / (0.3 * R)) ** 2) + 0.5 * (r / R)
+        h[m.outlet > 0] = 0.0
+        return h
+    if kind == "ties":
+        return np.round(h0)
+    raise ValueError(kind)
+
+
+def _compare_steps(m, h, p, nsteps=3):
+    from gospl_amd.model import Model
+
+    g = Model(m, h, tend=1e12, **p)
+    o = oracle_model(m, h, p)
+    g.initialise()
+    o.initialise()
+    for s in range(nsteps):
+        o.h = m.to_input_order(g.hLocal.cpu().numpy()).copy()
+        o.cumED = m.to_input_order(g.cumED.cpu().numpy()).copy()
+        o.tNow = g.tNow
+        g.step()
+        o.step()
+        assert np.array_equal(m.to_input_order(g.pitIDs.cpu().numpy()), o.pitIDs), s
+        assert np.array_equal(m.to_input_order(g.lFill.cpu().numpy()), o.lFill), s
+        r_in = dev_index_to_input(m, m.to_input_order(g.rcvIDi.cpu().numpy().T))
+        assert np.array_equal(r_in, o.rcvIDi), s
+        assert g.cascade_passes == o.cascade_passes, s
+        h_g = m.to_input_order(g.hLocal.cpu().numpy())
+        assert rel_err(h_g, o.h) < TOL, (s, rel_err(h_g, o.h))
+        fal = m.to_input_order(g.FAL.cpu().numpy())
+        if np.abs(o.FAL).max() > 0:
+            assert rel_err(fal, o.FAL) < TOL, s
+        else:
+            assert np.abs(fal).max() == 0.0, s
+        ed = m.to_input_order(g.cumED.cpu().numpy())
+        assert np.abs(ed - o.cumED).max() <= TOL * max(np.abs(o.h).max(), 1.0), s
+    return g, o
+
+
+@pytest.mark.parametrize("kind", ["plateau", "bowl", "ties"])
+def test_planar_surfaces(kind):
+    m, h0, p = case("planar", 40)
+    h = _surface(kind, m, h0)
+    g, o = _compare_steps(m, h, p)
+    if kind == "bowl":
+        assert g.npits >= 1
+
+
+def test_all_sea_sphere():
+    m, h0, p = case("sphere", 4)
+    h = _surface("all_sea", m, h0)
+    g, o = _compare_steps(m, h, p)
+    assert float(g.FAL.abs().max()) == 0.0
+    assert float(g.Eb.abs().max()) == 0.0
+
+
+def test_smallest_planar_mesh():
+    m, h0, p = case("planar", 4)
+    _compare_steps(m, h0, p, nsteps=2)
+
+
+def test_sphere_ties():
+    m, h0, p = case("sphere", 5)
+    _compare_steps(m, np.round(h0 / 10.0) * 10.0, p)
+
+
+@pytest.mark.parametrize("drain", [True, False])
+def test_flow_receiver_cycle(drain):
+    """Receivers with a two-vertex cycle through gospl_flowacc: a cycle that
+    drains is solved by the fixed-point pass to the oracle's values; one
+    that never drains is reported as unsolved."""
+    import ctypes
+
+    import torch
+
+    from gospl_amd import _lib
+    from gospl_amd.model import DeviceMesh, _ptr
+    from oracle import oracle as orc
+    from tests.helpers import make_cycle
+
+    m, h0, p = case("planar", 12)
+    o = oracle_model(m, h0, p)
+    N = m.N
+    z = m.to_input_order(h0)
+    rcv, dist, wgt = orc.mfdreceivers(3, 1.1, z, -1e9, np.arange(N, dtype=np.int32), o.ngb, o.nnb, o.elgt)
+    rc, wc, _ = make_cycle(rcv, wgt, drain=drain)
+    b = np.random.default_rng(4).uniform(0.5, 2.0, N) * m.to_input_order(m.area)
+    qo, bado = orc.flowacc(rc, wc, b, o.ngb, o.nnb)
+    dm = DeviceMesh(m, "cuda")
+    rdev = m.iperm[rc[m.perm]].T.astype(np.int32)  # (K, N) device indices
+    wdev = wc[m.perm].T
+    R = torch.as_tensor(np.ascontiguousarray(rdev), device="cuda")
+    Wt = torch.as_tensor(np.ascontiguousarray(wdev), device="cuda")
+    B = torch.as_tensor(m.to_device_order(b), device="cuda")
+    Q = torch.zeros_like(B)
+    lev = ctypes.c_int32(0)
+    bad = ctypes.c_int64(0)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert _lib.load().gospl_flowacc(dm.handle, 3, _ptr(R), _ptr(Wt), _ptr(B), _ptr(Q), ctypes.byref(lev),
+                                     ctypes.byref(bad), s) == 0
+    torch.cuda.synchronize()
+    if drain:
+        assert bado == 0 and bad.value == 0
+        q = m.to_input_order(Q.cpu().numpy())
+        assert rel_err(q, qo) < 1e-13
+    else:
+        assert bado > 0 and bad.value == bado

@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-9
 
-CASES = [("planar", 60), ("sphere", 5), ("planar", 120), ("sphere", 6)]
+# planar 200 is the 200x200 (40,000-vertex) configuration goSPL's CPU benchmark runs
+CASES = [("planar", 60), ("sphere", 5), ("planar", 120), ("sphere", 6), ("planar", 200)]
 
 
 def _gpu_model(m, h0, p):
@@ -113,7 +114,7 @@ def _sync_oracle(o, g, m):
     o.tNow = g.tNow
 
 
-@pytest.mark.parametrize("kind,size", CASES[:2] + [("sphere", 6)])
+@pytest.mark.parametrize("kind,size", CASES[:2] + [("sphere", 6), ("planar", 200)])
 def test_steps_match_oracle(kind, size):
     """Per step from identical state: the model's decisions (sea/land at
     h <= sea level, filled-rim cells at hl <= waterFilled) compare values

@@ -178,6 +178,28 @@ def test_flowacc_exact_and_mass_balance():
     assert abs(q[sinks].sum() - b.sum()) <= 1e-10 * b.sum()
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_flowacc_cycle_fixed_point(seed):
+    """A receiver cycle of gain < 1: the left-over vertices are solved by the
+    fixed-point iteration, which must reach the dense solution; a cycle
+    that never drains is reported (goSPL's solvers give up there)."""
+    from tests.helpers import make_cycle
+
+    v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(12, seed)
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(0, 30, len(v)) + 0.05 * v[:, 0]
+    rcv, dist, wgt = orc.mfdreceivers(3, 1.1, z, -1e9, np.arange(len(v), dtype=np.int32), ngb, nnb, elgt)
+    b = rng.uniform(0.5, 2.0, len(v)) * area
+    rc, wc, _ = make_cycle(rcv, wgt, drain=True)
+    q, bad = orc.flowacc(rc, wc, b, ngb, nnb)
+    assert bad == 0
+    qd = _dense_flow(rc, wc, b)
+    assert np.abs(q - qd).max() <= 1e-12 * np.abs(qd).max()
+    rc, wc, _ = make_cycle(rcv, wgt, drain=False)
+    q, bad = orc.flowacc(rc, wc, b, ngb, nnb)
+    assert bad > 0
+
+
 def test_spl_solve_exact():
     v, c, ngb, nnb, elgt, vdist, area, out = _small_plane(12, 5)
     rng = np.random.default_rng(5)

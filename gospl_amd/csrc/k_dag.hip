@@ -146,22 +146,61 @@ struct DagArgs {
   int32_t* pend;        // (N) unsolved dependencies
 };
 
+// Decrement of a pending counter, returning the old value.  Device scope
+// while several workgroups sweep, workgroup scope in the one-workgroup tail
+// (there the counters are touched by one workgroup only, so the atomic can
+// stay in this XCD's L2).
+template <bool WG>
+__device__ __forceinline__ int dec(int32_t* p) {
+  if (WG) return __hip_atomic_fetch_add(p, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return atomicSub(p, 1);
+}
+
+// Appends the K candidate values of every lane whose flag is set with one
+// reservation per wavefront.  Must be reached by all lanes of the wavefront.
+template <int K, class Append>
+__device__ __forceinline__ void append_all(const bool (&pred)[K], const int (&val)[K], int kmax,
+                                           const Append& app) {
+  unsigned long long m[K];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    m[k] = (k < kmax) ? __ballot(pred[k]) : 0ull;
+    total += __popcll(m[k]);
+  }
+  if (total == 0) return;
+  const int lane = __lane_id();
+  int base = 0;
+  if (lane == 0) base = app.reserve(total);
+  base = __shfl(base, 0);
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k < kmax && pred[k]) app.write(base + __popcll(m[k] & below), val[k]);
+    base += __popcll(m[k]);
+  }
+}
+
 // Solve vertex v (if valid) and release its dependents into the next
-// frontier.  Reached by all lanes of a wavefront (the appends are
-// wavefront-aggregated).
-template <int MODE, class Append>
+// frontier.  Reached by all lanes of a wavefront.  All loads of a vertex,
+// then all its counter decrements, are issued together (unrolled over the
+// stencil width, results used only after the last one is issued), so a vertex
+// costs three dependent memory round trips and one frontier reservation.
+template <int MODE, class Append, bool WG = false>
 __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs& a,
-                                               Append append) {
+                                               const Append& app) {
   const int64_t N = a.N;
   const int NR = a.NR;
   if (MODE == 0) {
     // q_v = b_v + sum over donor entries (row order) of w * q_donor
     int r[GOSPL_MAXW];
-    bool rel[GOSPL_MAXW];
+    int old[GOSPL_MAXW];
+    bool dep[GOSPL_MAXW];
 #pragma unroll
     for (int k = 0; k < GOSPL_MAXW; ++k) {
       r[k] = 0;
-      rel[k] = false;
+      old[k] = 0;
+      dep[k] = false;
     }
     if (valid) {
       double w[GOSPL_MAXW];
@@ -189,22 +228,24 @@ __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs&
 #pragma unroll
       for (int k = 0; k < GOSPL_MAXW; ++k) {
         if (k < NR && r[k] != v && r[k] >= 0 && w[k] != 0.0) {
+          dep[k] = true;
           if (sl[k] >= 0) a.acc[sl[k]] = w[k] * s;
-          rel[k] = atomicSub(&a.pend[r[k]], 1) == 1;
+          old[k] = dec<WG>(&a.pend[r[k]]);
         }
       }
     }
+    bool rel[GOSPL_MAXW];
 #pragma unroll
-    for (int k = 0; k < GOSPL_MAXW; ++k)
-      if (k < NR) append(rel[k], r[k]);
+    for (int k = 0; k < GOSPL_MAXW; ++k) rel[k] = dep[k] && old[k] == 1;
+    append_all<GOSPL_MAXW>(rel, r, NR, app);
   } else {
     // x_v = (h_v - sum_k off_k x_{rcv_k}) / diag_v, then release the donors
     int j[GOSPL_MAXW];
-    bool rel[GOSPL_MAXW];
+    int old[GOSPL_MAXW];
 #pragma unroll
     for (int c = 0; c < GOSPL_MAXW; ++c) {
       j[c] = 0;
-      rel[c] = false;
+      old[c] = 0;
     }
     int b = 0, e = 0;
     if (valid) {
@@ -233,19 +274,21 @@ __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs&
       a.x[v] = s / dg;
 #pragma unroll
       for (int c = 0; c < GOSPL_MAXW; ++c)
-        if (b + c < e) rel[c] = atomicSub(&a.pend[j[c]], 1) == 1;
+        if (b + c < e) old[c] = dec<WG>(&a.pend[j[c]]);
     }
+    bool rel[GOSPL_MAXW];
 #pragma unroll
-    for (int c = 0; c < GOSPL_MAXW; ++c) append(rel[c], j[c]);
+    for (int c = 0; c < GOSPL_MAXW; ++c) rel[c] = b + c < e && old[c] == 1;
+    append_all<GOSPL_MAXW>(rel, j, GOSPL_MAXW, app);
     // rows longer than the stencil width (not produced by an FV mesh)
     for (int c = GOSPL_MAXW; __ballot(b + c < e) != 0ull; ++c) {
-      bool rl = false;
-      int jj = 0;
+      bool rl[1] = {false};
+      int jj[1] = {0};
       if (b + c < e) {
-        jj = a.don[b + c];
-        rl = atomicSub(&a.pend[jj], 1) == 1;
+        jj[0] = a.don[b + c];
+        rl[0] = dec<WG>(&a.pend[jj[0]]) == 1;
       }
-      append(rl, jj);
+      append_all<1>(rl, jj, 1, app);
     }
   }
 }
@@ -254,9 +297,8 @@ __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs&
 struct GlobalAppend {
   int32_t* cnt;
   int32_t* out;
-  __device__ __forceinline__ void operator()(bool pred, int value) const {
-    wave_append(pred, value, cnt, out);
-  }
+  __device__ __forceinline__ int reserve(int n) const { return atomicAdd(cnt, n); }
+  __device__ __forceinline__ void write(int idx, int value) const { out[idx] = value; }
 };
 
 // Level 0 (vertices without dependencies, typically a third of the mesh) on
@@ -332,19 +374,10 @@ struct TailAppend {
   int32_t* cnt;  // LDS counter
   int32_t* lq;   // LDS list
   int32_t* gq;   // global list (capacity N)
-  __device__ __forceinline__ void operator()(bool pred, int value) const {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(cnt, __popcll(mask));
-    base = __shfl(base, leader);
-    if (pred) {
-      const int idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-      if (idx < TAIL_Q) lq[idx] = value;
-      else gq[idx] = value;
-    }
+  __device__ __forceinline__ int reserve(int n) const { return atomicAdd(cnt, n); }
+  __device__ __forceinline__ void write(int idx, int value) const {
+    if (idx < TAIL_Q) lq[idx] = value;
+    else gq[idx] = value;
   }
 };
 
@@ -383,7 +416,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
       const bool valid = t < n;
       int v = 0;
       if (valid) v = (first || t >= TAIL_Q) ? gcur[t] : sq[p][t];
-      process_vertex<MODE>(valid, v, a, app);
+      process_vertex<MODE, TailAppend, true>(valid, v, a, app);
     }
     __syncthreads();
     n = sc[(level + 1) % 3];

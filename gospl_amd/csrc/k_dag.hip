@@ -93,39 +93,50 @@ __global__ void k_pending(int mode, int64_t N, int NR, const int32_t* __restrict
   pend[i] = c;
 }
 
-// Grid barrier (all blocks co-resident: cooperative launch).  bar[0]
-// arrivals, bar[1] generation, bar[2] abort.  A waiter that spins too long
-// raises the abort flag instead of hanging; all blocks then leave the level
-// loop and the host reports the failure.
-__device__ __forceinline__ bool grid_barrier(unsigned int* bar, unsigned int nblocks) {
+// Grid barrier that also sums the next frontier's size (all blocks
+// co-resident: cooperative launch).  bar[0] (64-bit) collects arrivals in its
+// high word and the blocks' appended counts in its low word; the last block
+// to arrive publishes (level << 32 | total) in bar[1]; bar[2] (low word) is
+// the abort flag.  A waiter that spins too long raises the abort flag
+// instead of hanging; all blocks then leave the level loop and the host
+// reports the failure.  Returns the next frontier's size, or -1 on abort.
+__device__ __forceinline__ int grid_barrier_count(unsigned long long* bar, unsigned int nblocks,
+                                                  int level, int appended) {
   __syncthreads();
-  __shared__ int s_abort;
+  __shared__ int s_next;
   if (threadIdx.x == 0) {
-    const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
-    const unsigned int arrived =
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    int ab = 0;
-    if (arrived == nblocks - 1) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long old = __hip_atomic_fetch_add(
+        &bar[0], (1ull << 32) + (unsigned long long)(unsigned int)appended, __ATOMIC_ACQ_REL,
+        __HIP_MEMORY_SCOPE_AGENT);
+    int next;
+    if ((unsigned int)(old >> 32) == nblocks - 1) {
+      next = (int)((unsigned int)old + (unsigned int)appended);
+      __hip_atomic_store(&bar[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar[1], ((unsigned long long)(unsigned int)level << 32) | (unsigned int)next,
+                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       unsigned long long spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      for (;;) {
+        const unsigned long long w = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if ((unsigned int)(w >> 32) == (unsigned int)level) {
+          next = (int)(unsigned int)w;
+          break;
+        }
         __builtin_amdgcn_s_sleep(1);
         if (++spins > (1ull << 26) ||
             __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ab = 1;
+          __hip_atomic_store(&bar[2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          next = -1;
           break;
         }
       }
     }
     __threadfence();
-    s_abort = ab;
+    s_next = next;
   }
   __syncthreads();
-  return s_abort == 0;
+  return s_next;
 }
 
 #define DAG_BLOCK 1024
@@ -293,11 +304,16 @@ __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs&
   }
 }
 
-// Appends into a frontier list in global memory.
+// Appends into a frontier list in global memory; `blk` (LDS, may be null)
+// accumulates the workgroup's appended count.
 struct GlobalAppend {
   int32_t* cnt;
   int32_t* out;
-  __device__ __forceinline__ int reserve(int n) const { return atomicAdd(cnt, n); }
+  int32_t* blk;
+  __device__ __forceinline__ int reserve(int n) const {
+    if (blk) atomicAdd(blk, n);
+    return atomicAdd(cnt, n);
+  }
   __device__ __forceinline__ void write(int idx, int value) const { out[idx] = value; }
 };
 
@@ -322,7 +338,7 @@ __global__ void k_dag_level0(DagArgs a, int32_t* __restrict__ fb, int32_t* __res
       }
     }
   }
-  process_vertex<MODE>(valid, v, a, GlobalAppend{&counts[1], fb});
+  process_vertex<MODE>(valid, v, a, GlobalAppend{&counts[1], fb, nullptr});
 }
 
 // Levels 1.. on a small co-resident grid.  Stops at the first level whose
@@ -331,8 +347,9 @@ __global__ void k_dag_level0(DagArgs a, int32_t* __restrict__ fb, int32_t* __res
 template <int MODE>
 __global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
     DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb, int32_t* __restrict__ counts,
-    unsigned int* __restrict__ bar, int32_t* __restrict__ nlev, long long* __restrict__ trace,
+    unsigned long long* __restrict__ bar, int32_t* __restrict__ nlev, long long* __restrict__ trace,
     int interleave, int tail_n) {
+  __shared__ int32_t s_app[2];  // appended by this workgroup, by level parity
   // Wavefront w of block b takes frontier slice (w * gridDim.x + b) when
   // interleaving, so a small frontier spreads over all CUs instead of filling
   // the first blocks.
@@ -342,8 +359,10 @@ __global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
   const int64_t tid = wave * 64 + (threadIdx.x & 63);
   const int64_t nth = (int64_t)gridDim.x * blockDim.x;
   int level = 1;  // level 0 ran in k_dag_level0
+  int n = load_count(&counts[1]);
+  if (threadIdx.x == 0) s_app[1] = 0;
+  __syncthreads();
   for (;;) {
-    const int n = load_count(&counts[level]);
     if (trace && tid == 0 && level < 8192) {
       trace[2 * level] = (long long)wall_clock64();
       trace[2 * level + 1] = n;
@@ -352,14 +371,20 @@ __global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
     int32_t* cur = (level & 1) ? fb : fa;
     int32_t* nxt = (level & 1) ? fa : fb;
     int32_t* cnt = &counts[level + 1];
+    // this level appends to s_app[level & 1]; the other slot was read at the
+    // previous barrier, before its closing workgroup barrier
+    if (threadIdx.x == 0) s_app[(level + 1) & 1] = 0;
+    const GlobalAppend app{cnt, nxt, &s_app[level & 1]};
     // uniform trip count per wavefront so the aggregated appends see all lanes
     for (int64_t t0 = tid - (tid & 63); t0 < n; t0 += nth) {
       const int64_t t = t0 + (tid & 63);
       const bool valid = t < n;
       const int v = valid ? cur[t] : 0;
-      process_vertex<MODE>(valid, v, a, GlobalAppend{cnt, nxt});
+      process_vertex<MODE>(valid, v, a, app);
     }
-    if (!grid_barrier(bar, gridDim.x)) break;
+    __syncthreads();
+    n = grid_barrier_count(bar, gridDim.x, level, s_app[level & 1]);
+    if (n < 0) break;
     level++;
   }
   if (tid == 0) *nlev = level;
@@ -388,7 +413,7 @@ struct TailAppend {
 template <int MODE>
 __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
     DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb,
-    const int32_t* __restrict__ counts, const unsigned int* __restrict__ bar,
+    const int32_t* __restrict__ counts, const unsigned long long* __restrict__ bar,
     int32_t* __restrict__ nlev, long long* __restrict__ trace) {
   __shared__ int32_t sq[2][TAIL_Q];
   __shared__ int32_t sc[3];
@@ -595,8 +620,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   if (ensure_donors(m, (int64_t)nrcv * N)) return 1;
   GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (N + 2), s));
   GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
-  unsigned int* bar = reinterpret_cast<unsigned int*>(dev + 32);
-  GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned int) * 4, s));
+  unsigned long long* bar = reinterpret_cast<unsigned long long*>(dev + 32);
+  GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned long long) * 3, s));
   int32_t* slot = mode == 0 ? m->wslot : nullptr;
   if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
   // donor rows
@@ -640,7 +665,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
-  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, bar + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, bar + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));  // abort word
   GOSPL_CHECK(hipStreamSynchronize(s));
   if (m->hflag[34]) {
     set_error("receiver-DAG sweep aborted: a grid barrier did not complete");

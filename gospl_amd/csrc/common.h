@@ -42,6 +42,7 @@ struct gospl_mesh {
   int32_t* wslot;             // (nrcv, N) slot of each (vertex, receiver) in its receiver's row
   int64_t wdon_cap;
   long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
+  int iter_hint[4];           // passes the relaxation loops needed last time
   // cells grouped by depression (sorted (pit id << 32 | cell) keys + runs)
   unsigned long long* pg_keys[2];
   int32_t* pg_seg;            // start (pg_cap + 64) then end (pg_cap + 64)

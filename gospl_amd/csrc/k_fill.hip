@@ -85,7 +85,7 @@ __global__ void k_fill_descent(int64_t N, int W, const int32_t* __restrict__ nbr
 }
 
 // In-place pointer jumping; a few hops per launch.
-__global__ void k_jump(int64_t N, int32_t* __restrict__ root, int32_t* __restrict__ flag) {
+__global__ void k_jump(int64_t N, int32_t* __restrict__ root, int32_t* __restrict__ flag, int pass) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int r = root[i];
@@ -98,21 +98,15 @@ __global__ void k_jump(int64_t N, int32_t* __restrict__ root, int32_t* __restric
   }
   if (r != r0) {
     root[i] = r;
-    if (root[r] != r) *flag = 1;
+    if (root[r] != r) *flag = pass;
   }
-}
-
-__global__ void k_jump_check(int64_t N, const int32_t* __restrict__ root, int32_t* __restrict__ flag) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int r = root[i];
-  if (root[r] != r) *flag = 1;
 }
 
 __global__ void k_fill_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
                              const int32_t* __restrict__ deg, const double* __restrict__ dem,
                              const int32_t* __restrict__ state, const int32_t* __restrict__ root,
-                             unsigned long long* __restrict__ S, int32_t* __restrict__ flag) {
+                             unsigned long long* __restrict__ S, int32_t* __restrict__ flag,
+                             int pass) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= N) return;
   if (state[u] < 0) return;
@@ -136,7 +130,7 @@ __global__ void k_fill_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
   if (best < cur) {
     unsigned long long old = atomicMin(&S[ru], best);
-    if (best < old) *flag = 1;
+    if (best < old) *flag = pass;
   }
 }
 
@@ -337,7 +331,7 @@ __global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* 
 __global__ void k_ptdir_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
                               const int32_t* __restrict__ deg, const double* __restrict__ fill,
                               const int32_t* __restrict__ ncells, const int32_t* __restrict__ cells,
-                              int32_t* ptdir, int32_t* __restrict__ flag) {
+                              int32_t* ptdir, int32_t* __restrict__ flag, int pass) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= *ncells) return;
   const int c = cells[t];
@@ -357,7 +351,7 @@ __global__ void k_ptdir_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
   if (best != cur) {
     ptdir[c] = best;
-    *flag = 1;
+    *flag = pass;
   }
 }
 
@@ -403,25 +397,37 @@ __global__ void k_pit_marine(int64_t N, const double* __restrict__ hl, double* _
   }
 }
 
-// host helper: run `launch` until the device flag stays 0
+// Host helper: run relaxation passes until one changes nothing.  A pass
+// that changes something writes its (1-based) number into the device flag,
+// so after a batch the flag holds the last changing pass; if a later pass of
+// the batch exists, the fixed point has been reached and confirmed.  The
+// first batch is as long as the previous call needed (`hint`, kept in the
+// mesh handle), so a call usually checks once.
 template <typename F>
-static int until_stable(gospl_mesh* m, hipStream_t s, int max_iter, int check_every, F launch,
-                        int* iters) {
-  int it = 0;
+static int until_stable(gospl_mesh* m, hipStream_t s, int max_iter, int check_every, int* hint,
+                        F launch, int* iters) {
+  int32_t* flag = m->dflag + 8;
+  GOSPL_CHECK(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+  int done = 0;
+  int batch = *hint > check_every ? *hint : check_every;
   for (;;) {
-    GOSPL_CHECK(hipMemsetAsync(m->dflag + 8, 0, sizeof(int32_t), s));
-    for (int k = 0; k < check_every; ++k) launch(m->dflag + 8);
-    it += check_every;
+    for (int k = 0; k < batch; ++k) launch(flag, done + k + 1);
+    done += batch;
     GOSPL_CHECK(hipGetLastError());
-    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 8, m->dflag + 8, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 8, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GOSPL_CHECK(hipStreamSynchronize(s));
-    if (m->hflag[8] == 0) break;
-    if (it >= max_iter) {
+    const int last = m->hflag[8];
+    if (last < done) {
+      *hint = last + 2;  // passes to reach and confirm the fixed point, plus one
+      break;
+    }
+    if (done >= max_iter) {
       set_error("iteration did not converge within %d passes", max_iter);
       return 1;
     }
+    batch = check_every;
   }
-  if (iters) *iters = it;
+  if (iters) *iters = done;
   return 0;
 }
 
@@ -444,12 +450,12 @@ int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, doubl
   k_fill_descent<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S);
   GOSPL_CHECK(hipGetLastError());
   int it = 0;
-  if (until_stable(m, s, (int)N + 16, 3, [&](int32_t* f) {
-        k_jump<<<g, GOSPL_BLOCK, 0, s>>>(N, root, f);
+  if (until_stable(m, s, (int)N + 16, 3, &m->iter_hint[0], [&](int32_t* f, int pass) {
+        k_jump<<<g, GOSPL_BLOCK, 0, s>>>(N, root, f, pass);
       }, &it))
     return 1;
-  if (until_stable(m, s, (int)N + 16, 2, [&](int32_t* f) {
-        k_fill_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, f);
+  if (until_stable(m, s, (int)N + 16, 2, &m->iter_hint[1], [&](int32_t* f, int pass) {
+        k_fill_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, f, pass);
       }, &it))
     return 1;
   k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, sealevel, lfill);
@@ -517,9 +523,9 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   GOSPL_CHECK(hipGetLastError());
   if (npits > 0 && npc > 0) {
     int it = 0;
-    if (until_stable(m, s, npc + 32, 16, [&](int32_t* f) {
+    if (until_stable(m, s, npc + 32, 16, &m->iter_hint[2], [&](int32_t* f, int pass) {
           k_ptdir_relax<<<nblocks(npc), GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, ncells,
-                                                             cells, ptdir, f);
+                                                             cells, ptdir, f, pass);
         }, &it))
       return 1;
   }

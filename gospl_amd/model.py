@@ -199,6 +199,12 @@ class Model:
             self._events = []
         return out
 
+    def _h2d(self, a):
+        """Small host array -> device tensor without waiting for the stream
+        (staged through pinned memory)."""
+        t = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+        return t.to(self.device, non_blocking=True)
+
     def _make_kfac(self):
         if self.sedfacVal is None and self.coeffd == 0.0:
             return None
@@ -234,13 +240,16 @@ class Model:
                    _ptr(self._fvol), ctypes.byref(npits), S)
         n1 = int(npits.value) + 1
         self.npits = n1 - 1
-        self.pitParams = self._pitparams[:, :n1].T.cpu().numpy().copy()
-        spill = self._spill[:n1].cpu().numpy().astype(np.int64)
+        # the per-pit tables in one transfer (spill ids are exact in f64)
+        packed = torch.cat([self._spill[:n1].to(torch.float64), self._pitparams[:, :n1].reshape(-1),
+                            self._flvl[:, :n1].reshape(-1), self._fvol[:, :n1].reshape(-1)]).cpu().numpy()
+        spill = packed[:n1].astype(np.int64)
+        self.pitParams = packed[n1:4 * n1].reshape(3, n1).T.copy()
+        self.filled_lvl = packed[4 * n1:9 * n1].reshape(5, n1).T.copy()
+        self.filled_vol = packed[9 * n1:14 * n1].reshape(5, n1).T.copy()
         self.pitInfo = np.zeros((n1, 2), dtype=np.int64)
         self.pitInfo[:, 0] = spill
         self.pitInfo[:, 1] = np.where(spill >= 0, 0, -1)
-        self.filled_lvl = self._flvl[:, :n1].T.cpu().numpy().copy()
-        self.filled_vol = self._fvol[:, :n1].T.cpu().numpy().copy()
 
     # -------------------------------------------------------------- flow
     def _buildFlowDirection(self, h, down=True):
@@ -288,7 +297,10 @@ class Model:
                 # numpy fancy assignment: a repeated index keeps the last value
                 u, last = np.unique(localPts[::-1], return_index=True)
                 vv = vals[::-1][last]
-                self._nfa[torch.as_tensor(u, device=self.device)] = torch.as_tensor(vv, device=self.device)
+                self._nfa[self._h2d(u)] = self._h2d(vv)
+                nfa_sum = float(np.sum(vv / self.dt))
+            else:
+                nfa_sum = 0.0
             have_nfa = True
         if (eV < 0.0).any():
             eIDs = np.where((eV < 0.0) & (inV > 0.0))[0]
@@ -297,8 +309,8 @@ class Model:
                 nid = np.absolute(self.filled_vol[eIDs] - pitVol[eIDs][:, None]).argmin(axis=1)
                 level[eIDs] = self.filled_lvl[eIDs, nid]
         if full.any() or np.isfinite(level).any():
-            self._full[:npit] = torch.as_tensor(full, device=self.device)
-            self._level[:npit] = torch.as_tensor(level, device=self.device)
+            self._full[:npit] = self._h2d(full)
+            self._level[:npit] = self._h2d(level)
             self._call("gospl_pit_water", self.H, _ptr(self.pitIDs), _ptr(self._full), _ptr(self._level),
                        _ptr(self.lFill), _ptr(self.waterFilled), S)
         nFA = None
@@ -308,7 +320,9 @@ class Model:
             else:
                 self._buildFlowDirection(self.waterFilled)
             rhs = self._nfa / self.dt if have_nfa else torch.zeros_like(self._nfa)
-            resid = float(rhs.sum().item())
+            # sum of rhs: its nonzero entries are the spill values above (host
+            # side, no device synchronisation)
+            resid = nfa_sum if have_nfa else 0.0
             self._cascade_resid = resid
             if resid > self.maxarea[0]:
                 excess = True

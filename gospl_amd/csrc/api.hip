@@ -176,6 +176,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->wdon);
   dfree(m->wacc);
   dfree(m->wslot);
+  dfree(m->wrec);
   dfree(m->trace_buf);
   dfree(m->pg_keys[0]);
   dfree(m->pg_keys[1]);

@@ -28,6 +28,7 @@ struct gospl_mesh {
   int dag_interleave;         // frontier slices interleaved over blocks
   int dag_tail;               // frontier size below which one workgroup finishes the sweep
   int dag_coop;               // launch the persistent DAG kernel with the cooperative API
+  int dag_csr;                // use the CSR rows even when packed records apply
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64
@@ -40,6 +41,7 @@ struct gospl_mesh {
   int32_t* wdon;              // donor rows of the receiver DAG
   double* wacc;               // weighted donor values, one per donor entry
   int32_t* wslot;             // (nrcv, N) slot of each (vertex, receiver) in its receiver's row
+  double* wrec;               // packed vertex records of the DAG sweep (16 words per vertex)
   int64_t wdon_cap;
   long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
   int iter_hint[4];           // passes the relaxation loops needed last time

@@ -155,6 +155,9 @@ struct DagArgs {
   double* acc;          // flow: weighted donor values, one per donor entry
   const int32_t* slot;  // flow: (NR, N) slot of (vertex, receiver k) in acc
   int32_t* pend;        // (N) unsolved dependencies
+  double* rec;          // packed vertex records (nrcv <= 3), 16 words per vertex
+  double* oacc;         // flow: row entries past the record, (c - 8) * N + i
+  const int32_t* odon;  // stream power: row entries past the record, (c - 14) * N + i
 };
 
 // Decrement of a pending counter, returning the old value.  Device scope
@@ -304,6 +307,232 @@ __device__ __forceinline__ void process_vertex(bool valid, int v, const DagArgs&
   }
 }
 
+// ---------------------------------------------------------------------------
+// Packed vertex records (up to three receivers).
+//
+// Past level 0 a level costs as much as the distinct cache lines a vertex
+// touches on its dependent path, so with up to three receivers each vertex
+// keeps what the sweep reads about it in one 128-byte record (16 words), and
+// the values its dependencies produce are pushed into that record:
+//   flow accumulation: w0 = row length, w1 = b, w2..4 = weights, w5..7 =
+//     receivers r[3] then their slots sl[3] (int32), w8..15 = the first 8 row
+//     entries (w * q of each donor, in row order; later entries in `oacc`).
+//     sl[k] = word index of this vertex's entry in receiver k's record, or
+//     -2 - its index in `oacc`, or -1 (none).
+//   stream power: w0 = row length | dependency mask << 32, w1 = h,
+//     w2 = diagonal, w3..5 = off-diagonal coefficients, w6..8 = the receivers'
+//     values (pushed by them), w9..15 = the first 14 row entries as int32 word
+//     indices of the donors' receiver-value slots (later entries in `odon`).
+// A vertex then reads one line, and writes its pushes and counter decrements.
+// ---------------------------------------------------------------------------
+#define REC_W 16
+#define REC_LOG 4
+#define REC_ACC 8
+#define REC_NACC 8
+#define REC_XR 6
+#define REC_TGT 9
+#define REC_NTGT 14
+
+__device__ __forceinline__ int lo32(double d) {
+  return (int)(unsigned int)((unsigned long long)__double_as_longlong(d) & 0xffffffffull);
+}
+__device__ __forceinline__ int hi32(double d) {
+  return (int)(unsigned int)((unsigned long long)__double_as_longlong(d) >> 32);
+}
+
+__global__ void k_rec_init0(int64_t N, int NR, const int32_t* __restrict__ rcv,
+                            const double* __restrict__ coef, const double* __restrict__ rhs,
+                            double* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* r = rec + i * REC_W;
+  int32_t* ri = reinterpret_cast<int32_t*>(r);
+  ri[0] = 0;
+  ri[1] = 0;
+  r[1] = rhs[i];
+  for (int k = 0; k < 3; ++k) {
+    r[2 + k] = k < NR ? coef[(int64_t)k * N + i] : 0.0;
+    ri[10 + k] = k < NR ? rcv[(int64_t)k * N + i] : (int)i;
+    ri[13 + k] = -1;
+  }
+}
+
+// Row of receiver i in FV neighbour order; writes the donors' slots.
+__global__ void k_rec_fill0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
+                            const double* __restrict__ coef, double* __restrict__ rec,
+                            int32_t* __restrict__ pend) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int32_t* ri = reinterpret_cast<int32_t*>(rec);
+  int c = 0;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    const int j = nbr[(int64_t)p * N + i];
+    if (j < 0) continue;
+    for (int k = 0; k < NR; ++k) {
+      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
+        const int sl = c < REC_NACC ? (int)(i * REC_W + REC_ACC + c)
+                                    : (int)(-2 - ((int64_t)(c - REC_NACC) * N + i));
+        ri[(int64_t)j * 2 * REC_W + 13 + k] = sl;
+        c++;
+      }
+    }
+  }
+  ri[i * 2 * REC_W] = c;
+  pend[i] = c;
+}
+
+__global__ void k_rec_init1(int64_t N, int NR, const int32_t* __restrict__ rcv,
+                            const double* __restrict__ coef, const double* __restrict__ diag,
+                            const double* __restrict__ rhs, double* __restrict__ rec,
+                            int32_t* __restrict__ pend) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* r = rec + i * REC_W;
+  int32_t* ri = reinterpret_cast<int32_t*>(r);
+  int mask = 0;
+  for (int k = 0; k < 3; ++k) {
+    double o = 0.0;
+    if (k < NR) {
+      o = coef[(int64_t)k * N + i];
+      if (rcv[(int64_t)k * N + i] != (int)i && o != 0.0) mask |= 1 << k;
+    }
+    r[3 + k] = o;
+  }
+  ri[0] = 0;
+  ri[1] = mask;
+  r[1] = rhs[i];
+  r[2] = diag[i];
+  pend[i] = __popc(mask);
+}
+
+// Row of receiver i (its donors' receiver-value slots) in FV neighbour order.
+__global__ void k_rec_fill1(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
+                            const double* __restrict__ coef, double* __restrict__ rec,
+                            int32_t* __restrict__ odon) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int32_t* ri = reinterpret_cast<int32_t*>(rec + i * REC_W);
+  int c = 0;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    const int j = nbr[(int64_t)p * N + i];
+    if (j < 0) continue;
+    for (int k = 0; k < NR; ++k) {
+      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
+        const int t = (int)((int64_t)j * REC_W + REC_XR + k);
+        if (c < REC_NTGT) ri[2 * REC_TGT + c] = t;
+        else odon[(int64_t)(c - REC_NTGT) * N + i] = t;
+        c++;
+      }
+    }
+  }
+  ri[0] = c;
+}
+
+template <int MODE, class Append, bool WG>
+__device__ __forceinline__ void process_rec(bool valid, int v, const DagArgs& a, const Append& app) {
+  const int64_t N = a.N;
+  if (MODE == 0) {
+    int r[3] = {0, 0, 0};
+    int old[3] = {0, 0, 0};
+    bool dep[3] = {false, false, false};
+    if (valid) {
+      const double2* rp = reinterpret_cast<const double2*>(a.rec + (int64_t)v * REC_W);
+      const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
+      const double2 q4 = rp[4], q5 = rp[5], q6 = rp[6], q7 = rp[7];
+      const int nd = lo32(q0.x);
+      double s = q0.y;
+      const double w[3] = {q1.x, q1.y, q2.x};
+      r[0] = lo32(q2.y);
+      r[1] = hi32(q2.y);
+      r[2] = lo32(q3.x);
+      const int sl[3] = {hi32(q3.x), lo32(q3.y), hi32(q3.y)};
+      const double acc[REC_NACC] = {q4.x, q4.y, q5.x, q5.y, q6.x, q6.y, q7.x, q7.y};
+#pragma unroll
+      for (int c = 0; c < REC_NACC; ++c)
+        if (c < nd) s = s + acc[c];
+      for (int c = REC_NACC; c < nd; ++c) s = s + a.oacc[(int64_t)(c - REC_NACC) * N + v];
+      a.x[v] = s;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (r[k] != v && r[k] >= 0 && w[k] != 0.0) {
+          dep[k] = true;
+          const double val = w[k] * s;
+          if (sl[k] >= 0) a.rec[sl[k]] = val;
+          else if (sl[k] <= -2) a.oacc[-2 - (int64_t)sl[k]] = val;
+          old[k] = dec<WG>(&a.pend[r[k]]);
+        }
+      }
+    }
+    bool rel[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rel[k] = dep[k] && old[k] == 1;
+    append_all<3>(rel, r, 3, app);
+  } else {
+    int j[REC_NTGT];
+    int old[REC_NTGT];
+#pragma unroll
+    for (int c = 0; c < REC_NTGT; ++c) {
+      j[c] = 0;
+      old[c] = 0;
+    }
+    int nd = 0;
+    double xv = 0.0;
+    if (valid) {
+      const double2* rp = reinterpret_cast<const double2*>(a.rec + (int64_t)v * REC_W);
+      const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
+      const double2 q4 = rp[4], q5 = rp[5], q6 = rp[6], q7 = rp[7];
+      nd = lo32(q0.x);
+      const int mask = hi32(q0.x);
+      double s = q0.y;
+      const double dg = q1.x;
+      const double o[3] = {q1.y, q2.x, q2.y};
+      const double xr[3] = {q3.x, q3.y, q4.x};
+      const int t[REC_NTGT] = {lo32(q4.y), hi32(q4.y), lo32(q5.x), hi32(q5.x), lo32(q5.y),
+                               hi32(q5.y), lo32(q6.x), hi32(q6.x), lo32(q6.y), hi32(q6.y),
+                               lo32(q7.x), hi32(q7.x), lo32(q7.y), hi32(q7.y)};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if ((mask >> k) & 1) s = s - o[k] * xr[k];
+      xv = s / dg;
+      a.x[v] = xv;
+#pragma unroll
+      for (int c = 0; c < REC_NTGT; ++c) {
+        if (c < nd) {
+          a.rec[t[c]] = xv;
+          j[c] = t[c] >> REC_LOG;
+          old[c] = dec<WG>(&a.pend[j[c]]);
+        }
+      }
+    }
+    bool rel[REC_NTGT];
+#pragma unroll
+    for (int c = 0; c < REC_NTGT; ++c) rel[c] = c < nd && old[c] == 1;
+    append_all<REC_NTGT>(rel, j, REC_NTGT, app);
+    // rows longer than the record (more than 14 donor entries)
+    for (int c = REC_NTGT; __ballot(c < nd) != 0ull; ++c) {
+      bool rl[1] = {false};
+      int jj[1] = {0};
+      if (c < nd) {
+        const int tc = a.odon[(int64_t)(c - REC_NTGT) * N + v];
+        a.rec[tc] = xv;
+        jj[0] = tc >> REC_LOG;
+        rl[0] = dec<WG>(&a.pend[jj[0]]) == 1;
+      }
+      append_all<1>(rl, jj, 1, app);
+    }
+  }
+}
+
+template <int MODE, bool REC, bool WG, class Append>
+__device__ __forceinline__ void solve_vertex(bool valid, int v, const DagArgs& a, const Append& app) {
+  if (REC) process_rec<MODE, Append, WG>(valid, v, a, app);
+  else process_vertex<MODE, Append, WG>(valid, v, a, app);
+}
+
 // Appends into a frontier list in global memory; `blk` (LDS, may be null)
 // accumulates the workgroup's appended count.
 struct GlobalAppend {
@@ -322,14 +551,17 @@ struct GlobalAppend {
 // dependency count is recomputed from the rows (pend[] is being decremented
 // by other threads).  Same-address atomics serialise (~25 ns each on MI355X),
 // so the level-0 size is not counted.
-template <int MODE>
+template <int MODE, bool REC>
 __global__ void k_dag_level0(DagArgs a, int32_t* __restrict__ fb, int32_t* __restrict__ counts) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t - (t & 63) >= a.N) return;  // whole wavefront beyond the mesh
   bool valid = t < a.N;
   const int v = (int)t;
   if (valid) {
-    if (MODE == 0) {
+    if (REC) {
+      const double w0 = a.rec[(int64_t)v * REC_W];
+      valid = MODE == 0 ? lo32(w0) == 0 : hi32(w0) == 0;
+    } else if (MODE == 0) {
       valid = a.doff[v + 1] == a.doff[v];
     } else {
       for (int k = 0; k < a.NR && valid; ++k) {
@@ -338,13 +570,13 @@ __global__ void k_dag_level0(DagArgs a, int32_t* __restrict__ fb, int32_t* __res
       }
     }
   }
-  process_vertex<MODE>(valid, v, a, GlobalAppend{&counts[1], fb, nullptr});
+  solve_vertex<MODE, REC, false>(valid, v, a, GlobalAppend{&counts[1], fb, nullptr});
 }
 
 // Levels 1.. on a small co-resident grid.  Stops at the first level whose
 // frontier is empty or smaller than tail_n (k_dag_tail continues there);
 // *nlev = that level.
-template <int MODE>
+template <int MODE, bool REC>
 __global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
     DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb, int32_t* __restrict__ counts,
     unsigned long long* __restrict__ bar, int32_t* __restrict__ nlev, long long* __restrict__ trace,
@@ -380,7 +612,7 @@ __global__ __launch_bounds__(DAG_BLOCK) void k_dag_level(
       const int64_t t = t0 + (tid & 63);
       const bool valid = t < n;
       const int v = valid ? cur[t] : 0;
-      process_vertex<MODE>(valid, v, a, app);
+      solve_vertex<MODE, REC, false>(valid, v, a, app);
     }
     __syncthreads();
     n = grid_barrier_count(bar, gridDim.x, level, s_app[level & 1]);
@@ -410,7 +642,7 @@ struct TailAppend {
 // k_dag_level stopped (*nlev).  Frontiers live in LDS and levels are separated
 // by workgroup barriers instead of a device-wide barrier with its cache
 // write-back/invalidate.
-template <int MODE>
+template <int MODE, bool REC>
 __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
     DagArgs a, int32_t* __restrict__ fa, int32_t* __restrict__ fb,
     const int32_t* __restrict__ counts, const unsigned long long* __restrict__ bar,
@@ -441,7 +673,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
       const bool valid = t < n;
       int v = 0;
       if (valid) v = (first || t >= TAIL_Q) ? gcur[t] : sq[p][t];
-      process_vertex<MODE, TailAppend, true>(valid, v, a, app);
+      solve_vertex<MODE, REC, true>(valid, v, a, app);
     }
     __syncthreads();
     n = sc[(level + 1) % 3];
@@ -486,21 +718,61 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 // the solved vertices fixed, in one workgroup, until the largest change is at
 // most 1e-14 of the largest value.  Each pass sums a row in its fixed order,
 // so the result does not depend on scheduling.  res[0] = 1 when converged.
+// Pushes w_k * s of vertex v into its receivers' rows.
+template <bool REC>
+__device__ __forceinline__ void cycle_push(const DagArgs& a, int v, double s) {
+  const int64_t N = a.N;
+  if (REC) {
+    const double* r = a.rec + (int64_t)v * REC_W;
+    const int32_t* ri = reinterpret_cast<const int32_t*>(r);
+    for (int k = 0; k < 3; ++k) {
+      const int rk = ri[10 + k];
+      const int sl = ri[13 + k];
+      const double w = r[2 + k];
+      if (rk != v && rk >= 0 && w != 0.0) {
+        if (sl >= 0) a.rec[sl] = w * s;
+        else if (sl <= -2) a.oacc[-2 - (int64_t)sl] = w * s;
+      }
+    }
+  } else {
+    for (int k = 0; k < a.NR; ++k) {
+      const int r = a.rcv[(int64_t)k * N + v];
+      const double w = a.coef[(int64_t)k * N + v];
+      const int sl = a.slot[(int64_t)k * N + v];
+      if (r != v && r >= 0 && w != 0.0 && sl >= 0) a.acc[sl] = w * s;
+    }
+  }
+}
+
+// b_v plus the row of v, in row order.
+template <bool REC>
+__device__ __forceinline__ double cycle_row(const DagArgs& a, int v) {
+  const int64_t N = a.N;
+  if (REC) {
+    const double* r = a.rec + (int64_t)v * REC_W;
+    const int nd = reinterpret_cast<const int32_t*>(r)[0];
+    double s = r[1];
+    for (int c = 0; c < nd; ++c)
+      s = s + (c < REC_NACC ? r[REC_ACC + c] : a.oacc[(int64_t)(c - REC_NACC) * N + v]);
+    return s;
+  }
+  double s = a.rhs[v];
+  const int e = a.doff[v + 1];
+  for (int c = a.doff[v]; c < e; ++c) s = s + a.acc[c];
+  return s;
+}
+
+template <bool REC>
 __global__ __launch_bounds__(CYCLE_THREADS) void k_flow_cycles(DagArgs a, const int32_t* __restrict__ U,
                                                              const int32_t* __restrict__ nU,
                                                              double* __restrict__ xn,
                                                              int32_t* __restrict__ res) {
   __shared__ double sh[3][CYCLE_THREADS / 64];
   const int n = *nU;
-  const int64_t N = a.N;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int v = U[t];
     a.x[v] = 0.0;
-    for (int k = 0; k < a.NR; ++k) {
-      const int r = a.rcv[(int64_t)k * N + v];
-      const int sl = a.slot[(int64_t)k * N + v];
-      if (r != v && r >= 0 && a.coef[(int64_t)k * N + v] != 0.0 && sl >= 0) a.acc[sl] = 0.0;
-    }
+    cycle_push<REC>(a, v, 0.0);
   }
   __syncthreads();
   int conv = 0;
@@ -508,9 +780,7 @@ __global__ __launch_bounds__(CYCLE_THREADS) void k_flow_cycles(DagArgs a, const 
     double dmax = 0.0, xmax = 0.0, bad = 0.0;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
       const int v = U[t];
-      double s = a.rhs[v];
-      const int e = a.doff[v + 1];
-      for (int c = a.doff[v]; c < e; ++c) s = s + a.acc[c];
+      const double s = cycle_row<REC>(a, v);
       xn[t] = s;
       if (!isfinite(s)) bad = 1.0;
       dmax = fmax(dmax, fabs(s - a.x[v]));
@@ -524,12 +794,7 @@ __global__ __launch_bounds__(CYCLE_THREADS) void k_flow_cycles(DagArgs a, const 
       const int v = U[t];
       const double s = xn[t];
       a.x[v] = s;
-      for (int k = 0; k < a.NR; ++k) {
-        const int r = a.rcv[(int64_t)k * N + v];
-        const double w = a.coef[(int64_t)k * N + v];
-        const int sl = a.slot[(int64_t)k * N + v];
-        if (r != v && r >= 0 && w != 0.0 && sl >= 0) a.acc[sl] = w * s;
-      }
+      cycle_push<REC>(a, v, s);
     }
     __syncthreads();
     if (bad != 0.0) break;
@@ -590,6 +855,12 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
+static int ensure_records(gospl_mesh* m) {
+  if (m->wrec) return 0;
+  GOSPL_CHECK(hipMalloc(&m->wrec, sizeof(double) * REC_W * (m->N + 64)));
+  return 0;
+}
+
 static int ensure_donors(gospl_mesh* m, int64_t cap) {
   if (m->wdon_cap >= cap) return 0;
   if (m->wdon) GOSPL_CHECK(hipFree(m->wdon));
@@ -617,21 +888,38 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* doff = m->wi[5];
   int32_t* counts = m->counts;
   int32_t* dev = m->dflag;
-  if (ensure_donors(m, (int64_t)nrcv * N)) return 1;
+  const bool rec = nrcv <= 3 && !m->dag_csr && N * REC_W < (int64_t)INT32_MAX;  // int32 word indices
   GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (N + 2), s));
   GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
   unsigned long long* bar = reinterpret_cast<unsigned long long*>(dev + 32);
   GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned long long) * 3, s));
-  int32_t* slot = mode == 0 ? m->wslot : nullptr;
-  if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
-  // donor rows
-  k_dep_count<<<nblocks(N + 1), GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, dcnt);
-  size_t bytes = m->cub_tmp_bytes;
-  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
-  k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
-  k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend);
+  int32_t* slot = nullptr;
+  if (rec) {
+    // packed records; rows hold at most nrcv entries per neighbour, the ones
+    // past the record go to the overflow arrays
+    if (ensure_records(m)) return 1;
+    if (ensure_donors(m, (int64_t)nrcv * m->W * N)) return 1;
+    if (mode == 0) {
+      k_rec_init0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rcv, coef, rhs, m->wrec);
+      k_rec_fill0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, m->wrec, pend);
+    } else {
+      k_rec_init1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rcv, coef, diag, rhs, m->wrec, pend);
+      k_rec_fill1<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, m->wrec, m->wdon);
+    }
+  } else {
+    // CSR donor rows
+    if (ensure_donors(m, (int64_t)nrcv * N)) return 1;
+    slot = mode == 0 ? m->wslot : nullptr;
+    if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
+    k_dep_count<<<nblocks(N + 1), GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, dcnt);
+    size_t bytes = m->cub_tmp_bytes;
+    GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
+    k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
+    k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend);
+  }
   GOSPL_CHECK(hipGetLastError());
-  DagArgs a{N, nrcv, rcv, coef, diag, rhs, x, doff, m->wdon, m->wacc, slot, pend};
+  DagArgs a{N, nrcv, rcv, coef, diag, rhs, x, doff, m->wdon, m->wacc, slot, pend,
+            rec ? m->wrec : nullptr, m->wacc, m->wdon};
   int32_t* nlev = dev;
   // GOSPL_DAG_TRACE=1: per-level wall-clock stamps and frontier sizes, summarised on stderr
   static const bool do_trace = getenv("GOSPL_DAG_TRACE") && atoi(getenv("GOSPL_DAG_TRACE")) > 0;
@@ -642,22 +930,29 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     GOSPL_CHECK(hipMemsetAsync(trace, 0, sizeof(long long) * 2 * 8192, s));
   }
   void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
-  const void* kern = mode == 0 ? (const void*)k_dag_level<0> : (const void*)k_dag_level<1>;
+  const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
+                        : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
-  if (mode == 0)
-    k_dag_level0<0><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-  else
-    k_dag_level0<1><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+  if (rec) {
+    if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+    else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+  } else {
+    if (mode == 0) k_dag_level0<0, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+    else k_dag_level0<1, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+  }
   GOSPL_CHECK(hipGetLastError());
   if (m->dag_coop)
     GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
   else  // plain launch (counter collection); the grid fits the GPU many times over
     GOSPL_CHECK(hipLaunchKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
   if (m->dag_tail > 0) {
-    if (mode == 0)
-      k_dag_tail<0><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
-    else
-      k_dag_tail<1><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+    if (rec) {
+      if (mode == 0) k_dag_tail<0, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      else k_dag_tail<1, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+    } else {
+      if (mode == 0) k_dag_tail<0, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      else k_dag_tail<1, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+    }
     GOSPL_CHECK(hipGetLastError());
   }
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
@@ -673,7 +968,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   }
   if (mode == 0 && m->hflag[1] > 0) {
     // a receiver cycle: iterate on what the sweep left over
-    k_flow_cycles<<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
+    if (rec) k_flow_cycles<true><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
+    else k_flow_cycles<false><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
     GOSPL_CHECK(hipGetLastError());
     GOSPL_CHECK(hipMemcpyAsync(m->hflag + 2, dev + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GOSPL_CHECK(hipStreamSynchronize(s));
@@ -731,8 +1027,12 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* te_tail = getenv("GOSPL_DAG_TAIL");
   if (te_tail && atoi(te_tail) >= 0) m->dag_tail = atoi(te_tail);
   int per_cu = 1 << 30;
-  const void* ks[2] = {(const void*)k_dag_level<0>, (const void*)k_dag_level<1>};
-  for (int i = 0; i < 2; ++i) {
+  // GOSPL_DAG_CSR=1 keeps the CSR rows even for three receivers or fewer
+  const char* te_csr = getenv("GOSPL_DAG_CSR");
+  m->dag_csr = te_csr && atoi(te_csr) > 0;
+  const void* ks[4] = {(const void*)k_dag_level<0, false>, (const void*)k_dag_level<1, false>,
+                       (const void*)k_dag_level<0, true>, (const void*)k_dag_level<1, true>};
+  for (int i = 0; i < 4; ++i) {
     int b = 0;
     GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, ks[i], m->dag_threads, 0));
     per_cu = b < per_cu ? b : per_cu;

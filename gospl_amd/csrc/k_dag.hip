@@ -340,81 +340,93 @@ __device__ __forceinline__ int hi32(double d) {
   return (int)(unsigned int)((unsigned long long)__double_as_longlong(d) >> 32);
 }
 
-__global__ void k_rec_init0(int64_t N, int NR, const int32_t* __restrict__ rcv,
-                            const double* __restrict__ coef, const double* __restrict__ rhs,
-                            double* __restrict__ rec) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  double* r = rec + i * REC_W;
-  int32_t* ri = reinterpret_cast<int32_t*>(r);
-  ri[0] = 0;
-  ri[1] = 0;
-  r[1] = rhs[i];
-  for (int k = 0; k < 3; ++k) {
-    r[2 + k] = k < NR ? coef[(int64_t)k * N + i] : 0.0;
-    ri[10 + k] = k < NR ? rcv[(int64_t)k * N + i] : (int)i;
-    ri[13 + k] = -1;
-  }
+__device__ __forceinline__ double pack32(int lo, int hi) {
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) |
+                                          (unsigned long long)(unsigned int)lo));
 }
 
-// Row of receiver i in FV neighbour order; writes the donors' slots.
-__global__ void k_rec_fill0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+// Row of receiver i as a bitmask over (neighbour position p, receiver slot k),
+// bit p * NR + k: set when neighbour p sends flow to i through its slot k.
+// Bit order is row order.  pend[i] = row length.
+__global__ void k_rec_mask0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                             const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
-                            const double* __restrict__ coef, double* __restrict__ rec,
+                            const double* __restrict__ coef, unsigned long long* __restrict__ rowmask,
                             int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  int32_t* ri = reinterpret_cast<int32_t*>(rec);
-  int c = 0;
+  unsigned long long mk = 0ull;
   const int dg = deg[i];
   for (int p = 0; p < W && p < dg; ++p) {
     const int j = nbr[(int64_t)p * N + i];
     if (j < 0) continue;
-    for (int k = 0; k < NR; ++k) {
-      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
-        const int sl = c < REC_NACC ? (int)(i * REC_W + REC_ACC + c)
-                                    : (int)(-2 - ((int64_t)(c - REC_NACC) * N + i));
-        ri[(int64_t)j * 2 * REC_W + 13 + k] = sl;
-        c++;
+    for (int k = 0; k < NR; ++k)
+      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0)
+        mk |= 1ull << (p * NR + k);
+  }
+  rowmask[i] = mk;
+  pend[i] = __popcll(mk);
+}
+
+// Whole flow-accumulation record of vertex i, written at once.  The slot of
+// (i, receiver k) is the rank of its bit in the receiver's row mask.
+__global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
+                             const double* __restrict__ coef, const double* __restrict__ rhs,
+                             const unsigned long long* __restrict__ rowmask, double* __restrict__ rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int r[3], sl[3];
+  double w[3];
+  for (int k = 0; k < 3; ++k) {
+    r[k] = (int)i;
+    w[k] = 0.0;
+    sl[k] = -1;
+    if (k >= NR) continue;
+    r[k] = rcv[(int64_t)k * N + i];
+    w[k] = coef[(int64_t)k * N + i];
+    const int rk = r[k];
+    if (rk == (int)i || rk < 0 || w[k] == 0.0) continue;
+    const int dr = deg[rk];
+    for (int p = 0; p < W && p < dr; ++p) {
+      if (nbr[(int64_t)p * N + rk] == (int)i) {
+        const unsigned long long below = rowmask[rk] & ((1ull << (p * NR + k)) - 1ull);
+        const int c = __popcll(below);
+        sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
+                             : (int)(-2 - ((int64_t)(c - REC_NACC) * N + rk));
+        break;
       }
     }
   }
-  ri[i * 2 * REC_W] = c;
-  pend[i] = c;
+  double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
+  out[0] = make_double2(pack32(__popcll(rowmask[i]), 0), rhs[i]);
+  out[1] = make_double2(w[0], w[1]);
+  out[2] = make_double2(w[2], pack32(r[0], r[1]));
+  out[3] = make_double2(pack32(r[2], sl[0]), pack32(sl[1], sl[2]));
+  out[4] = make_double2(0.0, 0.0);
+  out[5] = make_double2(0.0, 0.0);
+  out[6] = make_double2(0.0, 0.0);
+  out[7] = make_double2(0.0, 0.0);
 }
 
-__global__ void k_rec_init1(int64_t N, int NR, const int32_t* __restrict__ rcv,
-                            const double* __restrict__ coef, const double* __restrict__ diag,
-                            const double* __restrict__ rhs, double* __restrict__ rec,
-                            int32_t* __restrict__ pend) {
+// Whole stream-power record of vertex i, written at once: its dependency
+// mask, coefficients and its row (the donors' receiver-value slots, in FV
+// neighbour order; entries past the record go to `odon`).  pend[i] = number
+// of receivers i depends on.
+__global__ void k_rec_build1(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
+                             const double* __restrict__ coef, const double* __restrict__ diag,
+                             const double* __restrict__ rhs, double* __restrict__ rec,
+                             int32_t* __restrict__ odon, int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double* r = rec + i * REC_W;
-  int32_t* ri = reinterpret_cast<int32_t*>(r);
   int mask = 0;
-  for (int k = 0; k < 3; ++k) {
-    double o = 0.0;
-    if (k < NR) {
-      o = coef[(int64_t)k * N + i];
-      if (rcv[(int64_t)k * N + i] != (int)i && o != 0.0) mask |= 1 << k;
-    }
-    r[3 + k] = o;
+  double o[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < NR && k < 3; ++k) {
+    o[k] = coef[(int64_t)k * N + i];
+    if (rcv[(int64_t)k * N + i] != (int)i && o[k] != 0.0) mask |= 1 << k;
   }
-  ri[0] = 0;
-  ri[1] = mask;
-  r[1] = rhs[i];
-  r[2] = diag[i];
-  pend[i] = __popc(mask);
-}
-
-// Row of receiver i (its donors' receiver-value slots) in FV neighbour order.
-__global__ void k_rec_fill1(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
-                            const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
-                            const double* __restrict__ coef, double* __restrict__ rec,
-                            int32_t* __restrict__ odon) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  int32_t* ri = reinterpret_cast<int32_t*>(rec + i * REC_W);
+  int t[REC_NTGT];
+  for (int c = 0; c < REC_NTGT; ++c) t[c] = 0;
   int c = 0;
   const int dg = deg[i];
   for (int p = 0; p < W && p < dg; ++p) {
@@ -422,14 +434,28 @@ __global__ void k_rec_fill1(int64_t N, int W, int NR, const int32_t* __restrict_
     if (j < 0) continue;
     for (int k = 0; k < NR; ++k) {
       if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
-        const int t = (int)((int64_t)j * REC_W + REC_XR + k);
-        if (c < REC_NTGT) ri[2 * REC_TGT + c] = t;
-        else odon[(int64_t)(c - REC_NTGT) * N + i] = t;
+        const int tj = (int)((int64_t)j * REC_W + REC_XR + k);
+        if (c < REC_NTGT) {
+#pragma unroll
+          for (int q = 0; q < REC_NTGT; ++q)
+            if (q == c) t[q] = tj;
+        } else {
+          odon[(int64_t)(c - REC_NTGT) * N + i] = tj;
+        }
         c++;
       }
     }
   }
-  ri[0] = c;
+  pend[i] = __popc(mask);
+  double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
+  out[0] = make_double2(pack32(c, mask), rhs[i]);
+  out[1] = make_double2(diag[i], o[0]);
+  out[2] = make_double2(o[1], o[2]);
+  out[3] = make_double2(0.0, 0.0);
+  out[4] = make_double2(0.0, pack32(t[0], t[1]));
+  out[5] = make_double2(pack32(t[2], t[3]), pack32(t[4], t[5]));
+  out[6] = make_double2(pack32(t[6], t[7]), pack32(t[8], t[9]));
+  out[7] = make_double2(pack32(t[10], t[11]), pack32(t[12], t[13]));
 }
 
 template <int MODE, class Append, bool WG>
@@ -888,7 +914,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* doff = m->wi[5];
   int32_t* counts = m->counts;
   int32_t* dev = m->dflag;
-  const bool rec = nrcv <= 3 && !m->dag_csr && N * REC_W < (int64_t)INT32_MAX;  // int32 word indices
+  // packed records: up to 3 receivers, int32 word indices, row masks of W * nrcv <= 64 bits
+  const bool rec = nrcv <= 3 && !m->dag_csr && N * REC_W < (int64_t)INT32_MAX && m->W * nrcv <= 64;
   GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (N + 2), s));
   GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
   unsigned long long* bar = reinterpret_cast<unsigned long long*>(dev + 32);
@@ -900,11 +927,13 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (ensure_records(m)) return 1;
     if (ensure_donors(m, (int64_t)nrcv * m->W * N)) return 1;
     if (mode == 0) {
-      k_rec_init0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rcv, coef, rhs, m->wrec);
-      k_rec_fill0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, m->wrec, pend);
+      unsigned long long* rowmask = m->wu[0];
+      k_rec_mask0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, rowmask, pend);
+      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, rhs, rowmask,
+                                              m->wrec);
     } else {
-      k_rec_init1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rcv, coef, diag, rhs, m->wrec, pend);
-      k_rec_fill1<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, m->wrec, m->wdon);
+      k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, diag, rhs,
+                                              m->wrec, m->wdon, pend);
     }
   } else {
     // CSR donor rows

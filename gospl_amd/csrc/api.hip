@@ -177,6 +177,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->wacc);
   dfree(m->wslot);
   dfree(m->wrec);
+  dfree(m->rev);
   dfree(m->trace_buf);
   dfree(m->pg_keys[0]);
   dfree(m->pg_keys[1]);

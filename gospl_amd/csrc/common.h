@@ -42,6 +42,7 @@ struct gospl_mesh {
   double* wacc;               // weighted donor values, one per donor entry
   int32_t* wslot;             // (nrcv, N) slot of each (vertex, receiver) in its receiver's row
   double* wrec;               // packed vertex records of the DAG sweep (16 words per vertex)
+  int8_t* rev;                // (W, N) position of i in the neighbour list of its neighbour q
   int64_t wdon_cap;
   long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
   int iter_hint[4];           // passes the relaxation loops needed last time

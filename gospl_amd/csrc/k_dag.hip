@@ -345,34 +345,68 @@ __device__ __forceinline__ double pack32(int lo, int hi) {
                                           (unsigned long long)(unsigned int)lo));
 }
 
-// Row of receiver i as a bitmask over (neighbour position p, receiver slot k),
-// bit p * NR + k: set when neighbour p sends flow to i through its slot k.
-// Bit order is row order.  pend[i] = row length.
-__global__ void k_rec_mask0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
-                            const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
-                            const double* __restrict__ coef, unsigned long long* __restrict__ rowmask,
-                            int32_t* __restrict__ pend) {
+// Reverse neighbour positions (static per mesh): rev[q * N + i] = position of
+// i in the neighbour list of its neighbour q, -1 if absent.
+__global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
+                          const int32_t* __restrict__ deg, int8_t* __restrict__ rev) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  unsigned long long mk = 0ull;
   const int dg = deg[i];
-  for (int p = 0; p < W && p < dg; ++p) {
-    const int j = nbr[(int64_t)p * N + i];
-    if (j < 0) continue;
-    for (int k = 0; k < NR; ++k)
-      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0)
-        mk |= 1ull << (p * NR + k);
+  for (int q = 0; q < W; ++q) {
+    int pos = -1;
+    if (q < dg) {
+      const int j = nbr[(int64_t)q * N + i];
+      if (j >= 0) {
+        const int dj = deg[j];
+        for (int p = 0; p < W && p < dj; ++p)
+          if (nbr[(int64_t)p * N + j] == (int)i) {
+            pos = p;
+            break;
+          }
+      }
+    }
+    rev[(int64_t)q * N + i] = (int8_t)pos;
   }
-  rowmask[i] = mk;
-  pend[i] = __popcll(mk);
+}
+
+// Position of receiver r in the neighbour list of i, then of i in that of r.
+__device__ __forceinline__ int rev_pos(int64_t N, int W, const int32_t* __restrict__ nbr,
+                                       const int32_t* __restrict__ deg,
+                                       const int8_t* __restrict__ rev, int64_t i, int r) {
+  const int dg = deg[i];
+  for (int q = 0; q < W && q < dg; ++q)
+    if (nbr[(int64_t)q * N + i] == r) return rev[(int64_t)q * N + i];
+  return -1;
+}
+
+// Row masks: receiver r's row is the bitmask over (neighbour position p,
+// receiver slot k), bit p * NR + k, of the donors that send to r through
+// their slot k; bit order is row order (FV neighbour order, then slot).  Set
+// from the donor side, one atomicOr per dependency (the mask is zeroed
+// before).
+__global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                           const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
+                           const int32_t* __restrict__ rcv, const double* __restrict__ coef,
+                           unsigned long long* __restrict__ rowmask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int k = 0; k < NR; ++k) {
+    const int r = rcv[(int64_t)k * N + i];
+    if (r == (int)i || r < 0 || coef[(int64_t)k * N + i] == 0.0) continue;
+    const int p = rev_pos(N, W, nbr, deg, rev, i, r);
+    if (p >= 0) atomicOr(&rowmask[r], 1ull << (p * NR + k));
+  }
 }
 
 // Whole flow-accumulation record of vertex i, written at once.  The slot of
 // (i, receiver k) is the rank of its bit in the receiver's row mask.
+// pend[i] = row length.
 __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
-                             const double* __restrict__ coef, const double* __restrict__ rhs,
-                             const unsigned long long* __restrict__ rowmask, double* __restrict__ rec) {
+                             const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
+                             const int32_t* __restrict__ rcv, const double* __restrict__ coef,
+                             const double* __restrict__ rhs,
+                             const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
+                             int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int r[3], sl[3];
@@ -386,19 +420,16 @@ __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict
     w[k] = coef[(int64_t)k * N + i];
     const int rk = r[k];
     if (rk == (int)i || rk < 0 || w[k] == 0.0) continue;
-    const int dr = deg[rk];
-    for (int p = 0; p < W && p < dr; ++p) {
-      if (nbr[(int64_t)p * N + rk] == (int)i) {
-        const unsigned long long below = rowmask[rk] & ((1ull << (p * NR + k)) - 1ull);
-        const int c = __popcll(below);
-        sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
-                             : (int)(-2 - ((int64_t)(c - REC_NACC) * N + rk));
-        break;
-      }
-    }
+    const int p = rev_pos(N, W, nbr, deg, rev, i, rk);
+    if (p < 0) continue;
+    const int c = __popcll(rowmask[rk] & ((1ull << (p * NR + k)) - 1ull));
+    sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
+                         : (int)(-2 - ((int64_t)(c - REC_NACC) * N + rk));
   }
+  const int nd = __popcll(rowmask[i]);
+  pend[i] = nd;
   double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
-  out[0] = make_double2(pack32(__popcll(rowmask[i]), 0), rhs[i]);
+  out[0] = make_double2(pack32(nd, 0), rhs[i]);
   out[1] = make_double2(w[0], w[1]);
   out[2] = make_double2(w[2], pack32(r[0], r[1]));
   out[3] = make_double2(pack32(r[2], sl[0]), pack32(sl[1], sl[2]));
@@ -409,13 +440,13 @@ __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict
 }
 
 // Whole stream-power record of vertex i, written at once: its dependency
-// mask, coefficients and its row (the donors' receiver-value slots, in FV
-// neighbour order; entries past the record go to `odon`).  pend[i] = number
-// of receivers i depends on.
-__global__ void k_rec_build1(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
-                             const double* __restrict__ coef, const double* __restrict__ diag,
-                             const double* __restrict__ rhs, double* __restrict__ rec,
+// mask, coefficients and its row (the donors' receiver-value slots, in row
+// order, from its row mask; entries past the record go to `odon`).
+// pend[i] = number of receivers i depends on.
+__global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
+                             const int32_t* __restrict__ rcv, const double* __restrict__ coef,
+                             const double* __restrict__ diag, const double* __restrict__ rhs,
+                             const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
                              int32_t* __restrict__ odon, int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -426,29 +457,27 @@ __global__ void k_rec_build1(int64_t N, int W, int NR, const int32_t* __restrict
     if (rcv[(int64_t)k * N + i] != (int)i && o[k] != 0.0) mask |= 1 << k;
   }
   int t[REC_NTGT];
-  for (int c = 0; c < REC_NTGT; ++c) t[c] = 0;
-  int c = 0;
-  const int dg = deg[i];
-  for (int p = 0; p < W && p < dg; ++p) {
-    const int j = nbr[(int64_t)p * N + i];
-    if (j < 0) continue;
-    for (int k = 0; k < NR; ++k) {
-      if (rcv[(int64_t)k * N + j] == (int)i && coef[(int64_t)k * N + j] != 0.0) {
-        const int tj = (int)((int64_t)j * REC_W + REC_XR + k);
-        if (c < REC_NTGT) {
 #pragma unroll
-          for (int q = 0; q < REC_NTGT; ++q)
-            if (q == c) t[q] = tj;
-        } else {
-          odon[(int64_t)(c - REC_NTGT) * N + i] = tj;
-        }
-        c++;
-      }
+  for (int c = 0; c < REC_NTGT; ++c) t[c] = 0;
+  unsigned long long mk = rowmask[i];
+  const int nd = __popcll(mk);
+  for (int c = 0; mk != 0ull; ++c) {
+    const int b = __ffsll((long long)mk) - 1;
+    mk &= mk - 1ull;
+    const int p = b / NR, k = b - p * NR;
+    const int j = nbr[(int64_t)p * N + i];
+    const int tj = (int)((int64_t)j * REC_W + REC_XR + k);
+    if (c < REC_NTGT) {
+#pragma unroll
+      for (int q = 0; q < REC_NTGT; ++q)
+        if (q == c) t[q] = tj;
+    } else {
+      odon[(int64_t)(c - REC_NTGT) * N + i] = tj;
     }
   }
   pend[i] = __popc(mask);
   double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
-  out[0] = make_double2(pack32(c, mask), rhs[i]);
+  out[0] = make_double2(pack32(nd, mask), rhs[i]);
   out[1] = make_double2(diag[i], o[0]);
   out[2] = make_double2(o[1], o[2]);
   out[3] = make_double2(0.0, 0.0);
@@ -881,9 +910,12 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
-static int ensure_records(gospl_mesh* m) {
+static int ensure_records(gospl_mesh* m, hipStream_t s) {
   if (m->wrec) return 0;
   GOSPL_CHECK(hipMalloc(&m->wrec, sizeof(double) * REC_W * (m->N + 64)));
+  GOSPL_CHECK(hipMalloc(&m->rev, sizeof(int8_t) * m->W * (m->N + 64)));
+  k_rev_nbr<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, m->W, m->nbr, m->deg, m->rev);
+  GOSPL_CHECK(hipGetLastError());
   return 0;
 }
 
@@ -924,17 +956,17 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   if (rec) {
     // packed records; rows hold at most nrcv entries per neighbour, the ones
     // past the record go to the overflow arrays
-    if (ensure_records(m)) return 1;
+    if (ensure_records(m, s)) return 1;
     if (ensure_donors(m, (int64_t)nrcv * m->W * N)) return 1;
-    if (mode == 0) {
-      unsigned long long* rowmask = m->wu[0];
-      k_rec_mask0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, rowmask, pend);
-      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, rhs, rowmask,
-                                              m->wrec);
-    } else {
-      k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, diag, rhs,
-                                              m->wrec, m->wdon, pend);
-    }
+    unsigned long long* rowmask = m->wu[0];
+    GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
+    k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask);
+    if (mode == 0)
+      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rhs,
+                                              rowmask, m->wrec, pend);
+    else
+      k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rcv, coef, diag, rhs, rowmask, m->wrec,
+                                              m->wdon, pend);
   } else {
     // CSR donor rows
     if (ensure_donors(m, (int64_t)nrcv * N)) return 1;

@@ -82,3 +82,35 @@ def make_cycle(rcv, wgt, drain=True):
                 rcv[b, 0], wgt[b, 0] = a, 1.0
             return rcv, wgt, (a, b)
     raise ValueError("no vertex pair for a cycle")
+
+
+def spl_steady_case(n: int = 30, U: float = 1.0e-3, K: float = 1.0e-5):
+    """goSPL's stream-power benchmark (benchmarks/test_spl.py:443-577): a
+    planar mesh with draining borders, uniform uplift U, single-direction
+    routing, no diffusion, run to steady state, where E = U and the
+    steepness index ks = S * A^(m/n) = (U/K)^(1/n) everywhere on channels."""
+    m, h0, p = case("planar", n, 3)
+    q = dict(p)
+    interior = m.outlet == 0
+    q.update(K=K, m=0.5, d=0.0, flowdir=1, cda=0.0, cdm=0.0, dt=2.0e4, rain=1.0, sealevel=-1.0e4,
+             upsub=np.where(interior, U, 0.0))
+    return m, h0, q, interior
+
+
+def spl_steady_metrics(h, rcv, dist, fa, interior, U, K, h_prev, dt_span, dt):
+    """E/U over the last span, and on channel nodes (drainage area above 5 %
+    of the largest): ks median / (U/K), its relative spread (IQR / median)
+    and the fraction of nodes whose ks is within 1 % of U/K."""
+    rcv = np.asarray(rcv)
+    d = np.asarray(dist)
+    # slopes of the eroded surface, before the step's uplift (which lifts a
+    # node above a fixed border receiver by U * dt)
+    he = h - np.where(interior, U * dt, 0.0)
+    S = np.where(d > 0, (he - he[rcv]) / np.where(d > 0, d, 1.0), 0.0)
+    E = U - (h - h_prev)[interior].mean() / dt_span
+    ch = interior & (fa > 0.05 * fa[interior].max()) & (S > 1e-12)
+    ks = S[ch] * fa[ch] ** 0.5
+    med = float(np.median(ks))
+    iqr = float(np.percentile(ks, 75) - np.percentile(ks, 25)) / med
+    within = float(np.mean(np.abs(ks / (U / K) - 1.0) < 0.01))
+    return E / U, med / (U / K), iqr, within, int(ch.sum())

@@ -177,3 +177,31 @@ def test_other_receiver_counts(flowdir):
     q = dict(p)
     q["flowdir"] = flowdir
     _compare_steps(m, h0, q, nsteps=2)
+
+
+def test_spl_steady_state_analytic_gpu():
+    """The HIP path reaches goSPL's analytic stream-power steady state
+    (benchmarks/test_spl.py:443-577): E = U, median ks = U/K within 0.5 %,
+    ks spread below 10 %, on a 60x60 plane."""
+    from gospl_amd.model import Model
+    from tests.helpers import spl_steady_case, spl_steady_metrics
+
+    U, K = 1.0e-3, 1.0e-5
+    m, h0, q, interior = spl_steady_case(60, U, K)
+    g = Model(m, h0, tend=1e12, **q)
+    g.initialise()
+    for _ in range(400):
+        g.step()
+    prev = g.hLocal.cpu().numpy().copy()
+    for _ in range(50):
+        g.step()
+    h = g.hLocal.cpu().numpy()
+    rcv = g.rcvIDi.cpu().numpy()[0]
+    dist = g.distRcvi.cpu().numpy()[0]
+    fa = g.FAL.cpu().numpy()
+    eu, ksr, iqr, within, nch = spl_steady_metrics(h, rcv, dist, fa, interior.astype(bool), U, K, prev,
+                                                   50 * q["dt"], q["dt"])
+    assert nch > 100
+    assert abs(eu - 1.0) < 0.05, eu
+    assert abs(ksr - 1.0) < 0.005, ksr
+    assert iqr < 0.10, iqr

@@ -21,6 +21,16 @@ The linear systems are solved exactly (topological sweeps for the receiver
 DAGs, Jacobi to a fixed point for the diffusion operator) where goSPL stops
 its Krylov solvers at rtol 1e-8; parity of the HIP path is defined against
 these exact solutions.
+
+Parity unpinned: the parts of this glue that goSPL implements with numpy,
+pandas, numpy_indexed or PETSc (absent here, so no golden vector can be
+written) — the spill-point choice and flat receivers of _pitInformation /
+_dirFlats, the per-pit sums of _getPitParams and of the cascade inflow, the
+cascade bookkeeping of _distributeDownstream, and the Krylov solutions
+themselves.  They are restated from source text; the flow and erosion glue is
+checked against goSPL's analytic stream-power steady state
+(benchmarks/test_spl.py:443-577), the rest by invariants only
+(tests/test_oracle_pins.py).
 """
 
 from __future__ import annotations

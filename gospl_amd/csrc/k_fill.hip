@@ -134,6 +134,61 @@ __global__ void k_fill_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
 }
 
+// Cells that touch another basin: the only ones the relaxation reads from.
+struct IsBasinBoundary {
+  int64_t N;
+  int W;
+  const int32_t* nbr;
+  const int32_t* deg;
+  const int32_t* state;
+  const int32_t* root;
+  __device__ __forceinline__ bool operator()(const int32_t& u) const {
+    if (state[u] < 0) return false;
+    const int ru = root[u];
+    const int dg = deg[u];
+    for (int p = 0; p < W && p < dg; ++p) {
+      const int v = nbr[(int64_t)p * N + u];
+      if (v >= 0 && state[v] >= 0 && root[v] != ru) return true;
+    }
+    return false;
+  }
+};
+
+// k_fill_relax over the list of basin-boundary cells.
+__global__ void k_fill_relax_list(int64_t N, int W, const int32_t* __restrict__ nbr,
+                                  const int32_t* __restrict__ deg, const double* __restrict__ dem,
+                                  const int32_t* __restrict__ state, const int32_t* __restrict__ root,
+                                  unsigned long long* __restrict__ S, const int32_t* __restrict__ list,
+                                  const int32_t* __restrict__ count, int32_t* __restrict__ flag,
+                                  int pass) {
+  const int n = *count;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+  const int u = list[t];
+  const int ru = root[u];
+  const double zu = dem[u];
+  const unsigned long long cur = S[ru];
+  if (cur == KEY_NEG_INF) continue;
+  unsigned long long best = cur;
+  const int dg = deg[u];
+  for (int p = 0; p < W && p < dg; ++p) {
+    int v = nbr[(int64_t)p * N + u];
+    if (v < 0 || state[v] < 0) continue;
+    int rv = root[v];
+    if (rv == ru) continue;
+    unsigned long long sv = S[rv];
+    if (sv == KEY_POS_INF) continue;
+    double w = dem[v] > zu ? dem[v] : zu;
+    unsigned long long kw = dkey(w);
+    unsigned long long c = kw > sv ? kw : sv;
+    if (c < best) best = c;
+  }
+  if (best < cur) {
+    unsigned long long old = atomicMin(&S[ru], best);
+    if (best < old) *flag = pass;
+  }
+  }
+}
+
 __global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
                              const int32_t* __restrict__ state, const int32_t* __restrict__ root,
                              const unsigned long long* __restrict__ S, double level, double sl,
@@ -454,8 +509,20 @@ int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, doubl
         k_jump<<<g, GOSPL_BLOCK, 0, s>>>(N, root, f, pass);
       }, &it))
     return 1;
+  // relax the basin levels over the cells that touch another basin
+  int32_t* blist = m->wi[5];
+  int32_t* bcount = m->dflag + 44;
+  {
+    size_t bytes = m->cub_tmp_bytes;
+    hipcub::CountingInputIterator<int32_t> cit(0);
+    GOSPL_CHECK(hipcub::DeviceSelect::If(m->cub_tmp, bytes, cit, blist, bcount, (int)N,
+                                         IsBasinBoundary{N, m->W, m->nbr, m->deg, state, root}, s));
+  }
+  // grid-stride over the list, whose length stays on the device
+  const int gb = (int)std::min<int64_t>(g, 2048);
   if (until_stable(m, s, (int)N + 16, 2, &m->iter_hint[1], [&](int32_t* f, int pass) {
-        k_fill_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, f, pass);
+        k_fill_relax_list<<<gb, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, blist,
+                                                      bcount, f, pass);
       }, &it))
     return 1;
   k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, sealevel, lfill);

@@ -172,6 +172,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_pit_reduce_inflow(
 int pit_tmp_bytes(int64_t N, size_t* bytes) {
   size_t b1 = 0, b2 = 0;
   hipcub::CountingInputIterator<int32_t> it(0);
+  // (the select's workspace does not depend on its predicate)
   GOSPL_CHECK(hipcub::DeviceSelect::If(nullptr, b1, it, (int32_t*)nullptr, (int32_t*)nullptr, (int)N,
                                        IsPitCell{nullptr}));
   GOSPL_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, (unsigned long long*)nullptr,

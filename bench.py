@@ -39,7 +39,9 @@ ALG_BYTES = {
 }
 METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel err"
 STEP_BYTES = 564  # sum of ALG_BYTES: compulsory bytes per vertex and step
-SWEEP_KERNELS = "k_dag_level0<%d> + k_dag_level<%d> + k_dag_tail<%d>"
+# the sweep kernels between the library's timing events, per entry point
+SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_flow_chain",
+                 "gospl_spl": "k_dag_level0<1> + k_dag_level<1> + k_dag_tail<1>"}
 
 
 def log(*a):
@@ -148,12 +150,12 @@ def main():
     N = m.N
     phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
     dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms_total"])
-    # the dominant entry point's dominant kernel: the DAG sweep, i.e. its three
-    # back-to-back kernels (level 0 on the full grid, the persistent level
-    # kernel, the one-workgroup tail), timed by HIP events on their launch
-    # stream inside the library
-    kern = {"gospl_flowacc": (SWEEP_KERNELS % (0, 0, 0), fa_ms.value, fa_calls.value),
-            "gospl_spl": (SWEEP_KERNELS % (1, 1, 1), spl_ms.value, spl_calls.value)}.get(dom)
+    # the dominant entry point's dominant kernel: the DAG sweep, i.e. its
+    # back-to-back kernels (level 0 on the full grid, then the dataflow kernel
+    # or the persistent level kernel and the one-workgroup tail), timed by HIP
+    # events on their launch stream inside the library
+    kern = {"gospl_flowacc": (SWEEP_KERNELS["gospl_flowacc"], fa_ms.value, fa_calls.value),
+            "gospl_spl": (SWEEP_KERNELS["gospl_spl"], spl_ms.value, spl_calls.value)}.get(dom)
     if kern and kern[2] > 0:
         kname, kms, kcalls = kern[0], kern[1] / kern[2], kern[2]
     else:
@@ -200,9 +202,9 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": None, "alg_bytes_per_vertex": ALG_BYTES[dom],
                      "ms_per_launch": kms, "launches": kcalls,
-                     "note": "latency-bound: ~600 dependent levels per solve, each a few memory "
-                             "round trips plus a grid barrier; traffic null: rocprofv3 --pmc crashes "
-                             "in the profiler on this program"},
+                     "note": "latency-bound: ~600 dependent levels per solve (the longest flow "
+                             "path); traffic null: rocprofv3 --pmc crashes in the profiler on this "
+                             "program"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

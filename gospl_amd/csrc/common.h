@@ -29,6 +29,9 @@ struct gospl_mesh {
   int dag_tail;               // frontier size below which one workgroup finishes the sweep
   int dag_coop;               // launch the persistent DAG kernel with the cooperative API
   int dag_csr;                // use the CSR rows even when packed records apply
+  int dag_chain;              // flow accumulation as a dataflow sweep (records only)
+  int chain_blocks;           // its grid
+  int chain_pool;             // its pool capacity per wavefront
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64

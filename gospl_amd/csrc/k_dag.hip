@@ -398,15 +398,21 @@ __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__
   }
 }
 
+// An empty row entry: a signalling NaN, which no arithmetic produces, so a
+// reader can tell an entry its donor has not written yet.
+#define REC_EMPTY_BITS 0x7ff4000000000001ll
+__device__ __forceinline__ double rec_empty() { return __longlong_as_double(REC_EMPTY_BITS); }
+__device__ __forceinline__ bool is_empty(double d) { return __double_as_longlong(d) == REC_EMPTY_BITS; }
+
 // Whole flow-accumulation record of vertex i, written at once.  The slot of
-// (i, receiver k) is the rank of its bit in the receiver's row mask.
-// pend[i] = row length.
+// (i, receiver k) is the rank of its bit in the receiver's row mask.  Row
+// entries start empty.  pend[i] = row length.
 __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                              const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
                              const int32_t* __restrict__ rcv, const double* __restrict__ coef,
                              const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
-                             int32_t* __restrict__ pend) {
+                             double* __restrict__ oacc, int32_t* __restrict__ pend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int r[3], sl[3];
@@ -428,15 +434,17 @@ __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict
   }
   const int nd = __popcll(rowmask[i]);
   pend[i] = nd;
+  const double e = rec_empty();
   double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
   out[0] = make_double2(pack32(nd, 0), rhs[i]);
   out[1] = make_double2(w[0], w[1]);
   out[2] = make_double2(w[2], pack32(r[0], r[1]));
   out[3] = make_double2(pack32(r[2], sl[0]), pack32(sl[1], sl[2]));
-  out[4] = make_double2(0.0, 0.0);
-  out[5] = make_double2(0.0, 0.0);
-  out[6] = make_double2(0.0, 0.0);
-  out[7] = make_double2(0.0, 0.0);
+  out[4] = make_double2(e, e);
+  out[5] = make_double2(e, e);
+  out[6] = make_double2(e, e);
+  out[7] = make_double2(e, e);
+  for (int c = REC_NACC; c < nd; ++c) oacc[(int64_t)(c - REC_NACC) * N + i] = e;
 }
 
 // Whole stream-power record of vertex i, written at once: its dependency
@@ -744,6 +752,176 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Flow accumulation as a dataflow sweep (records only).  After level 0, the
+// thread whose decrement releases a receiver goes on with it at once, so a
+// flow path costs two dependent memory round trips per vertex (its record,
+// then its pushes and counter decrements) and no grid barrier.  Each
+// wavefront owns a contiguous share of the start list (the level-0 releases)
+// and an LDS pool: a vertex that releases several receivers keeps one and
+// pools the others, and idle lanes take work from the pool first, then from
+// the share.  A wavefront leaves when its lanes, pool and share are empty; it
+// never waits for another thread except for the row entries of a released
+// vertex: pushes are coherent stores into entries that start empty, and a
+// reader that still sees an empty entry (its donor's store has not landed)
+// reads it again, a bounded number of times (then it raises the abort flag
+// and the host reports the failure).  Releases that do not fit the pool go
+// to the overflow list, which the host runs as the next start list.
+// ctr[0] overflow count, ctr[1] abort flag.
+// ---------------------------------------------------------------------------
+#define CHAIN_THREADS 256
+#define CHAIN_POOL 512
+#define CHAIN_SPIN_MAX (1 << 21)
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS written by some lanes of a wavefront and read by others: order the
+// accesses (one wavefront's LDS operations complete in issue order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(CHAIN_THREADS) void k_flow_chain(DagArgs a, const int32_t* __restrict__ list,
+                                                            const int32_t* __restrict__ nlist,
+                                                            int32_t* __restrict__ olist,
+                                                            int32_t* __restrict__ ctr, int pool_cap) {
+  __shared__ int32_t s_pool[CHAIN_THREADS / 64][CHAIN_POOL];
+  const int64_t N = a.N;
+  const int lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int32_t* pool = s_pool[threadIdx.x >> 6];
+  const int64_t nw = (int64_t)gridDim.x * (CHAIN_THREADS / 64);
+  const int64_t w = (int64_t)blockIdx.x * (CHAIN_THREADS / 64) + (threadIdx.x >> 6);
+  const int64_t n0 = *nlist;
+  int64_t cur = w * n0 / nw;               // this wavefront's share of the start list
+  const int64_t end = (w + 1) * n0 / nw;
+  int np = 0;                              // pooled vertices (wave-uniform)
+  int v = -1;                              // vertex this lane works on
+  int retry = 0;                           // passes this lane found its row incomplete
+  for (;;) {
+    // idle lanes take pooled vertices, then the next ones of the share
+    const bool need = v < 0;
+    const unsigned long long nm = __ballot(need);
+    if (nm != 0ull) {
+      const int rank = __popcll(nm & below);
+      const int nn = __popcll(nm);
+      const int fp = nn < np ? nn : np;
+      if (need && rank < fp) v = pool[np - 1 - rank];
+      const int64_t left = end - cur;
+      const int fs = (int)((int64_t)(nn - fp) < left ? (int64_t)(nn - fp) : left);
+      if (need && rank >= fp && rank - fp < fs) v = list[cur + (rank - fp)];
+      np -= fp;
+      cur += fs;
+      wave_sync();
+    }
+    if (__ballot(v >= 0) == 0ull) break;  // lanes, pool and share are empty
+    // solve the lanes' vertices whose rows are complete
+    int r[3] = {0, 0, 0};
+    int old[3] = {0, 0, 0};
+    bool dep[3] = {false, false, false};
+    bool solved = false;
+    if (v >= 0) {
+      const double* rv = a.rec + (int64_t)v * REC_W;
+      const double2* rp = reinterpret_cast<const double2*>(rv);
+      const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
+      const int nd = lo32(q0.x);
+      double e[REC_NACC];
+#pragma unroll
+      for (int c = 0; c < REC_NACC; ++c) e[c] = ld_agent(rv + REC_ACC + c);  // with the rest of the line
+      bool ready = true;
+      double s = q0.y;
+#pragma unroll
+      for (int c = 0; c < REC_NACC; ++c)
+        if (c < nd) {
+          ready = ready && !is_empty(e[c]);
+          s = s + e[c];
+        }
+      for (int c = REC_NACC; c < nd; ++c) {
+        const double o = ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
+        ready = ready && !is_empty(o);
+        s = s + o;
+      }
+      if (ready) {
+        solved = true;
+        a.x[v] = s;
+        const double wk[3] = {q1.x, q1.y, q2.x};
+        r[0] = lo32(q2.y);
+        r[1] = hi32(q2.y);
+        r[2] = lo32(q3.x);
+        const int sl[3] = {hi32(q3.x), lo32(q3.y), hi32(q3.y)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (r[k] != v && r[k] >= 0 && wk[k] != 0.0) {
+            dep[k] = true;
+            const double val = wk[k] * s;
+            if (sl[k] >= 0) st_agent(a.rec + sl[k], val);
+            else if (sl[k] <= -2) st_agent(a.oacc + (-2 - (int64_t)sl[k]), val);
+            old[k] = atomicSub(&a.pend[r[k]], 1);
+          }
+        }
+      }
+    }
+    // go on with the first released receiver, pool the others
+    int next = -1;
+    bool ex[3] = {false, false, false};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (dep[k] && old[k] == 1) {
+        if (next < 0) next = r[k];
+        else ex[k] = true;
+      }
+    unsigned long long em[3];
+    int ne = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      em[k] = __ballot(ex[k]);
+      ne += __popcll(em[k]);
+    }
+    if (ne > 0) {
+      const int fit = ne < pool_cap - np ? ne : pool_cap - np;
+      int obase = 0;
+      if (fit < ne) {
+        if (lane == 0) obase = atomicAdd(&ctr[0], ne - fit);
+        obase = __shfl(obase, 0);
+      }
+      int base = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (ex[k]) {
+          const int idx = base + __popcll(em[k] & below);
+          const int64_t oi = (int64_t)obase + idx - fit;
+          if (idx < fit) pool[np + idx] = r[k];
+          else if (oi < N) olist[oi] = r[k];
+          else __hip_atomic_store(&ctr[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        base += __popcll(em[k]);
+      }
+      np += fit;
+      wave_sync();
+    }
+    const bool waiting = v >= 0 && !solved;
+    if (solved) {
+      v = next;
+      retry = 0;
+    } else if (waiting && ++retry > CHAIN_SPIN_MAX) {
+      __hip_atomic_store(&ctr[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the abort flag is read only by wavefronts that wait on a row
+    if (__ballot(waiting) != 0ull) {
+      int ab = 0;
+      if (lane == 0) ab = __hip_atomic_load(&ctr[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__shfl(ab, 0) != 0) break;
+    }
+  }
+}
+
 // Vertices never reached (a cycle in the receivers and everything below it):
 // x = 0, listed.
 __global__ void k_dag_unsolved(int64_t N, const int32_t* __restrict__ pend, double* __restrict__ x,
@@ -963,7 +1141,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask);
     if (mode == 0)
       k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rhs,
-                                              rowmask, m->wrec, pend);
+                                              rowmask, m->wrec, m->wacc, pend);
     else
       k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rcv, coef, diag, rhs, rowmask, m->wrec,
                                               m->wdon, pend);
@@ -993,8 +1171,19 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
+  const bool chain = rec && mode == 0 && m->dag_chain;
+  // dataflow sweep: ctr[0] overflow count, ctr[1] abort flag (two sets, by pass parity)
+  int32_t* ctr = dev + 56;
+  int32_t* ovf = m->wi[3];  // overflow list (start list of the next pass)
+  if (chain) GOSPL_CHECK(hipMemsetAsync(ctr, 0, sizeof(int32_t) * 4, s));
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
-  if (rec) {
+  if (chain) {
+    k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+    GOSPL_CHECK(hipGetLastError());
+    // the level-0 releases are the first start list
+    k_flow_chain<<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, ovf, ctr, m->chain_pool);
+    GOSPL_CHECK(hipGetLastError());
+  } else if (rec) {
     if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
   } else {
@@ -1002,11 +1191,12 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     else k_dag_level0<1, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
   }
   GOSPL_CHECK(hipGetLastError());
-  if (m->dag_coop)
+  if (chain) {
+  } else if (m->dag_coop)
     GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
   else  // plain launch (counter collection); the grid fits the GPU many times over
     GOSPL_CHECK(hipLaunchKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
-  if (m->dag_tail > 0) {
+  if (m->dag_tail > 0 && !chain) {
     if (rec) {
       if (mode == 0) k_dag_tail<0, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
       else k_dag_tail<1, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
@@ -1021,10 +1211,35 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
-  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, bar + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));  // abort word
+  // abort word; dataflow sweep: overflow count and abort flag
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, chain ? ctr + 1 : (int32_t*)(bar + 2), sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s));
+  if (chain) GOSPL_CHECK(hipMemcpyAsync(m->hflag + 35, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
+  if (chain && m->hflag[35] > 0 && !m->hflag[34]) {
+    // releases that overflowed a pool: run them as the next start list until
+    // none is left, then list the unsolved vertices again
+    for (int pass = 1; m->hflag[35] > 0 && !m->hflag[34]; ++pass) {
+      int32_t* in = (pass & 1) ? ovf : fb;
+      int32_t* out = (pass & 1) ? fb : ovf;
+      int32_t* cin = ctr + 2 * ((pass + 1) & 1);
+      int32_t* cout = ctr + 2 * (pass & 1);
+      GOSPL_CHECK(hipMemsetAsync(cout, 0, sizeof(int32_t) * 2, s));
+      k_flow_chain<<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, in, cin, out, cout, m->chain_pool);
+      GOSPL_CHECK(hipGetLastError());
+      GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, cout + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      GOSPL_CHECK(hipMemcpyAsync(m->hflag + 35, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      GOSPL_CHECK(hipStreamSynchronize(s));
+    }
+    GOSPL_CHECK(hipMemsetAsync(dev + 1, 0, sizeof(int32_t), s));
+    k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
+    GOSPL_CHECK(hipGetLastError());
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+  }
   if (m->hflag[34]) {
-    set_error("receiver-DAG sweep aborted: a grid barrier did not complete");
+    set_error(chain ? "flow-accumulation sweep aborted: a bounded wait ran out"
+                    : "receiver-DAG sweep aborted: a grid barrier did not complete");
     return 1;
   }
   if (mode == 0 && m->hflag[1] > 0) {
@@ -1100,6 +1315,20 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   }
   if (per_cu < 1) per_cu = 1;
   const int maxb = per_cu * m->num_cu;
+  // dataflow flow accumulation (GOSPL_DAG_CHAIN=0: level by level); its grid
+  // is what the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides)
+  const char* te_chain = getenv("GOSPL_DAG_CHAIN");
+  m->dag_chain = !(te_chain && atoi(te_chain) == 0);
+  int cb = 0;
+  GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&cb, (const void*)k_flow_chain, CHAIN_THREADS, 0));
+  if (cb < 1) cb = 1;
+  m->chain_blocks = cb * m->num_cu;
+  const char* te_cb = getenv("GOSPL_CHAIN_BLOCKS");
+  if (te_cb && atoi(te_cb) > 0) m->chain_blocks = atoi(te_cb);
+  // GOSPL_CHAIN_POOL: pool capacity per wavefront (1..512; small values force overflow passes)
+  m->chain_pool = CHAIN_POOL;
+  const char* te_pool = getenv("GOSPL_CHAIN_POOL");
+  if (te_pool && atoi(te_pool) >= 1 && atoi(te_pool) <= CHAIN_POOL) m->chain_pool = atoi(te_pool);
   // Few blocks: the grid barrier costs ~3 us at 32 blocks and ~18 us at 256 on
   // MI355X, and the frontiers past level 0 hold a few thousand vertices.
   int want = 32;

@@ -113,7 +113,10 @@ int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double s
  * Flow accumulation solve: replaces FAMesh._solve_KSP on the flow matrix
  * (I - W)^T built by matrixFlow (gospl/flow/flowplex.py:323-478, 843) with an
  * exact sweep over the receiver DAG.  rcv/wgt (nrcv, N), b (N) -> q (N).
- * *nlevels (HOST, may be NULL) receives the DAG depth.  Receivers built with
+ * *nlevels (HOST, may be NULL) receives the DAG depth when the sweep runs
+ * level by level, and 0 for the dataflow sweep (the default for up to three
+ * receivers; GOSPL_DAG_CHAIN=0 at mesh creation selects the level sweep).
+ * Both give the same bits.  Receivers built with
  * flat routing on a partly filled surface can hold a cycle; the vertices the
  * sweep cannot reach (the cycle and everything below it) are then solved by
  * the fixed-point iteration q <- b + W^T q restricted to them, to a relative

@@ -169,6 +169,34 @@ def test_csr_rows_match_packed_records():
         assert np.array_equal(a, b)
 
 
+def test_dataflow_sweep_matches_level_sweep():
+    """Flow accumulation as a dataflow sweep (the default), with its normal
+    pool and with a one-entry pool (releases overflow into extra passes),
+    gives the bits of the level-by-level sweep."""
+    import os
+
+    from gospl_amd.model import Model
+
+    m, h0, p = case("sphere", 6)
+    out = []
+    for chain, pool in (("0", "512"), ("1", "512"), ("1", "1")):
+        os.environ["GOSPL_DAG_CHAIN"] = chain
+        os.environ["GOSPL_CHAIN_POOL"] = pool
+        try:
+            g = Model(m, h0, tend=1e12, **p)
+        finally:
+            del os.environ["GOSPL_DAG_CHAIN"]
+            del os.environ["GOSPL_CHAIN_POOL"]
+        g.initialise()
+        for _ in range(3):
+            g.step()
+        out.append([t.cpu().numpy().copy() for t in (g.hLocal, g.FAL, g.cumED, g.fillFAL)])
+        g.dm.close()
+    for res in out[1:]:
+        for a, b in zip(out[0], res):
+            assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("flowdir", [1, 6])
 def test_other_receiver_counts(flowdir):
     """Single- and six-receiver routing (the latter on the general row

@@ -40,8 +40,8 @@ ALG_BYTES = {
 METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel err"
 STEP_BYTES = 564  # sum of ALG_BYTES: compulsory bytes per vertex and step
 # the sweep kernels between the library's timing events, per entry point
-SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_flow_chain",
-                 "gospl_spl": "k_dag_level0<1> + k_dag_level<1> + k_dag_tail<1>"}
+SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_chain<0>",
+                 "gospl_spl": "k_dag_level0<1> + k_chain<1>"}
 
 
 def log(*a):
@@ -151,9 +151,8 @@ def main():
     phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
     dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms_total"])
     # the dominant entry point's dominant kernel: the DAG sweep, i.e. its
-    # back-to-back kernels (level 0 on the full grid, then the dataflow kernel
-    # or the persistent level kernel and the one-workgroup tail), timed by HIP
-    # events on their launch stream inside the library
+    # back-to-back kernels (level 0 on the full grid, then the dataflow
+    # kernel), timed by HIP events on their launch stream inside the library
     kern = {"gospl_flowacc": (SWEEP_KERNELS["gospl_flowacc"], fa_ms.value, fa_calls.value),
             "gospl_spl": (SWEEP_KERNELS["gospl_spl"], spl_ms.value, spl_calls.value)}.get(dom)
     if kern and kern[2] > 0:

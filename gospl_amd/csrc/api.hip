@@ -168,6 +168,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   for (int k = 0; k < 4; ++k) dfree(m->wd[k]);
   dfree(m->counts);
   dfree(m->dflag);
+  dfree(m->chain_ctl);
   if (m->hflag) (void)hipHostFree(m->hflag);
   dfree(m->cub_tmp);
   dfree(m->wcoef);

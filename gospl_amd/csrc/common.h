@@ -32,6 +32,8 @@ struct gospl_mesh {
   int dag_chain;              // flow accumulation as a dataflow sweep (records only)
   int chain_blocks;           // its grid
   int chain_pool;             // its pool capacity per wavefront
+  int chain_share;            // pool size from which batches go to the global queue
+  int32_t* chain_ctl;         // its control words
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64

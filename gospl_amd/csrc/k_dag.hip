@@ -448,8 +448,9 @@ __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict
 }
 
 // Whole stream-power record of vertex i, written at once: its dependency
-// mask, coefficients and its row (the donors' receiver-value slots, in row
-// order, from its row mask; entries past the record go to `odon`).
+// mask, coefficients, its receivers' values (empty until pushed) and its row
+// (the donors' receiver-value slots, in row order, from its row mask;
+// entries past the record go to `odon`).
 // pend[i] = number of receivers i depends on.
 __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
                              const int32_t* __restrict__ rcv, const double* __restrict__ coef,
@@ -484,12 +485,13 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
     }
   }
   pend[i] = __popc(mask);
+  const double e = rec_empty();
   double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
   out[0] = make_double2(pack32(nd, mask), rhs[i]);
   out[1] = make_double2(diag[i], o[0]);
   out[2] = make_double2(o[1], o[2]);
-  out[3] = make_double2(0.0, 0.0);
-  out[4] = make_double2(0.0, pack32(t[0], t[1]));
+  out[3] = make_double2(e, e);
+  out[4] = make_double2(e, pack32(t[0], t[1]));
   out[5] = make_double2(pack32(t[2], t[3]), pack32(t[4], t[5]));
   out[6] = make_double2(pack32(t[6], t[7]), pack32(t[8], t[9]));
   out[7] = make_double2(pack32(t[10], t[11]), pack32(t[12], t[13]));
@@ -753,30 +755,48 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
 }
 
 // ---------------------------------------------------------------------------
-// Flow accumulation as a dataflow sweep (records only).  After level 0, the
-// thread whose decrement releases a receiver goes on with it at once, so a
-// flow path costs two dependent memory round trips per vertex (its record,
-// then its pushes and counter decrements) and no grid barrier.  Each
-// wavefront owns a contiguous share of the start list (the level-0 releases)
-// and an LDS pool: a vertex that releases several receivers keeps one and
-// pools the others, and idle lanes take work from the pool first, then from
-// the share.  A wavefront leaves when its lanes, pool and share are empty; it
-// never waits for another thread except for the row entries of a released
-// vertex: pushes are coherent stores into entries that start empty, and a
-// reader that still sees an empty entry (its donor's store has not landed)
-// reads it again, a bounded number of times (then it raises the abort flag
-// and the host reports the failure).  Releases that do not fit the pool go
-// to the overflow list, which the host runs as the next start list.
-// ctr[0] overflow count, ctr[1] abort flag.
+// Dataflow sweeps (records only).  After level 0, the thread whose decrement
+// releases a vertex goes on with it at once, so a dependency chain costs two
+// dependent memory round trips per vertex (its record, then its pushes and
+// counter decrements) and no grid barrier.  Each wavefront owns a contiguous
+// share of the start list (the level-0 releases) and an LDS pool: a vertex
+// that releases several vertices keeps one and pools the others, and idle
+// lanes take work from the pool first, then from the share.  Flow
+// accumulation mostly merges paths; the stream-power sweep runs upstream and
+// fans out, so a wavefront whose pool fills hands batches of 64 vertices to a
+// global queue, from which idle wavefronts claim them.
+//
+// Nothing waits on another thread except for data already on its way: pushes
+// are coherent stores into row entries that start empty (a reader that still
+// sees an empty entry, its donor's store not landed yet, reads it again), and
+// a queue batch is claimed only once reserved, its entries starting at -1.
+// These waits are bounded: past CHAIN_SPIN_MAX re-reads the abort flag is
+// raised, every wavefront leaves and the host reports the failure.  An idle
+// wavefront leaves when the queue is empty and no wavefront holds work (or
+// after CHAIN_SPIN_MAX idle passes); work is only ever held by a running
+// wavefront, so leaving early costs parallelism, never vertices.
 // ---------------------------------------------------------------------------
 #define CHAIN_THREADS 256
 #define CHAIN_POOL 512
 #define CHAIN_SPIN_MAX (1 << 21)
+#define CHAIN_POLLERS 256  // idle wavefronts that stay to poll the queue
+// control words, one 128-byte line each
+#define CQ_QUEUE 0    // 64-bit: batches reserved << 32 | batches claimed
+#define CQ_ACTIVE 32  // wavefronts holding work
+#define CQ_IDLERS 64  // idle wavefronts polling the queue
+#define CQ_ABORT 96
+#define CQ_WORDS 128
 
 __device__ __forceinline__ double ld_agent(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent_i(const int32_t* p) {
+  return __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent_i(int32_t* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -788,135 +808,271 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(CHAIN_THREADS) void k_flow_chain(DagArgs a, const int32_t* __restrict__ list,
-                                                            const int32_t* __restrict__ nlist,
-                                                            int32_t* __restrict__ olist,
-                                                            int32_t* __restrict__ ctr, int pool_cap) {
+// One wavefront's work state.  Wave-uniform except v, ci, next.
+struct ChainWave {
+  int32_t* pool;
+  int np;           // pooled vertices
+  int pool_cap;     // pool capacity (<= CHAIN_POOL, >= 128)
+  int share_at;     // pool size from which a batch goes to the global queue
+  int32_t* gq;      // global queue, 64 entries per batch
+  int64_t gq_cap;   // in batches
+  int32_t* ctl;
+  int lane;
+  unsigned long long below;
+  int next;         // the vertex this lane goes on with (-1: none)
+
+  // Move the newest 64 pooled vertices to a global batch.
+  __device__ __forceinline__ void spill() {
+    int b = 0;
+    if (lane == 0)
+      b = (int)(atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[CQ_QUEUE]), 1ull << 32) >> 32);
+    b = __shfl(b, 0);
+    const int u = pool[np - 64 + lane];
+    if ((int64_t)b < gq_cap) st_agent_i(&gq[(int64_t)b * 64 + lane], u);
+    else if (lane == 0) st_agent_i(&ctl[CQ_ABORT], 1);
+    np -= 64;
+    wave_sync();
+  }
+  // A released vertex `val` of lanes with `rl`: kept as the lane's next
+  // vertex if it has none, pooled otherwise.  Reached by all lanes.
+  __device__ __forceinline__ void offer(bool rl, int val) {
+    const bool keep = rl && next < 0;
+    if (keep) next = val;
+    const bool ex = rl && !keep;
+    const unsigned long long em = __ballot(ex);
+    if (em == 0ull) return;
+    const int e = __popcll(em);
+    while (np + e > pool_cap) spill();  // pool_cap >= 128 and e <= 64: np >= 64 here
+    if (ex) pool[np + __popcll(em & below)] = val;
+    np += e;
+    wave_sync();
+  }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ nlist,
+                                                       int32_t* __restrict__ gq, int64_t gq_cap,
+                                                       int32_t* __restrict__ ctl, int pool_cap,
+                                                       int share_at) {
   __shared__ int32_t s_pool[CHAIN_THREADS / 64][CHAIN_POOL];
   const int64_t N = a.N;
   const int lane = __lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
-  int32_t* pool = s_pool[threadIdx.x >> 6];
+  ChainWave W{s_pool[threadIdx.x >> 6], 0, pool_cap, share_at, gq, gq_cap, ctl, lane,
+              (1ull << lane) - 1ull, -1};
   const int64_t nw = (int64_t)gridDim.x * (CHAIN_THREADS / 64);
   const int64_t w = (int64_t)blockIdx.x * (CHAIN_THREADS / 64) + (threadIdx.x >> 6);
   const int64_t n0 = *nlist;
-  int64_t cur = w * n0 / nw;               // this wavefront's share of the start list
+  int64_t cur = w * n0 / nw;  // this wavefront's share of the start list
   const int64_t end = (w + 1) * n0 / nw;
-  int np = 0;                              // pooled vertices (wave-uniform)
-  int v = -1;                              // vertex this lane works on
-  int retry = 0;                           // passes this lane found its row incomplete
+  int v = -1;                 // vertex this lane works on
+  int64_t ci = -1;            // global-queue entry this lane waits for
+  int retry = 0;              // passes this lane waited on a row or a queue entry
+  int idle = 0;               // consecutive idle passes of the wavefront
+  bool act = true;            // counted in ctl[CQ_ACTIVE]
+  bool poller = false;        // counted in ctl[CQ_IDLERS]
+  unsigned long long* qword = reinterpret_cast<unsigned long long*>(&ctl[CQ_QUEUE]);
+  if (lane == 0) atomicAdd(&ctl[CQ_ACTIVE], 1);
   for (;;) {
+    // lanes waiting on a claimed queue entry
+    if (ci >= 0) {
+      const int q = ld_agent_i(&gq[ci]);
+      if (q >= 0) {
+        v = q;
+        ci = -1;
+        retry = 0;
+      }
+    }
     // idle lanes take pooled vertices, then the next ones of the share
-    const bool need = v < 0;
+    const bool need = v < 0 && ci < 0;
     const unsigned long long nm = __ballot(need);
     if (nm != 0ull) {
-      const int rank = __popcll(nm & below);
+      const int rank = __popcll(nm & W.below);
       const int nn = __popcll(nm);
-      const int fp = nn < np ? nn : np;
-      if (need && rank < fp) v = pool[np - 1 - rank];
+      const int fp = nn < W.np ? nn : W.np;
+      if (need && rank < fp) v = W.pool[W.np - 1 - rank];
       const int64_t left = end - cur;
       const int fs = (int)((int64_t)(nn - fp) < left ? (int64_t)(nn - fp) : left);
       if (need && rank >= fp && rank - fp < fs) v = list[cur + (rank - fp)];
-      np -= fp;
+      W.np -= fp;
       cur += fs;
       wave_sync();
     }
-    if (__ballot(v >= 0) == 0ull) break;  // lanes, pool and share are empty
-    // solve the lanes' vertices whose rows are complete
-    int r[3] = {0, 0, 0};
-    int old[3] = {0, 0, 0};
-    bool dep[3] = {false, false, false};
-    bool solved = false;
-    if (v >= 0) {
-      const double* rv = a.rec + (int64_t)v * REC_W;
-      const double2* rp = reinterpret_cast<const double2*>(rv);
-      const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
-      const int nd = lo32(q0.x);
-      double e[REC_NACC];
-#pragma unroll
-      for (int c = 0; c < REC_NACC; ++c) e[c] = ld_agent(rv + REC_ACC + c);  // with the rest of the line
-      bool ready = true;
-      double s = q0.y;
-#pragma unroll
-      for (int c = 0; c < REC_NACC; ++c)
-        if (c < nd) {
-          ready = ready && !is_empty(e[c]);
-          s = s + e[c];
+    if (__ballot(v >= 0 || ci >= 0) == 0ull) {
+      // pool and share are empty: claim a global batch, or leave.  A
+      // wavefront leaves only after it has seen the queue empty, which
+      // includes its own batches (same word), so no batch is left behind.
+      // Up to CHAIN_POLLERS idle wavefronts stay while others hold work.
+      int claimed = -1, stop = 0;
+      if (lane == 0) {
+        if (act) {
+          atomicSub(&ctl[CQ_ACTIVE], 1);
+          act = false;
         }
-      for (int c = REC_NACC; c < nd; ++c) {
-        const double o = ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
-        ready = ready && !is_empty(o);
-        s = s + o;
+        const unsigned long long qw = __hip_atomic_load(qword, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int h = (int)(unsigned int)qw, t = (int)(qw >> 32);
+        if (h < t) {
+          if (atomicCAS(qword, qw, qw + 1ull) == qw) {
+            atomicAdd(&ctl[CQ_ACTIVE], 1);
+            act = true;
+            claimed = h;
+            if (poller) atomicSub(&ctl[CQ_IDLERS], 1);
+            poller = false;
+          }
+        } else if (ld_agent_i(&ctl[CQ_ABORT]) != 0) {
+          stop = 1;
+        } else if (!poller) {
+          if (atomicAdd(&ctl[CQ_IDLERS], 1) < CHAIN_POLLERS) poller = true;
+          else stop = 1;
+        } else if (ld_agent_i(&ctl[CQ_ACTIVE]) == 0) {
+          stop = 1;
+        }
       }
-      if (ready) {
-        solved = true;
-        a.x[v] = s;
-        const double wk[3] = {q1.x, q1.y, q2.x};
-        r[0] = lo32(q2.y);
-        r[1] = hi32(q2.y);
-        r[2] = lo32(q3.x);
-        const int sl[3] = {hi32(q3.x), lo32(q3.y), hi32(q3.y)};
+      claimed = __shfl(claimed, 0);
+      stop = __shfl(stop, 0);
+      act = __shfl((int)act, 0) != 0;
+      poller = __shfl((int)poller, 0) != 0;
+      if (claimed >= 0) {
+        if ((int64_t)claimed >= gq_cap) {
+          if (lane == 0) st_agent_i(&ctl[CQ_ABORT], 1);
+          break;
+        }
+        ci = (int64_t)claimed * 64 + lane;
+        idle = 0;
+        continue;
+      }
+      if (stop) break;
+      if (++idle > CHAIN_SPIN_MAX) {
+        if (lane == 0) st_agent_i(&ctl[CQ_ABORT], 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(16);
+      continue;
+    }
+    idle = 0;
+    // solve the lanes' vertices whose rows are complete; push, decrement
+    W.next = -1;
+    bool solved = false;
+    if (MODE == 0) {
+      int r[3] = {0, 0, 0};
+      int old[3] = {0, 0, 0};
+      bool dep[3] = {false, false, false};
+      if (v >= 0) {
+        const double* rv = a.rec + (int64_t)v * REC_W;
+        const double2* rp = reinterpret_cast<const double2*>(rv);
+        const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3];
+        double e[REC_NACC];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (r[k] != v && r[k] >= 0 && wk[k] != 0.0) {
-            dep[k] = true;
-            const double val = wk[k] * s;
-            if (sl[k] >= 0) st_agent(a.rec + sl[k], val);
-            else if (sl[k] <= -2) st_agent(a.oacc + (-2 - (int64_t)sl[k]), val);
-            old[k] = atomicSub(&a.pend[r[k]], 1);
+        for (int c = 0; c < REC_NACC; ++c) e[c] = ld_agent(rv + REC_ACC + c);  // with the rest of the line
+        const int nd = lo32(q0.x);
+        bool ready = true;
+        double s = q0.y;
+#pragma unroll
+        for (int c = 0; c < REC_NACC; ++c)
+          if (c < nd) {
+            ready = ready && !is_empty(e[c]);
+            s = s + e[c];
+          }
+        for (int c = REC_NACC; c < nd; ++c) {
+          const double o = ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
+          ready = ready && !is_empty(o);
+          s = s + o;
+        }
+        if (ready) {
+          solved = true;
+          a.x[v] = s;
+          const double wk[3] = {q1.x, q1.y, q2.x};
+          r[0] = lo32(q2.y);
+          r[1] = hi32(q2.y);
+          r[2] = lo32(q3.x);
+          const int sl[3] = {hi32(q3.x), lo32(q3.y), hi32(q3.y)};
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            if (r[k] != v && r[k] >= 0 && wk[k] != 0.0) {
+              dep[k] = true;
+              const double val = wk[k] * s;
+              if (sl[k] >= 0) st_agent(a.rec + sl[k], val);
+              else if (sl[k] <= -2) st_agent(a.oacc + (-2 - (int64_t)sl[k]), val);
+              old[k] = atomicSub(&a.pend[r[k]], 1);
+            }
           }
         }
       }
-    }
-    // go on with the first released receiver, pool the others
-    int next = -1;
-    bool ex[3] = {false, false, false};
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (dep[k] && old[k] == 1) {
-        if (next < 0) next = r[k];
-        else ex[k] = true;
+      for (int k = 0; k < 3; ++k) W.offer(dep[k] && old[k] == 1, r[k]);
+    } else {
+      int j[REC_NTGT];
+      int old[REC_NTGT];
+#pragma unroll
+      for (int c = 0; c < REC_NTGT; ++c) {
+        j[c] = 0;
+        old[c] = 0;
       }
-    unsigned long long em[3];
-    int ne = 0;
+      int nd = 0;
+      double xv = 0.0;
+      if (v >= 0) {
+        const double* rv = a.rec + (int64_t)v * REC_W;
+        const double2* rp = reinterpret_cast<const double2*>(rv);
+        const double2 q0 = rp[0], q1 = rp[1], q2 = rp[2];
+        const double2 q5 = rp[5], q6 = rp[6], q7 = rp[7];
+        const double q4y = rv[9];
+        const double xr[3] = {ld_agent(rv + REC_XR), ld_agent(rv + REC_XR + 1), ld_agent(rv + REC_XR + 2)};
+        const int mask = hi32(q0.x);
+        bool ready = true;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      em[k] = __ballot(ex[k]);
-      ne += __popcll(em[k]);
-    }
-    if (ne > 0) {
-      const int fit = ne < pool_cap - np ? ne : pool_cap - np;
-      int obase = 0;
-      if (fit < ne) {
-        if (lane == 0) obase = atomicAdd(&ctr[0], ne - fit);
-        obase = __shfl(obase, 0);
-      }
-      int base = 0;
+        for (int k = 0; k < 3; ++k)
+          if ((mask >> k) & 1) ready = ready && !is_empty(xr[k]);
+        if (ready) {
+          solved = true;
+          nd = lo32(q0.x);
+          double s = q0.y;
+          const double dg = q1.x;
+          const double o[3] = {q1.y, q2.x, q2.y};
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (ex[k]) {
-          const int idx = base + __popcll(em[k] & below);
-          const int64_t oi = (int64_t)obase + idx - fit;
-          if (idx < fit) pool[np + idx] = r[k];
-          else if (oi < N) olist[oi] = r[k];
-          else __hip_atomic_store(&ctr[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          for (int k = 0; k < 3; ++k)
+            if ((mask >> k) & 1) s = s - o[k] * xr[k];
+          xv = s / dg;
+          a.x[v] = xv;
+          const int t[REC_NTGT] = {lo32(q4y), hi32(q4y), lo32(q5.x), hi32(q5.x), lo32(q5.y),
+                                   hi32(q5.y), lo32(q6.x), hi32(q6.x), lo32(q6.y), hi32(q6.y),
+                                   lo32(q7.x), hi32(q7.x), lo32(q7.y), hi32(q7.y)};
+#pragma unroll
+          for (int c = 0; c < REC_NTGT; ++c) {
+            if (c < nd) {
+              st_agent(a.rec + t[c], xv);
+              j[c] = t[c] >> REC_LOG;
+              old[c] = atomicSub(&a.pend[j[c]], 1);
+            }
+          }
         }
-        base += __popcll(em[k]);
       }
-      np += fit;
-      wave_sync();
+#pragma unroll
+      for (int c = 0; c < REC_NTGT; ++c) W.offer(c < nd && old[c] == 1, j[c]);
+      // rows longer than the record (more than 14 donor entries)
+      for (int c = REC_NTGT; __ballot(c < nd) != 0ull; ++c) {
+        bool rl = false;
+        int jj = 0;
+        if (c < nd) {
+          const int tc = a.odon[(int64_t)(c - REC_NTGT) * N + v];
+          st_agent(a.rec + tc, xv);
+          jj = tc >> REC_LOG;
+          rl = atomicSub(&a.pend[jj], 1) == 1;
+        }
+        W.offer(rl, jj);
+      }
     }
+    // share work with idle wavefronts once the pool is long
+    if (W.np >= W.share_at + 64) W.spill();
     const bool waiting = v >= 0 && !solved;
     if (solved) {
-      v = next;
+      v = W.next;
       retry = 0;
-    } else if (waiting && ++retry > CHAIN_SPIN_MAX) {
-      __hip_atomic_store(&ctr[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // the abort flag is read only by wavefronts that wait on a row
-    if (__ballot(waiting) != 0ull) {
+    if ((waiting || ci >= 0) && ++retry > CHAIN_SPIN_MAX) st_agent_i(&ctl[CQ_ABORT], 1);
+    // the abort flag is read only by wavefronts that wait on something
+    if (__ballot(waiting || ci >= 0) != 0ull) {
       int ab = 0;
-      if (lane == 0) ab = __hip_atomic_load(&ctr[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) ab = ld_agent_i(&ctl[CQ_ABORT]);
       if (__shfl(ab, 0) != 0) break;
     }
   }
@@ -1171,74 +1327,65 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
-  const bool chain = rec && mode == 0 && m->dag_chain;
-  // dataflow sweep: ctr[0] overflow count, ctr[1] abort flag (two sets, by pass parity)
-  int32_t* ctr = dev + 56;
-  int32_t* ovf = m->wi[3];  // overflow list (start list of the next pass)
-  if (chain) GOSPL_CHECK(hipMemsetAsync(ctr, 0, sizeof(int32_t) * 4, s));
+  const bool chain = rec && m->dag_chain;
+  // dataflow sweep: global batch queue (entries start at -1) and control words
+  int32_t* gq = m->wi[3];
+  const int64_t gq_cap = N / 64 + 1;
+  if (chain) {
+    if (!m->chain_ctl) GOSPL_CHECK(hipMalloc(&m->chain_ctl, sizeof(int32_t) * CQ_WORDS));
+    GOSPL_CHECK(hipMemsetAsync(m->chain_ctl, 0, sizeof(int32_t) * CQ_WORDS, s));
+    GOSPL_CHECK(hipMemsetAsync(gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
+  }
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   if (chain) {
-    k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-    GOSPL_CHECK(hipGetLastError());
-    // the level-0 releases are the first start list
-    k_flow_chain<<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, ovf, ctr, m->chain_pool);
-    GOSPL_CHECK(hipGetLastError());
-  } else if (rec) {
+    // level 0 on the full grid; its releases are the start list
     if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+    GOSPL_CHECK(hipGetLastError());
+    if (mode == 0)
+      k_chain<0><<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
+                                                           m->chain_pool, m->chain_share);
+    else
+      k_chain<1><<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
+                                                           m->chain_pool, m->chain_share);
+    GOSPL_CHECK(hipGetLastError());
   } else {
-    if (mode == 0) k_dag_level0<0, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-    else k_dag_level0<1, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-  }
-  GOSPL_CHECK(hipGetLastError());
-  if (chain) {
-  } else if (m->dag_coop)
-    GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
-  else  // plain launch (counter collection); the grid fits the GPU many times over
-    GOSPL_CHECK(hipLaunchKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
-  if (m->dag_tail > 0 && !chain) {
+    // level by level: level 0 on the full grid, the persistent level kernel,
+    // then the one-workgroup tail
     if (rec) {
-      if (mode == 0) k_dag_tail<0, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
-      else k_dag_tail<1, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+      else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     } else {
-      if (mode == 0) k_dag_tail<0, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
-      else k_dag_tail<1, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      if (mode == 0) k_dag_level0<0, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
+      else k_dag_level0<1, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     }
     GOSPL_CHECK(hipGetLastError());
+    if (m->dag_coop)
+      GOSPL_CHECK(hipLaunchCooperativeKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
+    else  // plain launch (counter collection); the grid fits the GPU many times over
+      GOSPL_CHECK(hipLaunchKernel(kern, dim3(m->coop_blocks), dim3(m->dag_threads), args, 0, s));
+    if (m->dag_tail > 0) {
+      if (rec) {
+        if (mode == 0) k_dag_tail<0, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+        else k_dag_tail<1, true><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      } else {
+        if (mode == 0) k_dag_tail<0, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+        else k_dag_tail<1, false><<<1, TAIL_THREADS, 0, s>>>(a, fa, fb, counts, bar, nlev, trace);
+      }
+      GOSPL_CHECK(hipGetLastError());
+    }
   }
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
   int32_t* left = fa;  // the frontier lists are free again
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
-  // abort word; dataflow sweep: overflow count and abort flag
-  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, chain ? ctr + 1 : (int32_t*)(bar + 2), sizeof(int32_t),
-                             hipMemcpyDeviceToHost, s));
-  if (chain) GOSPL_CHECK(hipMemcpyAsync(m->hflag + 35, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  // abort word
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, chain ? m->chain_ctl + CQ_ABORT : (int32_t*)(bar + 2),
+                             sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
-  if (chain && m->hflag[35] > 0 && !m->hflag[34]) {
-    // releases that overflowed a pool: run them as the next start list until
-    // none is left, then list the unsolved vertices again
-    for (int pass = 1; m->hflag[35] > 0 && !m->hflag[34]; ++pass) {
-      int32_t* in = (pass & 1) ? ovf : fb;
-      int32_t* out = (pass & 1) ? fb : ovf;
-      int32_t* cin = ctr + 2 * ((pass + 1) & 1);
-      int32_t* cout = ctr + 2 * (pass & 1);
-      GOSPL_CHECK(hipMemsetAsync(cout, 0, sizeof(int32_t) * 2, s));
-      k_flow_chain<<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, in, cin, out, cout, m->chain_pool);
-      GOSPL_CHECK(hipGetLastError());
-      GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, cout + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      GOSPL_CHECK(hipMemcpyAsync(m->hflag + 35, cout, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      GOSPL_CHECK(hipStreamSynchronize(s));
-    }
-    GOSPL_CHECK(hipMemsetAsync(dev + 1, 0, sizeof(int32_t), s));
-    k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
-    GOSPL_CHECK(hipGetLastError());
-    GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
-    GOSPL_CHECK(hipStreamSynchronize(s));
-  }
   if (m->hflag[34]) {
-    set_error(chain ? "flow-accumulation sweep aborted: a bounded wait ran out"
+    set_error(chain ? "dataflow sweep aborted: a bounded wait ran out"
                     : "receiver-DAG sweep aborted: a grid barrier did not complete");
     return 1;
   }
@@ -1315,20 +1462,31 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   }
   if (per_cu < 1) per_cu = 1;
   const int maxb = per_cu * m->num_cu;
-  // dataflow flow accumulation (GOSPL_DAG_CHAIN=0: level by level); its grid
-  // is what the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides)
+  // dataflow sweeps (GOSPL_DAG_CHAIN=0: level by level); the grid is what
+  // the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides)
   const char* te_chain = getenv("GOSPL_DAG_CHAIN");
   m->dag_chain = !(te_chain && atoi(te_chain) == 0);
-  int cb = 0;
-  GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&cb, (const void*)k_flow_chain, CHAIN_THREADS, 0));
+  int cb = 1 << 30;
+  const void* kc[2] = {(const void*)k_chain<0>, (const void*)k_chain<1>};
+  for (int i = 0; i < 2; ++i) {
+    int b = 0;
+    GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kc[i], CHAIN_THREADS, 0));
+    cb = b < cb ? b : cb;
+  }
   if (cb < 1) cb = 1;
   m->chain_blocks = cb * m->num_cu;
   const char* te_cb = getenv("GOSPL_CHAIN_BLOCKS");
   if (te_cb && atoi(te_cb) > 0) m->chain_blocks = atoi(te_cb);
-  // GOSPL_CHAIN_POOL: pool capacity per wavefront (1..512; small values force overflow passes)
+  // pool capacity per wavefront (GOSPL_CHAIN_POOL, 128..512) and the pool
+  // size from which batches go to the global queue (GOSPL_CHAIN_SHARE,
+  // 64..capacity - 64; small values exercise the queue)
   m->chain_pool = CHAIN_POOL;
   const char* te_pool = getenv("GOSPL_CHAIN_POOL");
-  if (te_pool && atoi(te_pool) >= 1 && atoi(te_pool) <= CHAIN_POOL) m->chain_pool = atoi(te_pool);
+  if (te_pool && atoi(te_pool) >= 128 && atoi(te_pool) <= CHAIN_POOL) m->chain_pool = atoi(te_pool);
+  m->chain_share = 128;
+  const char* te_share = getenv("GOSPL_CHAIN_SHARE");
+  if (te_share && atoi(te_share) >= 64) m->chain_share = atoi(te_share);
+  if (m->chain_share > m->chain_pool - 64) m->chain_share = m->chain_pool - 64;
   // Few blocks: the grid barrier costs ~3 us at 32 blocks and ~18 us at 256 on
   // MI355X, and the frontiers past level 0 hold a few thousand vertices.
   int want = 32;

@@ -115,8 +115,8 @@ int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double s
  * exact sweep over the receiver DAG.  rcv/wgt (nrcv, N), b (N) -> q (N).
  * *nlevels (HOST, may be NULL) receives the DAG depth when the sweep runs
  * level by level, and 0 for the dataflow sweep (the default for up to three
- * receivers; GOSPL_DAG_CHAIN=0 at mesh creation selects the level sweep).
- * Both give the same bits.  Receivers built with
+ * receivers; GOSPL_DAG_CHAIN=0 at mesh creation selects the level sweep for
+ * this call and gospl_spl).  Both give the same bits.  Receivers built with
  * flat routing on a partly filled surface can hold a cycle; the vertices the
  * sweep cannot reach (the cycle and everything below it) are then solved by
  * the fixed-point iteration q <- b + W^T q restricted to them, to a relative
@@ -136,7 +136,8 @@ int gospl_flowacc(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const do
  * kfac (N, may be NULL) = per-vertex K multiplier (sedfactor * rain^d);
  * sea = lfill <= sealevel.  Updates h, cumed; writes eb (N) =
  * erosion/deposition rate with outlets (flat meshes) and lsink cells zeroed,
- * and eblocal = (eb*dt)/dt.
+ * and eblocal = (eb*dt)/dt.  *nlevels (HOST, may be NULL) as for
+ * gospl_flowacc.
  */
 int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
               const double* disti, const double* fal, const double* kfac, double K, double m,

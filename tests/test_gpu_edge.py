@@ -169,24 +169,28 @@ def test_csr_rows_match_packed_records():
         assert np.array_equal(a, b)
 
 
-def test_dataflow_sweep_matches_level_sweep():
-    """Flow accumulation as a dataflow sweep (the default), with its normal
-    pool and with a one-entry pool (releases overflow into extra passes),
-    gives the bits of the level-by-level sweep."""
+@pytest.mark.parametrize("level", [6, 8])
+def test_dataflow_sweep_matches_level_sweep(level):
+    """The dataflow sweeps (the default for flow accumulation and stream
+    power), with their normal pools and with small pools that hand most
+    releases to the global queue, give the bits of the level-by-level
+    sweeps."""
     import os
 
     from gospl_amd.model import Model
 
-    m, h0, p = case("sphere", 6)
+    m, h0, p = case("sphere", level)
     out = []
-    for chain, pool in (("0", "512"), ("1", "512"), ("1", "1")):
+    for chain, pool, share in (("0", "512", "128"), ("1", "512", "128"), ("1", "128", "64")):
         os.environ["GOSPL_DAG_CHAIN"] = chain
         os.environ["GOSPL_CHAIN_POOL"] = pool
+        os.environ["GOSPL_CHAIN_SHARE"] = share
         try:
             g = Model(m, h0, tend=1e12, **p)
         finally:
             del os.environ["GOSPL_DAG_CHAIN"]
             del os.environ["GOSPL_CHAIN_POOL"]
+            del os.environ["GOSPL_CHAIN_SHARE"]
         g.initialise()
         for _ in range(3):
             g.step()

@@ -30,10 +30,12 @@ struct gospl_mesh {
   int dag_coop;               // launch the persistent DAG kernel with the cooperative API
   int dag_csr;                // use the CSR rows even when packed records apply
   int dag_chain;              // flow accumulation as a dataflow sweep (records only)
-  int chain_blocks;           // its grid
+  int chain_blocks[2];        // its grid, by sweep (flow, stream power)
   int chain_pool;             // its pool capacity per wavefront
   int chain_share;            // pool size from which batches go to the global queue
+  int chain_pollers;          // idle wavefronts that stay to poll it
   int32_t* chain_ctl;         // its control words
+  long long* trace_v;         // its per-vertex solve times (GOSPL_DAG_TRACE)
   // workspace
   int32_t* wi[8];             // 8 x N int32
   unsigned long long* wu[2];  // 2 x N uint64

@@ -21,7 +21,9 @@
 // costs about three dependent memory round trips plus the barrier.
 // Every value is a fixed-order sum over its own row, so results do not depend
 // on the schedule (no floating-point atomics).
+#include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -786,6 +788,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
 #define CQ_IDLERS 64  // idle wavefronts polling the queue
 #define CQ_ABORT 96
 #define CQ_WORDS 128
+#define CQ_RETRIES 112  // traced sweeps: passes that found a row incomplete
 
 __device__ __forceinline__ double ld_agent(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -854,7 +857,8 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
                                                        const int32_t* __restrict__ nlist,
                                                        int32_t* __restrict__ gq, int64_t gq_cap,
                                                        int32_t* __restrict__ ctl, int pool_cap,
-                                                       int share_at) {
+                                                       int share_at, int pollers,
+                                                       long long* __restrict__ trace) {
   __shared__ int32_t s_pool[CHAIN_THREADS / 64][CHAIN_POOL];
   const int64_t N = a.N;
   const int lane = __lane_id();
@@ -874,6 +878,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
   unsigned long long* qword = reinterpret_cast<unsigned long long*>(&ctl[CQ_QUEUE]);
   if (lane == 0) atomicAdd(&ctl[CQ_ACTIVE], 1);
   for (;;) {
+    const long long t_top = trace ? (long long)wall_clock64() : 0;
     // lanes waiting on a claimed queue entry
     if (ci >= 0) {
       const int q = ld_agent_i(&gq[ci]);
@@ -922,7 +927,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         } else if (ld_agent_i(&ctl[CQ_ABORT]) != 0) {
           stop = 1;
         } else if (!poller) {
-          if (atomicAdd(&ctl[CQ_IDLERS], 1) < CHAIN_POLLERS) poller = true;
+          if (atomicAdd(&ctl[CQ_IDLERS], 1) < pollers) poller = true;
           else stop = 1;
         } else if (ld_agent_i(&ctl[CQ_ACTIVE]) == 0) {
           stop = 1;
@@ -981,6 +986,10 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         if (ready) {
           solved = true;
           a.x[v] = s;
+          if (trace) {
+            trace[v] = (long long)wall_clock64();
+            trace[N + v] = t_top;
+          }
           const double wk[3] = {q1.x, q1.y, q2.x};
           r[0] = lo32(q2.y);
           r[1] = hi32(q2.y);
@@ -1033,6 +1042,10 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
             if ((mask >> k) & 1) s = s - o[k] * xr[k];
           xv = s / dg;
           a.x[v] = xv;
+          if (trace) {
+            trace[v] = (long long)wall_clock64();
+            trace[N + v] = t_top;
+          }
           const int t[REC_NTGT] = {lo32(q4y), hi32(q4y), lo32(q5.x), hi32(q5.x), lo32(q5.y),
                                    hi32(q5.y), lo32(q6.x), hi32(q6.x), lo32(q6.y), hi32(q6.y),
                                    lo32(q7.x), hi32(q7.x), lo32(q7.y), hi32(q7.y)};
@@ -1064,6 +1077,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
     // share work with idle wavefronts once the pool is long
     if (W.np >= W.share_at + 64) W.spill();
     const bool waiting = v >= 0 && !solved;
+    if (trace && waiting) atomicAdd(&ctl[CQ_RETRIES], 1);
     if (solved) {
       v = W.next;
       retry = 0;
@@ -1324,6 +1338,13 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     trace = m->trace_buf;
     GOSPL_CHECK(hipMemsetAsync(trace, 0, sizeof(long long) * 2 * 8192, s));
   }
+  // dataflow sweeps trace one solve time per vertex instead
+  long long* vtrace = nullptr;
+  if (do_trace && rec && m->dag_chain) {
+    if (!m->trace_v) GOSPL_CHECK(hipMalloc(&m->trace_v, sizeof(long long) * 2 * N));
+    vtrace = m->trace_v;
+    GOSPL_CHECK(hipMemsetAsync(vtrace, 0, sizeof(long long) * 2 * N, s));
+  }
   void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
@@ -1343,11 +1364,11 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     GOSPL_CHECK(hipGetLastError());
     if (mode == 0)
-      k_chain<0><<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
-                                                           m->chain_pool, m->chain_share);
+      k_chain<0><<<m->chain_blocks[0], CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
+                                                           m->chain_pool, m->chain_share, m->chain_pollers, vtrace);
     else
-      k_chain<1><<<m->chain_blocks, CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
-                                                           m->chain_pool, m->chain_share);
+      k_chain<1><<<m->chain_blocks[1], CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
+                                                           m->chain_pool, m->chain_share, m->chain_pollers, vtrace);
     GOSPL_CHECK(hipGetLastError());
   } else {
     // level by level: level 0 on the full grid, the persistent level kernel,
@@ -1403,6 +1424,64 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
     m->t_ms[mode] += ms;
     m->t_calls[mode] += 1;
+  }
+  if (vtrace) {
+    // solve times relative to the first one: quantiles, and what the last 10 % of the time solved
+    std::vector<long long> t(N);
+    GOSPL_CHECK(hipMemcpy(t.data(), vtrace, sizeof(long long) * N, hipMemcpyDeviceToHost));
+    std::vector<long long> u;
+    for (long long x : t)
+      if (x) u.push_back(x);
+    std::sort(u.begin(), u.end());
+    if (!u.empty()) {
+      const double span = (u.back() - u.front()) * 0.01;  // us, 100 MHz counter
+      auto q = [&](double f) { return (u[(size_t)(f * (u.size() - 1))] - u.front()) * 0.01; };
+      size_t late = 0;
+      for (long long x : u)
+        if ((x - u.front()) * 0.01 > 0.9 * span) late++;
+      int retries = 0;
+      GOSPL_CHECK(hipMemcpy(&retries, m->chain_ctl + CQ_RETRIES, sizeof(int), hipMemcpyDeviceToHost));
+      fprintf(stderr, "[dataflow mode %d] %zu solves over %.0f us; quantiles (us) 50%% %.0f 90%% %.0f "
+              "99%% %.0f 99.9%% %.0f; last 10%% of the time: %zu solves; %d incomplete-row passes\n",
+              mode, u.size(), span, q(0.5), q(0.9), q(0.99), q(0.999), late, retries);
+      // hop latency: a vertex's solve time after the last of its dependencies
+      std::vector<int32_t> hr((size_t)nrcv * N);
+      std::vector<double> hc((size_t)nrcv * N);
+      GOSPL_CHECK(hipMemcpy(hr.data(), rcv, sizeof(int32_t) * nrcv * N, hipMemcpyDeviceToHost));
+      GOSPL_CHECK(hipMemcpy(hc.data(), coef, sizeof(double) * nrcv * N, hipMemcpyDeviceToHost));
+      std::vector<long long> last(N, 0);
+      for (int64_t i = 0; i < N; ++i)
+        for (int k = 0; k < nrcv; ++k) {
+          const int r = hr[(size_t)k * N + i];
+          if (r == (int)i || r < 0 || hc[(size_t)k * N + i] == 0.0) continue;
+          // flow: i before r; stream power: r before i
+          const int64_t from = mode == 0 ? i : r, to = mode == 0 ? r : i;
+          if (t[from] > last[to]) last[to] = t[from];
+        }
+      std::vector<long long> hop, pre, post;
+      std::vector<long long> top(N);
+      GOSPL_CHECK(hipMemcpy(top.data(), vtrace + N, sizeof(long long) * N, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < N; ++i)
+        if (t[i] && last[i]) {
+          hop.push_back(t[i] - last[i]);
+          pre.push_back(top[i] - last[i]);  // last dependency solved -> pass that solves i starts
+          post.push_back(t[i] - top[i]);    // that pass: start -> i solved (record read)
+        }
+      std::sort(pre.begin(), pre.end());
+      std::sort(post.begin(), post.end());
+      if (!pre.empty()) {
+        auto pq = [&](std::vector<long long>& h, double f) { return h[(size_t)(f * (h.size() - 1))] * 10.0; };
+        fprintf(stderr, "[dataflow mode %d] dependency solved -> pass start (ns): 10%% %.0f 50%% %.0f 90%% %.0f; "
+                "pass start -> solved: 10%% %.0f 50%% %.0f 90%% %.0f\n", mode, pq(pre, 0.1), pq(pre, 0.5),
+                pq(pre, 0.9), pq(post, 0.1), pq(post, 0.5), pq(post, 0.9));
+      }
+      std::sort(hop.begin(), hop.end());
+      if (!hop.empty()) {
+        auto hq = [&](double f) { return hop[(size_t)(f * (hop.size() - 1))] * 10.0; };
+        fprintf(stderr, "[dataflow mode %d] hop after last dependency (ns): 10%% %.0f 50%% %.0f 90%% %.0f "
+                "99%% %.0f\n", mode, hq(0.1), hq(0.5), hq(0.9), hq(0.99));
+      }
+    }
   }
   if (trace) {
     static long long h[2 * 8192];
@@ -1466,17 +1545,14 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   // the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides)
   const char* te_chain = getenv("GOSPL_DAG_CHAIN");
   m->dag_chain = !(te_chain && atoi(te_chain) == 0);
-  int cb = 1 << 30;
   const void* kc[2] = {(const void*)k_chain<0>, (const void*)k_chain<1>};
+  const char* te_cb = getenv("GOSPL_CHAIN_BLOCKS");
   for (int i = 0; i < 2; ++i) {
     int b = 0;
     GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kc[i], CHAIN_THREADS, 0));
-    cb = b < cb ? b : cb;
+    m->chain_blocks[i] = (b < 1 ? 1 : b) * m->num_cu;
+    if (te_cb && atoi(te_cb) > 0) m->chain_blocks[i] = atoi(te_cb);
   }
-  if (cb < 1) cb = 1;
-  m->chain_blocks = cb * m->num_cu;
-  const char* te_cb = getenv("GOSPL_CHAIN_BLOCKS");
-  if (te_cb && atoi(te_cb) > 0) m->chain_blocks = atoi(te_cb);
   // pool capacity per wavefront (GOSPL_CHAIN_POOL, 128..512) and the pool
   // size from which batches go to the global queue (GOSPL_CHAIN_SHARE,
   // 64..capacity - 64; small values exercise the queue)
@@ -1487,6 +1563,11 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* te_share = getenv("GOSPL_CHAIN_SHARE");
   if (te_share && atoi(te_share) >= 64) m->chain_share = atoi(te_share);
   if (m->chain_share > m->chain_pool - 64) m->chain_share = m->chain_pool - 64;
+  // idle wavefronts that stay to poll the global queue (GOSPL_CHAIN_POLLERS)
+  m->chain_pollers = CHAIN_POLLERS;
+  const char* te_pol = getenv("GOSPL_CHAIN_POLLERS");
+  if (te_pol && atoi(te_pol) >= 0) m->chain_pollers = atoi(te_pol);
+
   // Few blocks: the grid barrier costs ~3 us at 32 blocks and ~18 us at 256 on
   // MI355X, and the frontiers past level 0 hold a few thousand vertices.
   int want = 32;

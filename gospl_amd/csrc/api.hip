@@ -169,6 +169,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->counts);
   dfree(m->dflag);
   dfree(m->chain_ctl);
+  dfree(m->chain_gq);
   dfree(m->trace_v);
   if (m->hflag) (void)hipHostFree(m->hflag);
   dfree(m->cub_tmp);

@@ -35,6 +35,7 @@ struct gospl_mesh {
   int chain_share;            // pool size from which batches go to the global queue
   int chain_pollers;          // idle wavefronts that stay to poll it
   int32_t* chain_ctl;         // its control words
+  int32_t* chain_gq;          // its global queue (-1 between sweeps)
   long long* trace_v;         // its per-vertex solve times (GOSPL_DAG_TRACE)
   // workspace
   int32_t* wi[8];             // 8 x N int32

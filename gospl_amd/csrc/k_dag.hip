@@ -371,32 +371,36 @@ __global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
 }
 
-// Position of receiver r in the neighbour list of i, then of i in that of r.
-__device__ __forceinline__ int rev_pos(int64_t N, int W, const int32_t* __restrict__ nbr,
-                                       const int32_t* __restrict__ deg,
-                                       const int8_t* __restrict__ rev, int64_t i, int r) {
-  const int dg = deg[i];
-  for (int q = 0; q < W && q < dg; ++q)
-    if (nbr[(int64_t)q * N + i] == r) return rev[(int64_t)q * N + i];
-  return -1;
-}
-
 // Row masks: receiver r's row is the bitmask over (neighbour position p,
 // receiver slot k), bit p * NR + k, of the donors that send to r through
 // their slot k; bit order is row order (FV neighbour order, then slot).  Set
 // from the donor side, one atomicOr per dependency (the mask is zeroed
 // before).
+// The position of i in the list of its receiver k is kept in rpos[k * N + i]
+// (-1: no dependency) for the record builds.
 __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                            const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
                            const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-                           unsigned long long* __restrict__ rowmask) {
+                           unsigned long long* __restrict__ rowmask, int8_t* __restrict__ rpos) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  // the neighbour list once, searched for every receiver
+  int nb[GOSPL_MAXW];
+  const int dg = deg[i];
+#pragma unroll
+  for (int q = 0; q < GOSPL_MAXW; ++q) nb[q] = (q < W && q < dg) ? nbr[(int64_t)q * N + i] : -1;
   for (int k = 0; k < NR; ++k) {
     const int r = rcv[(int64_t)k * N + i];
-    if (r == (int)i || r < 0 || coef[(int64_t)k * N + i] == 0.0) continue;
-    const int p = rev_pos(N, W, nbr, deg, rev, i, r);
-    if (p >= 0) atomicOr(&rowmask[r], 1ull << (p * NR + k));
+    int p = -1;
+    if (r != (int)i && r >= 0 && coef[(int64_t)k * N + i] != 0.0) {
+      int q = -1;
+#pragma unroll
+      for (int t = GOSPL_MAXW - 1; t >= 0; --t)
+        if (nb[t] == r) q = t;
+      if (q >= 0) p = rev[(int64_t)q * N + i];
+      if (p >= 0) atomicOr(&rowmask[r], 1ull << (p * NR + k));
+    }
+    rpos[(int64_t)k * N + i] = (int8_t)p;
   }
 }
 
@@ -409,8 +413,7 @@ __device__ __forceinline__ bool is_empty(double d) { return __double_as_longlong
 // Whole flow-accumulation record of vertex i, written at once.  The slot of
 // (i, receiver k) is the rank of its bit in the receiver's row mask.  Row
 // entries start empty.  pend[i] = row length.
-__global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
+__global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
                              const int32_t* __restrict__ rcv, const double* __restrict__ coef,
                              const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
@@ -427,8 +430,7 @@ __global__ void k_rec_build0(int64_t N, int W, int NR, const int32_t* __restrict
     r[k] = rcv[(int64_t)k * N + i];
     w[k] = coef[(int64_t)k * N + i];
     const int rk = r[k];
-    if (rk == (int)i || rk < 0 || w[k] == 0.0) continue;
-    const int p = rev_pos(N, W, nbr, deg, rev, i, rk);
+    const int p = rpos[(int64_t)k * N + i];
     if (p < 0) continue;
     const int c = __popcll(rowmask[rk] & ((1ull << (p * NR + k)) - 1ull));
     sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
@@ -883,6 +885,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
     if (ci >= 0) {
       const int q = ld_agent_i(&gq[ci]);
       if (q >= 0) {
+        st_agent_i(&gq[ci], -1);  // the queue is empty again when the sweep ends
         v = q;
         ci = -1;
         retry = 0;
@@ -1296,7 +1299,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* dev = m->dflag;
   // packed records: up to 3 receivers, int32 word indices, row masks of W * nrcv <= 64 bits
   const bool rec = nrcv <= 3 && !m->dag_csr && N * REC_W < (int64_t)INT32_MAX && m->W * nrcv <= 64;
-  GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (N + 2), s));
+  // per-level frontier counts (the dataflow sweep uses counts[1] only)
+  GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (rec && m->dag_chain ? 2 : N + 2), s));
   GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
   unsigned long long* bar = reinterpret_cast<unsigned long long*>(dev + 32);
   GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned long long) * 3, s));
@@ -1308,10 +1312,10 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (ensure_donors(m, (int64_t)nrcv * m->W * N)) return 1;
     unsigned long long* rowmask = m->wu[0];
     GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
-    k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask);
+    int8_t* rpos = reinterpret_cast<int8_t*>(m->wi[5]);  // 3 N bytes
+    k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos);
     if (mode == 0)
-      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rhs,
-                                              rowmask, m->wrec, m->wacc, pend);
+      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc, pend);
     else
       k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rcv, coef, diag, rhs, rowmask, m->wrec,
                                               m->wdon, pend);
@@ -1349,14 +1353,18 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
   const bool chain = rec && m->dag_chain;
-  // dataflow sweep: global batch queue (entries start at -1) and control words
-  int32_t* gq = m->wi[3];
+  // dataflow sweep: global batch queue (entries at -1 between sweeps: each
+  // taken entry is reset) and control words
   const int64_t gq_cap = N / 64 + 1;
   if (chain) {
-    if (!m->chain_ctl) GOSPL_CHECK(hipMalloc(&m->chain_ctl, sizeof(int32_t) * CQ_WORDS));
+    if (!m->chain_ctl) {
+      GOSPL_CHECK(hipMalloc(&m->chain_ctl, sizeof(int32_t) * CQ_WORDS));
+      GOSPL_CHECK(hipMalloc(&m->chain_gq, sizeof(int32_t) * gq_cap * 64));
+      GOSPL_CHECK(hipMemsetAsync(m->chain_gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
+    }
     GOSPL_CHECK(hipMemsetAsync(m->chain_ctl, 0, sizeof(int32_t) * CQ_WORDS, s));
-    GOSPL_CHECK(hipMemsetAsync(gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
   }
+  int32_t* gq = m->chain_gq;
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   if (chain) {
     // level 0 on the full grid; its releases are the start list
@@ -1406,6 +1414,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
                              sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
   if (m->hflag[34]) {
+    // an aborted dataflow sweep can leave queue entries behind
+    if (chain) GOSPL_CHECK(hipMemsetAsync(gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
     set_error(chain ? "dataflow sweep aborted: a bounded wait ran out"
                     : "receiver-DAG sweep aborted: a grid barrier did not complete");
     return 1;
@@ -1541,8 +1551,10 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   }
   if (per_cu < 1) per_cu = 1;
   const int maxb = per_cu * m->num_cu;
-  // dataflow sweeps (GOSPL_DAG_CHAIN=0: level by level); the grid is what
-  // the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides)
+  // dataflow sweeps (GOSPL_DAG_CHAIN=0: level by level); two workgroups per
+  // CU, within what the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides).
+  // More resident wavefronts do not shorten the dependency chains and only
+  // share the SIMDs with the chain that is on the longest path.
   const char* te_chain = getenv("GOSPL_DAG_CHAIN");
   m->dag_chain = !(te_chain && atoi(te_chain) == 0);
   const void* kc[2] = {(const void*)k_chain<0>, (const void*)k_chain<1>};
@@ -1550,7 +1562,7 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   for (int i = 0; i < 2; ++i) {
     int b = 0;
     GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kc[i], CHAIN_THREADS, 0));
-    m->chain_blocks[i] = (b < 1 ? 1 : b) * m->num_cu;
+    m->chain_blocks[i] = (b < 1 ? 1 : b > 2 ? 2 : b) * m->num_cu;
     if (te_cb && atoi(te_cb) > 0) m->chain_blocks[i] = atoi(te_cb);
   }
   // pool capacity per wavefront (GOSPL_CHAIN_POOL, 128..512) and the pool

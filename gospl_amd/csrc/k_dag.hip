@@ -894,7 +894,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
     // idle lanes take pooled vertices, then the next ones of the share
     const bool need = v < 0 && ci < 0;
     const unsigned long long nm = __ballot(need);
-    if (nm != 0ull) {
+    if (nm != 0ull && (W.np > 0 || cur < end)) {
       const int rank = __popcll(nm & W.below);
       const int nn = __popcll(nm);
       const int fp = nn < W.np ? nn : W.np;
@@ -975,12 +975,14 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         const int nd = lo32(q0.x);
         bool ready = true;
         double s = q0.y;
+        // branch-free over the record's entries (selects, not branches)
 #pragma unroll
-        for (int c = 0; c < REC_NACC; ++c)
-          if (c < nd) {
-            ready = ready && !is_empty(e[c]);
-            s = s + e[c];
-          }
+        for (int c = 0; c < REC_NACC; ++c) {
+          const bool in = c < nd;
+          ready = ready && (!in || !is_empty(e[c]));
+          const double t = s + e[c];
+          s = in ? t : s;
+        }
         for (int c = REC_NACC; c < nd; ++c) {
           const double o = ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
           ready = ready && !is_empty(o);

@@ -34,6 +34,7 @@ struct gospl_mesh {
   int chain_pool;             // its pool capacity per wavefront
   int chain_share;            // pool size from which batches go to the global queue
   int chain_pollers;          // idle wavefronts that stay to poll it
+  int chain_spin;             // bound of its waits
   int32_t* chain_ctl;         // its control words
   int32_t* chain_gq;          // its global queue (-1 between sweeps)
   long long* trace_v;         // its per-vertex solve times (GOSPL_DAG_TRACE)

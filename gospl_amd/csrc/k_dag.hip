@@ -859,7 +859,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
                                                        const int32_t* __restrict__ nlist,
                                                        int32_t* __restrict__ gq, int64_t gq_cap,
                                                        int32_t* __restrict__ ctl, int pool_cap,
-                                                       int share_at, int pollers,
+                                                       int share_at, int pollers, int spin_max,
                                                        long long* __restrict__ trace) {
   __shared__ int32_t s_pool[CHAIN_THREADS / 64][CHAIN_POOL];
   const int64_t N = a.N;
@@ -950,7 +950,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         continue;
       }
       if (stop) break;
-      if (++idle > CHAIN_SPIN_MAX) {
+      if (++idle > spin_max) {
         if (lane == 0) st_agent_i(&ctl[CQ_ABORT], 1);
         break;
       }
@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
       v = W.next;
       retry = 0;
     }
-    if ((waiting || ci >= 0) && ++retry > CHAIN_SPIN_MAX) st_agent_i(&ctl[CQ_ABORT], 1);
+    if ((waiting || ci >= 0) && ++retry > spin_max) st_agent_i(&ctl[CQ_ABORT], 1);
     // the abort flag is read only by wavefronts that wait on something
     if (__ballot(waiting || ci >= 0) != 0ull) {
       int ab = 0;
@@ -1375,10 +1375,10 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     GOSPL_CHECK(hipGetLastError());
     if (mode == 0)
       k_chain<0><<<m->chain_blocks[0], CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
-                                                           m->chain_pool, m->chain_share, m->chain_pollers, vtrace);
+                                                           m->chain_pool, m->chain_share, m->chain_pollers, m->chain_spin, vtrace);
     else
       k_chain<1><<<m->chain_blocks[1], CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
-                                                           m->chain_pool, m->chain_share, m->chain_pollers, vtrace);
+                                                           m->chain_pool, m->chain_share, m->chain_pollers, m->chain_spin, vtrace);
     GOSPL_CHECK(hipGetLastError());
   } else {
     // level by level: level 0 on the full grid, the persistent level kernel,
@@ -1577,6 +1577,11 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   const char* te_share = getenv("GOSPL_CHAIN_SHARE");
   if (te_share && atoi(te_share) >= 64) m->chain_share = atoi(te_share);
   if (m->chain_share > m->chain_pool - 64) m->chain_share = m->chain_pool - 64;
+  // bound of every wait in the dataflow kernel (GOSPL_CHAIN_SPIN_MAX: small
+  // values force the abort path, for testing it)
+  m->chain_spin = CHAIN_SPIN_MAX;
+  const char* te_spin = getenv("GOSPL_CHAIN_SPIN_MAX");
+  if (te_spin && atoi(te_spin) >= 0) m->chain_spin = atoi(te_spin);
   // idle wavefronts that stay to poll the global queue (GOSPL_CHAIN_POLLERS)
   m->chain_pollers = CHAIN_POLLERS;
   const char* te_pol = getenv("GOSPL_CHAIN_POLLERS");

@@ -201,6 +201,39 @@ def test_dataflow_sweep_matches_level_sweep(level):
             assert np.array_equal(a, b)
 
 
+def test_dataflow_bounded_wait_aborts_cleanly():
+    """Every wait of the dataflow kernel is bounded.  With the bound forced
+    to zero from the host, the sweep gives up at its first wait: every
+    wavefront leaves, the call reports the abort as an error, and a sweep on
+    a normal handle afterwards is unaffected."""
+    import os
+
+    from gospl_amd.model import Model
+
+    m, h0, p = case("sphere", 7)
+    os.environ["GOSPL_CHAIN_SPIN_MAX"] = "0"
+    try:
+        g = Model(m, h0, tend=1e12, **p)
+    finally:
+        del os.environ["GOSPL_CHAIN_SPIN_MAX"]
+    with pytest.raises(RuntimeError, match="dataflow sweep aborted"):
+        g.initialise()
+    g.dm.close()
+    ref = []
+    for chain in ("1", "0"):
+        os.environ["GOSPL_DAG_CHAIN"] = chain
+        try:
+            g = Model(m, h0, tend=1e12, **p)
+        finally:
+            del os.environ["GOSPL_DAG_CHAIN"]
+        g.initialise()
+        g.step()
+        ref.append([t.cpu().numpy().copy() for t in (g.hLocal, g.FAL)])
+        g.dm.close()
+    for a, b in zip(*ref):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("flowdir", [1, 6])
 def test_other_receiver_counts(flowdir):
     """Single- and six-receiver routing (the latter on the general row

@@ -97,6 +97,22 @@ __global__ void k_finalize_b(int64_t N, const double* __restrict__ hl, const dou
   fillfal[i] = g < 0.0 ? 0.0 : g;
 }
 
+// Counts in *asym the neighbour relations that are not mutual.
+__global__ void k_nbr_symmetry(int64_t N, int W, const int32_t* __restrict__ nbr,
+                               const int32_t* __restrict__ deg, int32_t* __restrict__ asym) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int dg = deg[i];
+  for (int q = 0; q < W && q < dg; ++q) {
+    const int j = nbr[(int64_t)q * N + i];
+    if (j < 0) continue;
+    bool found = false;
+    const int dj = deg[j];
+    for (int p = 0; p < W && p < dj; ++p) found = found || nbr[(int64_t)p * N + j] == (int)i;
+    if (!found) atomicAdd(asym, 1);
+  }
+}
+
 }  // namespace gospl
 
 using namespace gospl;
@@ -111,6 +127,24 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
   if (!out || N <= 0 || W <= 0 || W > GOSPL_MAXW || N >= (int64_t)INT32_MAX - 2) {
     set_error("gospl_mesh_create: bad arguments (N=%lld, W=%d)", (long long)N, (int)W);
     return 1;
+  }
+  // The receiver-DAG sweeps find a donor's row entry through the receiver's
+  // neighbour list, so every neighbour relation must be mutual (as in a
+  // triangulation whose vertices have at most GOSPL_MAXW neighbours).
+  {
+    int32_t* asym = nullptr;
+    GOSPL_CHECK(hipMalloc(&asym, sizeof(int32_t)));
+    GOSPL_CHECK(hipMemset(asym, 0, sizeof(int32_t)));
+    k_nbr_symmetry<<<nblocks(N), GOSPL_BLOCK>>>(N, W, nbr, deg, asym);
+    GOSPL_CHECK(hipGetLastError());
+    int32_t h = 0;
+    GOSPL_CHECK(hipMemcpy(&h, asym, sizeof(int32_t), hipMemcpyDeviceToHost));
+    GOSPL_CHECK(hipFree(asym));
+    if (h) {
+      set_error("gospl_mesh_create: %d neighbour relations are not mutual (a vertex with more than "
+                "%d neighbours?)", (int)h, GOSPL_MAXW);
+      return 1;
+    }
   }
   gospl_mesh* m = new gospl_mesh();
   std::memset(m, 0, sizeof(*m));

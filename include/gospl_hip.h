@@ -49,6 +49,8 @@ int gospl_fv_geometry(const double* coords, int64_t nb, const int32_t* cells_nod
  * (functions.F90:10-36) that `definetin` fills.  The geometry arrays stay
  * owned by the caller and must outlive the handle; the handle owns O(N)
  * device workspace.  nbr/elgt/vdist are (W, N) column-major ELL tables.
+ * Neighbour relations must be mutual (they are for a triangulation whose
+ * vertices have at most 12 neighbours); the call fails otherwise.
  */
 int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr,
                       const int32_t* deg, const double* elgt, const double* vdist,

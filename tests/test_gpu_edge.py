@@ -234,6 +234,33 @@ def test_dataflow_bounded_wait_aborts_cleanly():
         assert np.array_equal(a, b)
 
 
+def test_mesh_with_one_sided_neighbour_is_refused():
+    """The sweeps rely on mutual neighbour lists; a handle on lists where one
+    relation is one-sided is refused with an error instead of giving wrong
+    solves."""
+    import ctypes
+
+    import torch
+
+    from gospl_amd import _lib
+    from gospl_amd.model import _ptr
+
+    m, h0, p = case("planar", 12)
+    nbr = m.nbr.copy()
+    deg = m.deg.copy()
+    i = int(np.argmax(deg))
+    nbr[deg[i] - 1, i] = -1  # i forgets its last neighbour; that one still lists i
+    deg[i] -= 1
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+    args = [t(nbr, torch.int32), t(deg, torch.int32), t(m.elgt, torch.float64), t(m.vdist, torch.float64),
+            t(m.area, torch.float64), t(m.gid, torch.int32), t(m.outlet, torch.int8)]
+    h = ctypes.c_void_p()
+    lib = _lib.load()
+    rc = lib.gospl_mesh_create(m.N, nbr.shape[0], 1, *[_ptr(a) for a in args], ctypes.byref(h))
+    assert rc != 0
+    assert b"not mutual" in lib.gospl_last_error()
+
+
 @pytest.mark.parametrize("flowdir", [1, 6])
 def test_other_receiver_counts(flowdir):
     """Single- and six-receiver routing (the latter on the general row

@@ -973,20 +973,23 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
 #pragma unroll
         for (int c = 0; c < REC_NACC; ++c) e[c] = ld_agent(rv + REC_ACC + c);  // with the rest of the line
         const int nd = lo32(q0.x);
-        bool ready = true;
         double s = q0.y;
         // branch-free over the record's entries (selects, not branches)
 #pragma unroll
         for (int c = 0; c < REC_NACC; ++c) {
-          const bool in = c < nd;
-          ready = ready && (!in || !is_empty(e[c]));
           const double t = s + e[c];
-          s = in ? t : s;
+          s = c < nd ? t : s;
         }
-        for (int c = REC_NACC; c < nd; ++c) {
-          const double o = ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
-          ready = ready && !is_empty(o);
-          s = s + o;
+        for (int c = REC_NACC; c < nd; ++c) s = s + ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v);
+        // an empty entry makes the sum NaN; only then look at the entries
+        // (a NaN can also have arrived with the data)
+        bool ready = !isnan(s);
+        if (!ready) {
+          ready = true;
+#pragma unroll
+          for (int c = 0; c < REC_NACC; ++c) ready = ready && (c >= nd || !is_empty(e[c]));
+          for (int c = REC_NACC; c < nd; ++c)
+            ready = ready && !is_empty(ld_agent(a.oacc + (int64_t)(c - REC_NACC) * N + v));
         }
         if (ready) {
           solved = true;
@@ -1032,20 +1035,24 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         const double q4y = rv[9];
         const double xr[3] = {ld_agent(rv + REC_XR), ld_agent(rv + REC_XR + 1), ld_agent(rv + REC_XR + 2)};
         const int mask = hi32(q0.x);
-        bool ready = true;
+        double s = q0.y;
+        const double dg = q1.x;
+        const double o[3] = {q1.y, q2.x, q2.y};
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          if ((mask >> k) & 1) ready = ready && !is_empty(xr[k]);
+          if ((mask >> k) & 1) s = s - o[k] * xr[k];
+        const double xs = s / dg;
+        // an empty receiver value makes the result NaN; only then look at them
+        bool ready = !isnan(xs);
+        if (!ready) {
+          ready = true;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) ready = ready && (((mask >> k) & 1) == 0 || !is_empty(xr[k]));
+        }
         if (ready) {
           solved = true;
           nd = lo32(q0.x);
-          double s = q0.y;
-          const double dg = q1.x;
-          const double o[3] = {q1.y, q2.x, q2.y};
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-            if ((mask >> k) & 1) s = s - o[k] * xr[k];
-          xv = s / dg;
+          xv = xs;
           a.x[v] = xv;
           if (trace) {
             trace[v] = (long long)wall_clock64();

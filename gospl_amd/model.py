@@ -135,10 +135,14 @@ class Model:
         self.ptdir = torch.full((N,), -1, **i32)
         self.lspill = torch.zeros(N, **i8)
         K_ = self.flowDir
-        self.rcvID = torch.zeros((K_, N), **i32)
-        self.distRcv = torch.zeros((K_, N), **f64)
-        self.wghtVal = torch.zeros((K_, N), **f64)
-        self.lsink = torch.zeros(N, **i8)
+        # two receiver sets: the step's main receivers (goSPL's rcvIDi/wghtVali/
+        # distRcvi/lsinki snapshots, written only by the first direction build)
+        # and the cascade's rebuilds; rcvID/distRcv/wghtVal/lsink name the set
+        # built last, as goSPL's live arrays do
+        self._rcv_sets = [(torch.zeros((K_, N), **i32), torch.zeros((K_, N), **f64),
+                           torch.zeros((K_, N), **f64), torch.zeros(N, **i8)) for _ in range(2)]
+        self.rcvIDi, self.distRcvi, self.wghtVali, self.lsinki = self._rcv_sets[0]
+        self.rcvID, self.distRcv, self.wghtVal, self.lsink = self._rcv_sets[0]
         self._b = torch.zeros(N, **f64)
         self._q = torch.zeros(N, **f64)
         self._nfa = torch.zeros(N, **f64)
@@ -253,6 +257,9 @@ class Model:
 
     # -------------------------------------------------------------- flow
     def _buildFlowDirection(self, h, down=True):
+        """flowplex.py:480-544.  The step's first build (down=False) fills the
+        main set, later builds (the cascade's) the other one."""
+        self.rcvID, self.distRcv, self.wghtVal, self.lsink = self._rcv_sets[1 if down else 0]
         self._call("gospl_flow_direction", self.H, self.flowDir, self.flowExp, _ptr(h), self.sealevel,
                    _ptr(self.lFill), 1 if down else 0, _ptr(self.pitIDs), _ptr(self.flatDirs),
                    _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
@@ -337,11 +344,7 @@ class Model:
         self.fillElevation(sed=False)
         pitVol = self.pitParams[:, 0].copy()
         hl = self.hLocal.clone()
-        self._buildFlowDirection(hl, False)
-        self.wghtVali = self.wghtVal.clone()
-        self.rcvIDi = self.rcvID.clone()
-        self.distRcvi = self.distRcv.clone()
-        self.lsinki = self.lsink.clone()
+        self._buildFlowDirection(hl, False)  # the main set: rcvIDi/wghtVali/distRcvi/lsinki
         S = self.stream
         self._call("gospl_rain_source", self.H, _ptr(self.rainVal), _ptr(self.lFill), self.sealevel,
                    _ptr(self._b), S)

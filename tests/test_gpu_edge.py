@@ -145,6 +145,55 @@ def test_flow_receiver_cycle(drain):
         assert bado > 0 and bad.value == bado
 
 
+@pytest.mark.parametrize("chain", ["1", "0"])
+def test_single_long_chain_closed_form(chain):
+    """Flow accumulation along one 40,000-vertex path (a snake over the
+    200x200 planar grid, unit weights, unit sources): the discharge at the
+    k-th vertex of the path is exactly k, for the dataflow and the level
+    sweeps (the deepest dependency chain the sweeps can meet)."""
+    import ctypes
+    import os
+
+    import torch
+
+    from gospl_amd import _lib, workloads
+    from gospl_amd.model import DeviceMesh, _ptr
+
+    n = 200
+    lib = _lib.load()
+    m, h0, p = workloads.planar_case(lib, n=n)
+    N = m.N
+    order = np.concatenate([np.arange(i * n, (i + 1) * n)[:: 1 if i % 2 == 0 else -1] for i in range(n)])
+    nxt = np.empty(N, np.int64)
+    nxt[order[:-1]] = order[1:]
+    nxt[order[-1]] = order[-1]
+    rcv = m.iperm[nxt[m.perm]].astype(np.int32)  # device order
+    linked = np.zeros(N, bool)
+    for k in range(m.nbr.shape[0]):
+        linked |= m.nbr[k] == rcv
+    assert linked.sum() == N - 1  # every link is a mesh edge (the last vertex is its own receiver)
+    os.environ["GOSPL_DAG_CHAIN"] = chain
+    try:
+        dm = DeviceMesh(m, "cuda")
+    finally:
+        del os.environ["GOSPL_DAG_CHAIN"]
+    R = torch.as_tensor(rcv[None, :].copy(), device="cuda")
+    Wt = torch.ones((1, N), dtype=torch.float64, device="cuda")
+    B = torch.ones(N, dtype=torch.float64, device="cuda")
+    Q = torch.zeros_like(B)
+    lev = ctypes.c_int32(0)
+    bad = ctypes.c_int64(0)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib.gospl_flowacc(dm.handle, 1, _ptr(R), _ptr(Wt), _ptr(B), _ptr(Q), ctypes.byref(lev),
+                             ctypes.byref(bad), s) == 0
+    q = m.to_input_order(Q.cpu().numpy())
+    assert bad.value == 0
+    assert np.array_equal(q[order], np.arange(1, N + 1, dtype=np.float64))
+    if chain == "0":
+        assert abs(lev.value - N) <= 1  # the depth of the path
+    dm.close()
+
+
 def test_csr_rows_match_packed_records():
     """The general dependency-row layout (any receiver count) and the packed
     per-vertex records (up to three receivers) give the same bits."""

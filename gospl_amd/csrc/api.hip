@@ -57,17 +57,19 @@ __global__ void k_cascade_add(int64_t N, const double* __restrict__ q, const dou
   fal[i] = fal[i] + f / dt;
 }
 
-// Discharge at filled pits and its maximum (flowplex.py:906-918).
+// Discharge at filled pits and its maximum (flowplex.py:906-918).  A fixed
+// grid strides over the mesh, so the maximum takes one atomic per workgroup
+// of that grid (same-address atomics serialise).
+#define FINALIZE_BLOCKS 512
 __global__ void k_finalize_a(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
                              const int32_t* __restrict__ pit, int has_pits, double* __restrict__ fal,
                              unsigned long long* __restrict__ maxkey) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v = -INFINITY;
-  if (i < N) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     double f = fal[i];
     if (has_pits && hl[i] <= water[i] && pit[i] > -1) f = 0.0;
     fal[i] = f;
-    v = f;
+    v = v > f ? v : f;
   }
   // wave-level max, block-level max, then one atomic per block
   for (int o = 32; o > 0; o >>= 1) {
@@ -249,7 +251,7 @@ int gospl_flow_finalize(gospl_mesh_t* m, const double* hl, const double* water,
                         gospl_stream_t stream) {
   unsigned long long* key = m->wu[1];
   GOSPL_CHECK(hipMemsetAsync(key, 0, sizeof(unsigned long long), stream));
-  k_finalize_a<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, fal, key);
+  k_finalize_a<<<FINALIZE_BLOCKS, GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, fal, key);
   k_finalize_b<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, key, fal,
                                                            fillfal);
   GOSPL_CHECK(hipGetLastError());

@@ -102,59 +102,28 @@ __global__ void k_jump(int64_t N, int32_t* __restrict__ root, int32_t* __restric
   }
 }
 
-__global__ void k_fill_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ deg, const double* __restrict__ dem,
-                             const int32_t* __restrict__ state, const int32_t* __restrict__ root,
-                             unsigned long long* __restrict__ S, int32_t* __restrict__ flag,
-                             int pass) {
-  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Cells that touch another basin (the only ones the relaxation reads
+// from), flagged one cell per thread; the list is then selected by flag.
+__global__ void k_basin_boundary(int64_t N, int W, const int32_t* __restrict__ nbr,
+                                 const int32_t* __restrict__ deg, const int32_t* __restrict__ state,
+                                 const int32_t* __restrict__ root, int8_t* __restrict__ flag) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= N) return;
-  if (state[u] < 0) return;
-  const int ru = root[u];
-  const double zu = dem[u];
-  unsigned long long cur = S[ru];
-  if (cur == KEY_NEG_INF) return;
-  unsigned long long best = cur;
-  const int dg = deg[u];
-  for (int p = 0; p < W && p < dg; ++p) {
-    int v = nbr[(int64_t)p * N + u];
-    if (v < 0 || state[v] < 0) continue;
-    int rv = root[v];
-    if (rv == ru) continue;
-    unsigned long long sv = S[rv];
-    if (sv == KEY_POS_INF) continue;
-    double w = dem[v] > zu ? dem[v] : zu;
-    unsigned long long kw = dkey(w);
-    unsigned long long c = kw > sv ? kw : sv;
-    if (c < best) best = c;
-  }
-  if (best < cur) {
-    unsigned long long old = atomicMin(&S[ru], best);
-    if (best < old) *flag = pass;
-  }
-}
-
-// Cells that touch another basin: the only ones the relaxation reads from.
-struct IsBasinBoundary {
-  int64_t N;
-  int W;
-  const int32_t* nbr;
-  const int32_t* deg;
-  const int32_t* state;
-  const int32_t* root;
-  __device__ __forceinline__ bool operator()(const int32_t& u) const {
-    if (state[u] < 0) return false;
+  int8_t f = 0;
+  if (state[u] >= 0) {
     const int ru = root[u];
     const int dg = deg[u];
     for (int p = 0; p < W && p < dg; ++p) {
       const int v = nbr[(int64_t)p * N + u];
-      if (v >= 0 && state[v] >= 0 && root[v] != ru) return true;
+      if (v >= 0 && state[v] >= 0 && root[v] != ru) f = 1;
     }
-    return false;
   }
-};
+  flag[u] = f;
+}
 
-// k_fill_relax over the list of basin-boundary cells.
+// Step 3 over the list of basin-boundary cells: S[ru] = min(S[ru],
+// max(key(max(z_u, z_v)), S[rv])) over included neighbours v in another
+// basin; a pass that lowers some S writes its number into *flag.
 __global__ void k_fill_relax_list(int64_t N, int W, const int32_t* __restrict__ nbr,
                                   const int32_t* __restrict__ deg, const double* __restrict__ dem,
                                   const int32_t* __restrict__ state, const int32_t* __restrict__ root,
@@ -513,10 +482,12 @@ int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, doubl
   int32_t* blist = m->wi[5];
   int32_t* bcount = m->dflag + 44;
   {
+    int8_t* bflag = reinterpret_cast<int8_t*>(m->wi[6]);
+    k_basin_boundary<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, state, root, bflag);
+    GOSPL_CHECK(hipGetLastError());
     size_t bytes = m->cub_tmp_bytes;
     hipcub::CountingInputIterator<int32_t> cit(0);
-    GOSPL_CHECK(hipcub::DeviceSelect::If(m->cub_tmp, bytes, cit, blist, bcount, (int)N,
-                                         IsBasinBoundary{N, m->W, m->nbr, m->deg, state, root}, s));
+    GOSPL_CHECK(hipcub::DeviceSelect::Flagged(m->cub_tmp, bytes, cit, bflag, blist, bcount, (int)N, s));
   }
   // grid-stride over the list, whose length stays on the device
   const int gb = (int)std::min<int64_t>(g, 2048);

@@ -177,7 +177,11 @@ int pit_tmp_bytes(int64_t N, size_t* bytes) {
                                        IsPitCell{nullptr}));
   GOSPL_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, (unsigned long long*)nullptr,
                                                 (unsigned long long*)nullptr, (int)N, 32, 64));
+  size_t b3 = 0;
+  GOSPL_CHECK(hipcub::DeviceSelect::Flagged(nullptr, b3, it, (const int8_t*)nullptr, (int32_t*)nullptr,
+                                            (int32_t*)nullptr, (int)N));
   *bytes = b1 > b2 ? b1 : b2;
+  if (b3 > *bytes) *bytes = b3;
   return 0;
 }
 

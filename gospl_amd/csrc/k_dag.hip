@@ -371,6 +371,13 @@ __global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
 }
 
+// Small int32 arrays cleared by the first workgroup of a launch.
+struct ZeroList {
+  int32_t* p[4];
+  int len[4];
+  int n;
+};
+
 // Row masks: receiver r's row is the bitmask over (neighbour position p,
 // receiver slot k), bit p * NR + k, of the donors that send to r through
 // their slot k; bit order is row order (FV neighbour order, then slot).  Set
@@ -378,10 +385,15 @@ __global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
 // before).
 // The position of i in the list of its receiver k is kept in rpos[k * N + i]
 // (-1: no dependency) for the record builds.
+// The first workgroup also clears the sweep's small counters: zero[0..nz).
 __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                            const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
                            const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-                           unsigned long long* __restrict__ rowmask, int8_t* __restrict__ rpos) {
+                           unsigned long long* __restrict__ rowmask, int8_t* __restrict__ rpos,
+                           ZeroList zl) {
+  if (blockIdx.x == 0)
+    for (int k = 0; k < zl.n; ++k)
+      for (int t = threadIdx.x; t < zl.len[k]; t += blockDim.x) zl.p[k][t] = 0;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   // the neighbour list once, searched for every receiver
@@ -1308,11 +1320,25 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* dev = m->dflag;
   // packed records: up to 3 receivers, int32 word indices, row masks of W * nrcv <= 64 bits
   const bool rec = nrcv <= 3 && !m->dag_csr && N * REC_W < (int64_t)INT32_MAX && m->W * nrcv <= 64;
-  // per-level frontier counts (the dataflow sweep uses counts[1] only)
-  GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (rec && m->dag_chain ? 2 : N + 2), s));
-  GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
   unsigned long long* bar = reinterpret_cast<unsigned long long*>(dev + 32);
-  GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned long long) * 3, s));
+  const bool chain = rec && m->dag_chain;
+  // dataflow sweep: global batch queue (entries at -1 between sweeps: each
+  // taken entry is reset) and control words
+  const int64_t gq_cap = N / 64 + 1;
+  if (chain && !m->chain_ctl) {
+    GOSPL_CHECK(hipMalloc(&m->chain_ctl, sizeof(int32_t) * CQ_WORDS));
+    GOSPL_CHECK(hipMalloc(&m->chain_gq, sizeof(int32_t) * gq_cap * 64));
+    GOSPL_CHECK(hipMemsetAsync(m->chain_gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
+  }
+  // the small counters (status words, barrier words, dataflow control words,
+  // the dataflow's frontier count) are cleared by the record-mask kernel;
+  // per-level frontier counts of the level sweeps by memset
+  ZeroList zl{{dev, dev + 32, counts, m->chain_ctl}, {4, 6, 2, CQ_WORDS}, chain ? 4 : 3};
+  if (!chain) GOSPL_CHECK(hipMemsetAsync(counts, 0, sizeof(int32_t) * (N + 2), s));
+  if (!rec) {
+    GOSPL_CHECK(hipMemsetAsync(dev, 0, sizeof(int32_t) * 4, s));
+    GOSPL_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned long long) * 3, s));
+  }
   int32_t* slot = nullptr;
   if (rec) {
     // packed records; rows hold at most nrcv entries per neighbour, the ones
@@ -1322,7 +1348,8 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     unsigned long long* rowmask = m->wu[0];
     GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
     int8_t* rpos = reinterpret_cast<int8_t*>(m->wi[5]);  // 3 N bytes
-    k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos);
+    k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos,
+                                         zl);
     if (mode == 0)
       k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc, pend);
     else
@@ -1361,18 +1388,6 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   void* args[] = {&a, &fa, &fb, &counts, &bar, &nlev, &trace, &m->dag_interleave, &m->dag_tail};
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
-  const bool chain = rec && m->dag_chain;
-  // dataflow sweep: global batch queue (entries at -1 between sweeps: each
-  // taken entry is reset) and control words
-  const int64_t gq_cap = N / 64 + 1;
-  if (chain) {
-    if (!m->chain_ctl) {
-      GOSPL_CHECK(hipMalloc(&m->chain_ctl, sizeof(int32_t) * CQ_WORDS));
-      GOSPL_CHECK(hipMalloc(&m->chain_gq, sizeof(int32_t) * gq_cap * 64));
-      GOSPL_CHECK(hipMemsetAsync(m->chain_gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
-    }
-    GOSPL_CHECK(hipMemsetAsync(m->chain_ctl, 0, sizeof(int32_t) * CQ_WORDS, s));
-  }
   int32_t* gq = m->chain_gq;
   if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   if (chain) {

@@ -416,6 +416,23 @@ __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__
   }
 }
 
+// Records of a workgroup staged in LDS (9 double2 per record: the padding
+// keeps the lanes' 16-byte writes on different banks) and written out as
+// one contiguous run, so each store instruction covers whole lines.
+#define REC_STAGE 9
+__device__ __forceinline__ void rec_store_staged(double2 (*stage)[REC_STAGE], const double2 (&out)[8], bool valid,
+                                                 double* __restrict__ rec, int64_t N) {
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) stage[threadIdx.x][k] = out[k];
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * GOSPL_BLOCK;
+  const int nrec = (int)(N - base < GOSPL_BLOCK ? N - base : GOSPL_BLOCK);
+  double2* dst = reinterpret_cast<double2*>(rec + base * REC_W);
+  for (int t = threadIdx.x; t < nrec * 8; t += GOSPL_BLOCK) dst[t] = stage[t >> 3][t & 7];
+}
+
 // An empty row entry: a signalling NaN, which no arithmetic produces, so a
 // reader can tell an entry its donor has not written yet.
 #define REC_EMPTY_BITS 0x7ff4000000000001ll
@@ -430,8 +447,12 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
                              const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
                              double* __restrict__ oacc, int32_t* __restrict__ pend) {
+  __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
+  const bool valid = i < N;
+  double2 out[8];
+  if (valid) {
   int r[3], sl[3];
   double w[3];
   for (int k = 0; k < 3; ++k) {
@@ -451,7 +472,6 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
   const int nd = __popcll(rowmask[i]);
   pend[i] = nd;
   const double e = rec_empty();
-  double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
   out[0] = make_double2(pack32(nd, 0), rhs[i]);
   out[1] = make_double2(w[0], w[1]);
   out[2] = make_double2(w[2], pack32(r[0], r[1]));
@@ -461,6 +481,8 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
   out[6] = make_double2(e, e);
   out[7] = make_double2(e, e);
   for (int c = REC_NACC; c < nd; ++c) oacc[(int64_t)(c - REC_NACC) * N + i] = e;
+  }
+  rec_store_staged(stage, out, valid, rec, N);
 }
 
 // Whole stream-power record of vertex i, written at once: its dependency
@@ -473,8 +495,12 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
                              const double* __restrict__ diag, const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
                              int32_t* __restrict__ odon, int32_t* __restrict__ pend) {
+  __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
+  const bool valid = i < N;
+  double2 out[8];
+  if (valid) {
   int mask = 0;
   double o[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < NR && k < 3; ++k) {
@@ -502,7 +528,6 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
   }
   pend[i] = __popc(mask);
   const double e = rec_empty();
-  double2* out = reinterpret_cast<double2*>(rec + i * REC_W);
   out[0] = make_double2(pack32(nd, mask), rhs[i]);
   out[1] = make_double2(diag[i], o[0]);
   out[2] = make_double2(o[1], o[2]);
@@ -511,6 +536,8 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
   out[5] = make_double2(pack32(t[2], t[3]), pack32(t[4], t[5]));
   out[6] = make_double2(pack32(t[6], t[7]), pack32(t[8], t[9]));
   out[7] = make_double2(pack32(t[10], t[11]), pack32(t[12], t[13]));
+  }
+  rec_store_staged(stage, out, valid, rec, N);
 }
 
 template <int MODE, class Append, bool WG>

@@ -352,31 +352,44 @@ __global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* 
 
 // Breadth-first distance from the spill cells over pit cells along
 // not-descending edges (fill_dir + nghb_dir, functions.F90:1845-1969).
-__global__ void k_ptdir_relax(int64_t N, int W, const int32_t* __restrict__ nbr,
-                              const int32_t* __restrict__ deg, const double* __restrict__ fill,
-                              const int32_t* __restrict__ ncells, const int32_t* __restrict__ cells,
-                              int32_t* ptdir, int32_t* __restrict__ flag, int pass) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= *ncells) return;
-  const int c = cells[t];
-  const int cur = ptdir[c];
-  if (cur == 1) return;
-  const double fc = fill[c];
-  int best = cur;
-  const int dg = deg[c];
-  for (int p = 0; p < W && p < dg; ++p) {
-    int i = nbr[(int64_t)p * N + c];
-    if (i < 0) continue;
-    int di = ptdir[i];
-    if (di > -1 && fc >= fill[i]) {
-      int cand = di + 1;
-      if (best < 0 || cand < best) best = cand;
+#define PTDIR_REPEAT 4
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_ptdir_relax(
+    int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+    const double* __restrict__ fill, const int32_t* __restrict__ ncells, const int32_t* __restrict__ cells,
+    int32_t* ptdir, int32_t* __restrict__ flag, int pass) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *ncells;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // whole workgroup past the list
+  const bool own = t < n;
+  const int c = own ? cells[t] : 0;
+  const double fc = own ? fill[c] : 0.0;
+  const int dg = own ? deg[c] : 0;
+  // a few rounds per launch, the workgroup's cells (consecutive pit cells, a
+  // compact patch) seeing each other's updates between rounds
+  bool changed = false;
+  for (int round = 0; round < PTDIR_REPEAT; ++round) {
+    if (own) {
+      const int cur = ptdir[c];
+      if (cur != 1) {
+        int best = cur;
+        for (int p = 0; p < W && p < dg; ++p) {
+          const int i = nbr[(int64_t)p * N + c];
+          if (i < 0) continue;
+          const int di = ptdir[i];
+          if (di > -1 && fc >= fill[i]) {
+            const int cand = di + 1;
+            if (best < 0 || cand < best) best = cand;
+          }
+        }
+        if (best != cur) {
+          ptdir[c] = best;
+          changed = true;
+        }
+      }
     }
+    __syncthreads();
   }
-  if (best != cur) {
-    ptdir[c] = best;
-    *flag = pass;
-  }
+  if (changed) *flag = pass;
 }
 
 // Flat receivers (pitfilling.py:479-517); spill cells and non-pit cells -1.

@@ -124,38 +124,51 @@ __global__ void k_basin_boundary(int64_t N, int W, const int32_t* __restrict__ n
 // Step 3 over the list of basin-boundary cells: S[ru] = min(S[ru],
 // max(key(max(z_u, z_v)), S[rv])) over included neighbours v in another
 // basin; a pass that lowers some S writes its number into *flag.
-__global__ void k_fill_relax_list(int64_t N, int W, const int32_t* __restrict__ nbr,
-                                  const int32_t* __restrict__ deg, const double* __restrict__ dem,
-                                  const int32_t* __restrict__ state, const int32_t* __restrict__ root,
-                                  unsigned long long* __restrict__ S, const int32_t* __restrict__ list,
-                                  const int32_t* __restrict__ count, int32_t* __restrict__ flag,
-                                  int pass) {
+#define FILL_REPEAT 4
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_fill_relax_list(
+    int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+    const double* __restrict__ dem, const int32_t* __restrict__ state, const int32_t* __restrict__ root,
+    unsigned long long* __restrict__ S, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+    int32_t* __restrict__ flag, int pass) {
   const int n = *count;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-  const int u = list[t];
-  const int ru = root[u];
-  const double zu = dem[u];
-  const unsigned long long cur = S[ru];
-  if (cur == KEY_NEG_INF) continue;
-  unsigned long long best = cur;
-  const int dg = deg[u];
-  for (int p = 0; p < W && p < dg; ++p) {
-    int v = nbr[(int64_t)p * N + u];
-    if (v < 0 || state[v] < 0) continue;
-    int rv = root[v];
-    if (rv == ru) continue;
-    unsigned long long sv = S[rv];
-    if (sv == KEY_POS_INF) continue;
-    double w = dem[v] > zu ? dem[v] : zu;
-    unsigned long long kw = dkey(w);
-    unsigned long long c = kw > sv ? kw : sv;
-    if (c < best) best = c;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // whole workgroup past the list
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool own = t < n;
+  const int u = own ? list[t] : 0;
+  const int ru = own ? root[u] : 0;
+  const double zu = own ? dem[u] : 0.0;
+  const int dg = own ? deg[u] : 0;
+  // a few rounds per launch, the workgroup's cells (consecutive boundary
+  // cells, a compact patch) seeing each other's lowered levels between rounds
+  bool changed = false;
+  for (int round = 0; round < FILL_REPEAT; ++round) {
+    if (own) {
+      const unsigned long long cur =
+          __hip_atomic_load(&S[ru], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur != KEY_NEG_INF) {
+        unsigned long long best = cur;
+        for (int p = 0; p < W && p < dg; ++p) {
+          const int v = nbr[(int64_t)p * N + u];
+          if (v < 0 || state[v] < 0) continue;
+          const int rv = root[v];
+          if (rv == ru) continue;
+          const unsigned long long sv =
+              __hip_atomic_load(&S[rv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (sv == KEY_POS_INF) continue;
+          const double w = dem[v] > zu ? dem[v] : zu;
+          const unsigned long long kw = dkey(w);
+          const unsigned long long c = kw > sv ? kw : sv;
+          if (c < best) best = c;
+        }
+        if (best < cur) {
+          const unsigned long long old = atomicMin(&S[ru], best);
+          if (best < old) changed = true;
+        }
+      }
+    }
+    __syncthreads();
   }
-  if (best < cur) {
-    unsigned long long old = atomicMin(&S[ru], best);
-    if (best < old) *flag = pass;
-  }
-  }
+  if (changed) *flag = pass;
 }
 
 __global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
@@ -502,8 +515,8 @@ int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, doubl
     hipcub::CountingInputIterator<int32_t> cit(0);
     GOSPL_CHECK(hipcub::DeviceSelect::Flagged(m->cub_tmp, bytes, cit, bflag, blist, bcount, (int)N, s));
   }
-  // grid-stride over the list, whose length stays on the device
-  const int gb = (int)std::min<int64_t>(g, 2048);
+  // one list entry per thread; the length stays on the device (workgroups past it return)
+  const int gb = g;
   if (until_stable(m, s, (int)N + 16, 2, &m->iter_hint[1], [&](int32_t* f, int pass) {
         k_fill_relax_list<<<gb, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S, blist,
                                                       bcount, f, pass);

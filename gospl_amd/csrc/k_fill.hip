@@ -366,7 +366,8 @@ __global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* 
 // Breadth-first distance from the spill cells over pit cells along
 // not-descending edges (fill_dir + nghb_dir, functions.F90:1845-1969).
 #define PTDIR_REPEAT 4
-__global__ __launch_bounds__(GOSPL_BLOCK) void k_ptdir_relax(
+#define PTDIR_BLOCK 256
+__global__ __launch_bounds__(PTDIR_BLOCK) void k_ptdir_relax(
     int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
     const double* __restrict__ fill, const int32_t* __restrict__ ncells, const int32_t* __restrict__ cells,
     int32_t* ptdir, int32_t* __restrict__ flag, int pass) {
@@ -588,7 +589,7 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   if (npits > 0 && npc > 0) {
     int it = 0;
     if (until_stable(m, s, npc + 32, 16, &m->iter_hint[2], [&](int32_t* f, int pass) {
-          k_ptdir_relax<<<nblocks(npc), GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, ncells,
+          k_ptdir_relax<<<(int)((npc + PTDIR_BLOCK - 1) / PTDIR_BLOCK), PTDIR_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, ncells,
                                                              cells, ptdir, f, pass);
         }, &it))
       return 1;

@@ -266,16 +266,37 @@ __device__ __forceinline__ int uf_root(const int32_t* parent, int x) {
   return x;
 }
 
+// Same-address atomics serialise, and a lake is one component of many
+// cells: the lanes of a wavefront that share a root reduce their minimum
+// first, one atomic per root and wavefront.
 __global__ void k_cc_compress(int64_t N, int32_t* parent, const int8_t* __restrict__ noflow,
                               const int32_t* __restrict__ gid, const int8_t* __restrict__ outlet,
                               int flat, int32_t* __restrict__ cmin, int32_t* __restrict__ cnon) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  if (parent[i] < 0) return;
-  int r = uf_root(parent, (int)i);
-  parent[i] = r;
-  if (noflow[i]) atomicMin(&cmin[r], gid[i]);
-  if (!(flat && outlet[i])) cnon[r] = 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i - (i & 63) >= N) return;  // whole wavefront past the mesh
+  const bool in = i < N && parent[i] >= 0;
+  int r = -1;
+  if (in) {
+    r = uf_root(parent, (int)i);
+    parent[i] = r;
+    if (!(flat && outlet[i])) cnon[r] = 1;
+  }
+  const bool want = in && noflow[i];
+  const int g = want ? gid[i] : INT_MAX;
+  unsigned long long todo = __ballot(want);
+  const int lane = __lane_id();
+  while (todo != 0ull) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int rl = __shfl(r, leader);
+    const bool mine = want && r == rl;
+    int m = mine ? g : INT_MAX;
+    for (int o = 32; o > 0; o >>= 1) {
+      const int t = __shfl_xor(m, o);
+      m = t < m ? t : m;
+    }
+    if (lane == leader) atomicMin(&cmin[rl], m);
+    todo &= ~__ballot(mine);
+  }
 }
 
 __global__ void k_pit_mark(int64_t N, const int32_t* __restrict__ parent,

@@ -42,17 +42,39 @@ STEP_BYTES = 564  # sum of ALG_BYTES: compulsory bytes per vertex and step
 # the sweep kernels between the library's timing events, per entry point
 SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_chain<0>",
                  "gospl_spl": "k_dag_level0<1> + k_chain<1>"}
+# HBM traffic per sweep (level 0 + dataflow kernel) on this workload, from two
+# rocprofv3 --pmc passes over `bench.py --steps 2 --warmup 1 --no-cpu
+# --surface z9.npy` (KiB per dispatch, mean over the passes' solves), each
+# scaled by what the counter reads for a 256 MiB device copy
+# (`--copy-probe 256`): FETCH_SIZE reads 0.5x the bytes moved (scale 2),
+# WRITE_SIZE 1.0x.  Counter runs need --surface: the SH construction's torch
+# kernels crash the counter-collecting profiler.
+PMC_SCALE = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
+PMC_KIB = {
+    "gospl_flowacc": {"FETCH_SIZE": 213866.4 + 50847.2, "WRITE_SIZE": 15296.6 + 140025.0},
+    "gospl_spl": {"FETCH_SIZE": 196744.1 + 46317.3, "WRITE_SIZE": 17259.2 + 132100.2},
+}
+
+
+def pmc_traffic(entry, level):
+    """HBM bytes per launch of the entry point's sweep (level-9 sphere only)."""
+    if level != 9 or entry not in PMC_KIB:
+        return None
+    return sum(kib * PMC_SCALE[c] * 1024.0 for c, kib in PMC_KIB[entry].items())
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(level, seed, device):
+def build_workload(level, seed, device, surface=None, save_surface=None):
     from gospl_amd import _lib, workloads
 
     t0 = time.time()
-    m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device)
+    z = np.load(surface) if surface else None
+    m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device, z=z)
+    if save_surface:
+        np.save(save_surface, m.to_input_order(h0))
     log("[bench] mesh level %d: N=%d built in %.1fs" % (level, m.N, time.time() - t0))
     return m, h0, p
 
@@ -81,6 +103,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--surface", default=None, metavar="NPY",
+                    help="initial surface (input order) saved by --save-surface instead of the SH field "
+                         "(keeps the SH construction's kernels out of a counter run)")
+    ap.add_argument("--save-surface", default=None, metavar="NPY")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -105,7 +131,9 @@ def main():
     from gospl_amd.model import Model
 
     dev = torch.device("cuda:0")
-    m, h0, p = build_workload(args.level, args.seed, dev)
+    m, h0, p = build_workload(args.level, args.seed, dev, args.surface, args.save_surface)
+    if args.save_surface and args.steps == 0:
+        return
     g = Model(m, h0, tend=1e12, device=dev, **p)
     t0 = time.time()
     g.initialise()
@@ -160,6 +188,7 @@ def main():
     else:
         kname, kms, kcalls = dom, phases[dom]["ms_per_call"], phases[dom]["calls"]
     bytes_per_launch = ALG_BYTES[dom] * N
+    traffic = pmc_traffic(dom, args.level) if kern and kern[2] > 0 else None
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
     log("[bench] dominant kernel %s: %.3f ms per launch over %d launches" % (kname, kms, kcalls))
     log("[bench] phases (ms per step): " + ", ".join(
@@ -199,11 +228,12 @@ def main():
                                      "frac": STEP_BYTES * N * sps / 1e9 / PEAK_HBM_GBS}},
         "roofline": {"bound": "hbm", "kernel": kname, "entry_point": dom, "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "alg_bytes_per_vertex": ALG_BYTES[dom],
+                     "traffic": traffic, "alg_bytes_per_vertex": ALG_BYTES[dom],
                      "ms_per_launch": kms, "launches": kcalls,
                      "note": "latency-bound: ~600 dependent levels per solve (the longest flow "
-                             "path); traffic null: rocprofv3 --pmc crashes in the profiler on this "
-                             "program"},
+                             "path); traffic = HBM bytes per launch from rocprofv3 --pmc "
+                             "FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated), measured on this "
+                             "workload"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -38,10 +38,13 @@ def planar_case(lib, n: int = 200, dx: float = 500.0, seed: int = 0, order="mort
 
 
 def sphere_case(lib, level: int = 9, seed: int = 0, lmax: int = 64, slope: float = 2.0,
-                device=None, order="morton"):
+                device=None, order="morton", z=None):
+    """The icosahedral-sphere workload; ``z`` (input order) replaces the SH
+    surface when given (e.g. one saved by an earlier run)."""
     coords, cells = _mesh.icosphere(level, 6.371e6)
     m = _mesh.build_mesh(coords, cells, lib, flat=False, order=order)
-    z = fields.affine_range(fields.sh_field(coords, lmax, slope, seed=seed, device=device), -6000.0, 5000.0)
+    if z is None:
+        z = fields.affine_range(fields.sh_field(coords, lmax, slope, seed=seed, device=device), -6000.0, 5000.0)
     rain = fields.lognormal_rain(m.N, 1.0, 0.6, 0.2, 3.0, seed=seed + 1)
     params = dict(K=5.0e-6, m=0.5, d=0.0, flowdir=3, cda=0.5, cdm=0.5, dt=5000.0,
                   rain=m.to_device_order(rain), upsub=None, sealevel=0.0)

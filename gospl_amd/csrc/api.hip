@@ -45,6 +45,37 @@ __global__ void k_pit_water(int64_t N, const int32_t* __restrict__ pit,
   }
 }
 
+// One cascade pass's device update from a packed staging buffer
+// (flowplex.py:602-666): the pit water levels as k_pit_water, with
+// full[p] = stage[p] != 0 and level[p] = stage[npit + p], and the cleared
+// right-hand side.  The spill values are written by k_cascade_spill after it.
+__global__ void k_cascade_seed(int64_t N, const int32_t* __restrict__ pit, const double* __restrict__ stage,
+                               int64_t npit, const double* __restrict__ lfill, double* __restrict__ water,
+                               double* __restrict__ rhs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  rhs[i] = 0.0;
+  int32_t p = pit[i];
+  if (p < 0 || p >= npit) return;
+  if (stage[p] != 0.0) {
+    water[i] = lfill[i];
+  } else {
+    double l = stage[npit + p];
+    if (!isnan(l) && water[i] <= l) water[i] = l;
+  }
+}
+
+// rhs[v_j] = vol_j / dt at the spill vertices v_j (distinct), the volumes a
+// full pit passes on (flowplex.py:612-622, 667).
+__global__ void k_cascade_spill(int64_t N, const double* __restrict__ vid, const double* __restrict__ vol,
+                                int64_t nspill, double dt, double* __restrict__ rhs) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nspill) return;
+  const double v = vid[j];
+  if (!(v >= 0.0 && v < (double)N)) return;  // the host checks the ids; never write out of range
+  rhs[(int64_t)v] = vol[j] / dt;
+}
+
 // nFA = q*dt ; FAL += (nFA masked where hl < water)/dt (flowplex.py:674-680).
 __global__ void k_cascade_add(int64_t N, const double* __restrict__ q, const double* __restrict__ hl,
                               const double* __restrict__ water, double dt, double* __restrict__ nfa,
@@ -235,6 +266,21 @@ int gospl_rain_source(gospl_mesh_t* m, const double* rain, const double* lfill, 
 int gospl_pit_water(gospl_mesh_t* m, const int32_t* pitids, const int8_t* full, const double* level,
                     const double* lfill, double* water, gospl_stream_t stream) {
   k_pit_water<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, pitids, full, level, lfill, water);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_cascade_seed(gospl_mesh_t* m, const double* stage, int64_t npit, int64_t nspill,
+                       const int32_t* pitids, const double* lfill, double* water, double dt, double* rhs,
+                       gospl_stream_t stream) {
+  if (npit < 0 || nspill < 0 || nspill > npit) {
+    set_error("gospl_cascade_seed: bad sizes npit=%lld nspill=%lld", (long long)npit, (long long)nspill);
+    return 1;
+  }
+  k_cascade_seed<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, pitids, stage, npit, lfill, water, rhs);
+  if (nspill > 0)
+    k_cascade_spill<<<nblocks(nspill), GOSPL_BLOCK, 0, stream>>>(m->N, stage + 2 * npit,
+                                                                 stage + 2 * npit + nspill, nspill, dt, rhs);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

@@ -77,8 +77,10 @@ void set_error(const char* fmt, ...);
 // depression grouping (k_pits.hip)
 int pit_tmp_bytes(int64_t N, size_t* bytes);  // hipcub workspace of the grouping
 int pit_cells(gospl_mesh* m, const int32_t* pitids, int32_t* cells, int32_t* ncells, hipStream_t s);
+int ensure_groups(gospl_mesh* m, int64_t rows);  // grouping buffers for `rows` pits
+// `cleared`: the caller has already zeroed the segment table (2 * (pg_cap + 64) ints)
 int pit_group(gospl_mesh* m, const int32_t* pitids, int64_t npits, const int32_t* cells,
-              const int32_t* ncells_dev, int64_t ncells, hipStream_t s);
+              const int32_t* ncells_dev, int64_t ncells, hipStream_t s, bool cleared = false);
 int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double sl, int64_t cap,
                      double* pitparams, double* filled_lvl, double* filled_vol, hipStream_t s);
 }  // namespace gospl
@@ -94,6 +96,21 @@ inline int nblocks(int64_t n, int b = GOSPL_BLOCK) {
   int64_t g = (n + b - 1) / b;
   return (int)(g < 1 ? 1 : g);
 }
+
+// Several buffers set to a byte value in one launch (each hipMemsetAsync is a
+// launch of its own, and back-to-back small ones leave the GPU waiting on the
+// host).  Buffers are 4-byte aligned; blockIdx.y selects the buffer.
+struct FillList {
+  void* p[8];
+  int64_t nbytes[8];
+  uint8_t byte[8];
+  int n;
+};
+
+__global__ void k_fill_list(FillList f);
+
+// Enqueues the fills on `s` (no-op for an empty list).
+int fill_list(const FillList& f, hipStream_t s);
 
 // Total order on doubles as unsigned 64-bit keys (no NaNs expected).
 __device__ __forceinline__ unsigned long long dkey(double x) {

@@ -593,12 +593,18 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
               (long long)cap);
     return 1;
   }
+  // one launch clears the grouping's segment table and the spill search's
+  // buffers
+  if (ensure_groups(m, npits + 1)) return 1;
+  FillList fl{{m->pg_seg, best, spill, lspill},
+              {(int64_t)sizeof(int32_t) * 2 * (m->pg_cap + 64), (int64_t)sizeof(int32_t) * np1,
+               (int64_t)sizeof(int32_t) * np1, N},
+              {0, 0x7f, 0xff, 0},
+              4};
+  if (fill_list(fl, s)) return 1;
   // cells grouped by pit for the per-pit reductions
-  if (pit_group(m, pitids, npits, cells, ncells, npc, s)) return 1;
+  if (pit_group(m, pitids, npits, cells, ncells, npc, s, true)) return 1;
   // spill points
-  GOSPL_CHECK(hipMemsetAsync(best, 0x7f, sizeof(int32_t) * np1, s));
-  GOSPL_CHECK(hipMemsetAsync(spill, 0xff, sizeof(int32_t) * np1, s));
-  GOSPL_CHECK(hipMemsetAsync(lspill, 0, N, s));
   k_spill_best<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, m->outlet, m->flat,
                                           m->gid, best, qual);
   k_spill_set<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, pitids, m->outlet, m->flat,

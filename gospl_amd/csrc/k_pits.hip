@@ -193,7 +193,40 @@ int pit_cells(gospl_mesh* m, const int32_t* pitids, int32_t* cells, int32_t* nce
   return 0;
 }
 
-static int ensure_groups(gospl_mesh* m, int64_t rows) {
+__global__ void k_fill_list(FillList f) {
+  const int e = blockIdx.y;
+  unsigned char* p = static_cast<unsigned char*>(f.p[e]);
+  const int64_t nb = f.nbytes[e];
+  const uint32_t w = 0x01010101u * f.byte[e];
+  const int64_t nw = nb >> 2;
+  uint32_t* pw = reinterpret_cast<uint32_t*>(p);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
+    pw[i] = w;
+  if (blockIdx.x == 0 && (int64_t)threadIdx.x < (nb & 3)) p[(nw << 2) + threadIdx.x] = f.byte[e];
+}
+
+int fill_list(const FillList& f, hipStream_t s) {
+  if (f.n <= 0) return 0;
+  if (f.n > 8) {
+    set_error("fill_list: %d buffers (at most 8)", f.n);
+    return 1;
+  }
+  int64_t most = 0;
+  for (int k = 0; k < f.n; ++k) {
+    if (reinterpret_cast<uintptr_t>(f.p[k]) & 3) {
+      set_error("fill_list: buffer %d is not 4-byte aligned", k);
+      return 1;
+    }
+    if (f.nbytes[k] > most) most = f.nbytes[k];
+  }
+  int g = nblocks((most + 3) / 4);
+  if (g > 1024) g = 1024;  // grid-stride beyond
+  k_fill_list<<<dim3(g, f.n), GOSPL_BLOCK, 0, s>>>(f);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int ensure_groups(gospl_mesh* m, int64_t rows) {
   if (!m->pg_keys[0]) {
     GOSPL_CHECK(hipMalloc(&m->pg_keys[0], sizeof(unsigned long long) * (m->N + 64)));
     GOSPL_CHECK(hipMalloc(&m->pg_keys[1], sizeof(unsigned long long) * (m->N + 64)));
@@ -208,11 +241,11 @@ static int ensure_groups(gospl_mesh* m, int64_t rows) {
 }
 
 int pit_group(gospl_mesh* m, const int32_t* pitids, int64_t npits, const int32_t* cells,
-              const int32_t* ncells_dev, int64_t ncells, hipStream_t s) {
+              const int32_t* ncells_dev, int64_t ncells, hipStream_t s, bool cleared) {
   if (ensure_groups(m, npits + 1)) return 1;
   int32_t* start = m->pg_seg;
   int32_t* end = m->pg_seg + (m->pg_cap + 64);
-  GOSPL_CHECK(hipMemsetAsync(m->pg_seg, 0, sizeof(int32_t) * 2 * (m->pg_cap + 64), s));
+  if (!cleared) GOSPL_CHECK(hipMemsetAsync(m->pg_seg, 0, sizeof(int32_t) * 2 * (m->pg_cap + 64), s));
   m->pg_ids = pitids;
   m->pg_npits = npits;
   m->pg_n = ncells;
@@ -235,9 +268,8 @@ int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double
   const int64_t npits = m->pg_npits;
   const int32_t* start = m->pg_seg;
   const int32_t* end = m->pg_seg + (m->pg_cap + 64);
-  GOSPL_CHECK(hipMemsetAsync(pitparams, 0, sizeof(double) * cap * 3, s));
-  GOSPL_CHECK(hipMemsetAsync(filled_lvl, 0, sizeof(double) * cap * 5, s));
-  GOSPL_CHECK(hipMemsetAsync(filled_vol, 0, sizeof(double) * cap * 5, s));
+  // the three kernels write rows 0..npits of every table (one block or thread
+  // per pit, empty pits included); rows past npits are left as they were
   k_pit_reduce_params<<<(int)(npits + 1), GOSPL_BLOCK, 0, s>>>(m->pg_keys[0], start, end, hl, fill,
                                                          m->area, sl, cap, pitparams);
   k_pit_levels<<<nblocks(npits + 1), GOSPL_BLOCK, 0, s>>>(npits + 1, cap, pitparams, filled_lvl);

@@ -153,8 +153,11 @@ class Model:
         self._flvl = torch.zeros((5, cap), **f64)
         self._fvol = torch.zeros((5, cap), **f64)
         self._inv = torch.zeros(cap, **f64)
-        self._full = torch.zeros(cap, **i8)
-        self._level = torch.zeros(cap, **f64)
+        self._rhs = torch.zeros(N, **f64)
+        # a cascade pass's host decisions go to the device in one transfer,
+        # staged through a pinned buffer (grown on demand)
+        self._stage_h = None
+        self._stage_d = None
         self.tStart = float(tstart)
         self.tNow = float(tstart)
         self.tEnd = float(tend) if tend is not None else float(tstart)
@@ -203,11 +206,15 @@ class Model:
             self._events = []
         return out
 
-    def _h2d(self, a):
-        """Small host array -> device tensor without waiting for the stream
-        (staged through pinned memory)."""
-        t = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
-        return t.to(self.device, non_blocking=True)
+    def _staging(self, n):
+        """A numpy view of the pinned staging buffer, at least n doubles.  The
+        previous pass's transfer out of it has completed: every cascade pass
+        synchronises the stream (its inflow read-back) before writing."""
+        if self._stage_h is None or self._stage_h.numel() < n:
+            size = max(4096, 2 * n)
+            self._stage_h = torch.empty(size, dtype=torch.float64, pin_memory=True)
+            self._stage_d = torch.empty(size, dtype=torch.float64, device=self.device)
+        return self._stage_h.numpy()
 
     def _make_kfac(self):
         if self.sedfacVal is None and self.coeffd == 0.0:
@@ -288,52 +295,56 @@ class Model:
                    _ptr(self._inv), S)
         inV = self._inv[:npit].cpu().numpy()
         eV = inV - pitVol
-        full = np.zeros(npit, np.int8)
+        full = np.zeros(npit)
         level = np.full(npit, np.nan)
-        have_nfa = False
+        u = vv = np.zeros(0)
+        nfa_sum = 0.0
         if (eV > 0.0).any():
             eIDs = eV > 0.0
             pitVol[eIDs] = 0.0
-            full[eIDs] = 1
+            full[eIDs] = 1.0
             spillIDs = self.pitInfo[eIDs, 0]
             localSpill = np.where(self.pitInfo[eIDs, 1] == 0)[0]
             localPts = spillIDs[localSpill]
             vals = eV[eIDs][localSpill]
-            self._nfa.zero_()
             if len(localPts):
                 # numpy fancy assignment: a repeated index keeps the last value
                 u, last = np.unique(localPts[::-1], return_index=True)
                 vv = vals[::-1][last]
-                self._nfa[self._h2d(u)] = self._h2d(vv)
                 nfa_sum = float(np.sum(vv / self.dt))
-            else:
-                nfa_sum = 0.0
-            have_nfa = True
         if (eV < 0.0).any():
             eIDs = np.where((eV < 0.0) & (inV > 0.0))[0]
             pitVol[eIDs] += eV[eIDs]
             if len(eIDs):
                 nid = np.absolute(self.filled_vol[eIDs] - pitVol[eIDs][:, None]).argmin(axis=1)
                 level[eIDs] = self.filled_lvl[eIDs, nid]
-        if full.any() or np.isfinite(level).any():
-            self._full[:npit] = self._h2d(full)
-            self._level[:npit] = self._h2d(level)
-            self._call("gospl_pit_water", self.H, _ptr(self.pitIDs), _ptr(self._full), _ptr(self._level),
-                       _ptr(self.lFill), _ptr(self.waterFilled), S)
+        go = bool((eV > 1.0e-3).any())
+        if go or full.any() or np.isfinite(level).any():
+            # water levels of the pits and the next solve's right-hand side
+            # (the spilled volumes / dt), in one transfer and one call
+            k = len(u)
+            st = self._staging(2 * npit + 2 * k)
+            st[:npit] = full
+            st[npit:2 * npit] = level
+            st[2 * npit:2 * npit + k] = u
+            st[2 * npit + k:2 * npit + 2 * k] = vv
+            n = 2 * npit + 2 * k
+            self._stage_d[:n].copy_(self._stage_h[:n], non_blocking=True)
+            self._call("gospl_cascade_seed", self.H, _ptr(self._stage_d), npit, k, _ptr(self.pitIDs),
+                       _ptr(self.lFill), _ptr(self.waterFilled), self.dt, _ptr(self._rhs), S)
         nFA = None
-        if (eV > 1.0e-3).any():
+        if go:
             if step == 100:
                 self._buildFlowDirection(self.lFill)
             else:
                 self._buildFlowDirection(self.waterFilled)
-            rhs = self._nfa / self.dt if have_nfa else torch.zeros_like(self._nfa)
-            # sum of rhs: its nonzero entries are the spill values above (host
-            # side, no device synchronisation)
-            resid = nfa_sum if have_nfa else 0.0
+            # sum of the right-hand side: its nonzero entries are the spill
+            # values above (host side, no device synchronisation)
+            resid = nfa_sum
             self._cascade_resid = resid
             if resid > self.maxarea[0]:
                 excess = True
-                self._solve_flow(rhs, self._q, fatal=False)
+                self._solve_flow(self._rhs, self._q, fatal=False)
                 self._call("gospl_cascade_add", self.H, _ptr(self._q), _ptr(hl), _ptr(self.waterFilled),
                            self.dt, _ptr(self._nfa), _ptr(self.FAL), S)
                 nFA = self._nfa

@@ -101,8 +101,8 @@ int gospl_fill_depressions(gospl_mesh_t* mesh, const double* h, double level,
  * flatdirs (N), ptdir (N), lspill (N int8).  Per-pit arrays have capacity
  * `cap` rows (index 0 unused, pits 1..npits): spill (cap), pitparams
  * (3, cap) = volume, max fill, max depth; filled_lvl (5, cap) and
- * filled_vol (5, cap).  *npits (HOST) receives the pit count; the call
- * fails if npits + 1 > cap.  Per-pit sums are reduced in a fixed order, so
+ * filled_vol (5, cap); rows past npits are not written.  *npits (HOST)
+ * receives the pit count; the call fails if npits + 1 > cap.  Per-pit sums are reduced in a fixed order, so
  * results are reproducible run to run (their last bits differ from numpy's
  * group-by order).
  */
@@ -188,6 +188,14 @@ int gospl_rain_source(gospl_mesh_t* mesh, const double* rain, const double* lfil
  *    synchronises `stream`.
  *  - gospl_pit_water: per-pit water levels; full[p] != 0 sets water = lfill,
  *    else if level[p] is not NaN and water <= level[p], water = level[p].
+ *  - gospl_cascade_seed: one pass's device update from the host decisions
+ *    (flowplex.py:602-666) in ONE device buffer `stage` of 2*npit + 2*nspill
+ *    doubles: full[p] = stage[p] != 0 and level[p] = stage[npit + p] applied
+ *    as gospl_pit_water (pits p >= npit untouched), then rhs = 0 except
+ *    rhs[stage[2 npit + j]] = stage[2 npit + nspill + j] / dt for j < nspill
+ *    (distinct spill vertex ids, exact in f64; ids outside [0, N) are
+ *    skipped).  Replaces gospl_pit_water plus the host-side scatter of the
+ *    spilled volumes into the next solve's right-hand side.
  *  - gospl_cascade_add: nfa = q*dt; fal += (nfa masked where hl < water)/dt.
  *  - gospl_flow_finalize (flowplex.py:906-933): zero fal at filled pit
  *    cells, fillfal = fal with 0.1*max(fal) there (when has_pits), clamp >= 0.
@@ -197,6 +205,9 @@ int gospl_pit_inflow(gospl_mesh_t* mesh, const double* fa, const int8_t* lsink,
 int gospl_pit_water(gospl_mesh_t* mesh, const int32_t* pitids, const int8_t* full,
                     const double* level, const double* lfill, double* water,
                     gospl_stream_t stream);
+int gospl_cascade_seed(gospl_mesh_t* mesh, const double* stage, int64_t npit, int64_t nspill,
+                       const int32_t* pitids, const double* lfill, double* water, double dt,
+                       double* rhs, gospl_stream_t stream);
 int gospl_cascade_add(gospl_mesh_t* mesh, const double* q, const double* hl,
                       const double* water, double dt, double* nfa, double* fal,
                       gospl_stream_t stream);

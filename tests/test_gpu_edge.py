@@ -346,3 +346,49 @@ def test_spl_steady_state_analytic_gpu():
     assert abs(eu - 1.0) < 0.05, eu
     assert abs(ksr - 1.0) < 0.005, ksr
     assert iqr < 0.10, iqr
+
+
+@pytest.mark.parametrize("nspill", [0, 7])
+def test_cascade_seed_matches_pit_water_and_scatter(nspill):
+    """The one-call cascade update gives bit for bit what the pit-water call
+    and the scatter of the spilled volumes (numpy fancy assignment, then a
+    true division by dt) give, on random pit ids, levels (NaN = none) and spill points, with
+    the right-hand side starting from garbage."""
+    import torch
+
+    from gospl_amd.model import DeviceMesh, _ptr
+
+    m, h0, p = case("planar", 30)
+    dm = DeviceMesh(m, torch.device("cuda:0"))
+    lib = dm.lib
+    N, npit, dt = m.N, 50, 1234.5
+    rng = np.random.default_rng(11)
+    t = lambda a, dtp: torch.as_tensor(np.ascontiguousarray(a), dtype=dtp, device="cuda")
+    pit = t(rng.integers(-1, npit, N), torch.int32)
+    lfill = t(rng.uniform(0.0, 100.0, N), torch.float64)
+    water0 = t(rng.uniform(0.0, 100.0, N), torch.float64)
+    full = (rng.random(npit) < 0.3).astype(np.float64)
+    level = np.where(rng.random(npit) < 0.5, rng.uniform(0.0, 100.0, npit), np.nan)
+    u = rng.choice(N, size=nspill, replace=False).astype(np.float64)
+    vv = rng.uniform(1.0, 1.0e6, nspill)
+    s = torch.cuda.current_stream().cuda_stream
+
+    w1 = water0.clone()
+    assert lib.gospl_pit_water(dm.handle, _ptr(pit), _ptr(t(full, torch.int8)), _ptr(t(level, torch.float64)),
+                               _ptr(lfill), _ptr(w1), s) == 0
+    # numpy's true division, as goSPL's `nFA / self.dt` (flowplex.py:663);
+    # torch divides a tensor by a scalar as a product with its reciprocal
+    nfa = np.zeros(N)
+    nfa[u.astype(np.int64)] = vv
+    rhs1 = nfa / dt
+
+    w2 = water0.clone()
+    rhs2 = torch.full((N,), 7.0, dtype=torch.float64, device="cuda")
+    stage = t(np.concatenate([full, level, u, vv]), torch.float64)
+    assert lib.gospl_cascade_seed(dm.handle, _ptr(stage), npit, nspill, _ptr(pit), _ptr(lfill), _ptr(w2), dt,
+                                  _ptr(rhs2), s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(w1, w2)
+    assert np.array_equal(rhs2.cpu().numpy(), rhs1)
+    assert not torch.equal(w1, water0)  # the case moves some water levels
+    dm.close()

@@ -252,6 +252,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->pg_keys[0]);
   dfree(m->pg_keys[1]);
   dfree(m->pg_seg);
+  dfree(m->pack_buf);
   delete m;
   return 0;
 }

@@ -62,6 +62,8 @@ struct gospl_mesh {
   const int32_t* pg_ids;      // pit-id buffer the grouping was built from
   int64_t pg_npits;
   int64_t pg_n;               // grouped cells
+  double* pack_buf;           // per-pit tables packed for one read-back
+  int64_t pack_cap;           // its capacity (doubles)
   int timing;                 // record HIP events around the DAG sweep kernels
   hipEvent_t tev[2];
   double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep

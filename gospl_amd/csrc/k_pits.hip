@@ -279,9 +279,48 @@ int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double
   return 0;
 }
 
+// Rows 0..np1-1 of the per-pit tables, one block of np1 after the other:
+// spill (as f64, exact), pitparams rows 0-2, filled_lvl rows 0-4, filled_vol
+// rows 0-4.
+__global__ void k_pit_pack(int64_t np1, int64_t cap, const int32_t* __restrict__ spill,
+                           const double* __restrict__ params, const double* __restrict__ lvl,
+                           const double* __restrict__ vol, double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np1) return;
+  out[p] = (double)spill[p];
+  for (int r = 0; r < 3; ++r) out[(1 + r) * np1 + p] = params[r * cap + p];
+  for (int r = 0; r < 5; ++r) out[(4 + r) * np1 + p] = lvl[r * cap + p];
+  for (int r = 0; r < 5; ++r) out[(9 + r) * np1 + p] = vol[r * cap + p];
+}
+
 }  // namespace gospl
 
 using namespace gospl;
+
+extern "C" int gospl_pit_tables(gospl_mesh_t* m, int64_t npits, int64_t cap, const int32_t* spill,
+                                const double* pitparams, const double* filled_lvl, const double* filled_vol,
+                                double* out, gospl_stream_t stream) {
+  hipStream_t s = stream;
+  const int64_t np1 = npits + 1;
+  if (npits < 0 || np1 > cap || !out) {
+    set_error("gospl_pit_tables: bad arguments (npits=%lld, cap=%lld)", (long long)npits, (long long)cap);
+    return 1;
+  }
+  const int64_t n = 14 * np1;
+  if (m->pack_cap < n) {
+    if (m->pack_buf) GOSPL_CHECK(hipFree(m->pack_buf));
+    m->pack_buf = nullptr;
+    m->pack_cap = 0;
+    const int64_t want = n < 4096 ? 4096 : 2 * n;
+    GOSPL_CHECK(hipMalloc(&m->pack_buf, sizeof(double) * want));
+    m->pack_cap = want;
+  }
+  k_pit_pack<<<nblocks(np1), GOSPL_BLOCK, 0, s>>>(np1, cap, spill, pitparams, filled_lvl, filled_vol, m->pack_buf);
+  GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipMemcpyAsync(out, m->pack_buf, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  GOSPL_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
 
 extern "C" int gospl_pit_inflow(gospl_mesh_t* m, const double* fa, const int8_t* lsink,
                                 const int32_t* pitids, int64_t npit, double* inv,

@@ -158,6 +158,7 @@ class Model:
         # staged through a pinned buffer (grown on demand)
         self._stage_h = None
         self._stage_d = None
+        self._tables_h = None
         self.tStart = float(tstart)
         self.tNow = float(tstart)
         self.tEnd = float(tend) if tend is not None else float(tstart)
@@ -206,6 +207,13 @@ class Model:
             self._events = []
         return out
 
+    def _host_tables(self, n):
+        """Pinned host buffer of at least n doubles for the per-pit tables
+        (read only after the call that fills it has synchronised)."""
+        if self._tables_h is None or self._tables_h.numel() < n:
+            self._tables_h = torch.empty(max(4096, 2 * n), dtype=torch.float64, pin_memory=True)
+        return self._tables_h
+
     def _staging(self, n):
         """A numpy view of the pinned staging buffer, at least n doubles.  The
         previous pass's transfer out of it has completed: every cascade pass
@@ -252,8 +260,10 @@ class Model:
         n1 = int(npits.value) + 1
         self.npits = n1 - 1
         # the per-pit tables in one transfer (spill ids are exact in f64)
-        packed = torch.cat([self._spill[:n1].to(torch.float64), self._pitparams[:, :n1].reshape(-1),
-                            self._flvl[:, :n1].reshape(-1), self._fvol[:, :n1].reshape(-1)]).cpu().numpy()
+        host = self._host_tables(14 * n1)
+        self._call("gospl_pit_tables", self.H, n1 - 1, self._cap, _ptr(self._spill), _ptr(self._pitparams),
+                   _ptr(self._flvl), _ptr(self._fvol), host.data_ptr(), S)
+        packed = host.numpy()[:14 * n1]
         spill = packed[:n1].astype(np.int64)
         self.pitParams = packed[n1:4 * n1].reshape(3, n1).T.copy()
         self.filled_lvl = packed[4 * n1:9 * n1].reshape(5, n1).T.copy()

@@ -112,6 +112,19 @@ int gospl_pit_info(gospl_mesh_t* mesh, const double* hl, double* lfill, double s
                    double* filled_vol, int64_t* npits, gospl_stream_t stream);
 
 /*
+ * Rows 0..npits of gospl_pit_info's per-pit tables (same buffers and `cap`)
+ * read back to HOST memory `out` in one transfer: 14 * (npits + 1) doubles,
+ * one block of npits + 1 after the other = spill (as f64, exact), pitparams
+ * rows 0-2, filled_lvl rows 0-4, filled_vol rows 0-4 (goSPL's pitParams,
+ * filled_lvl and filled_vol with pitInfo's spill ids, pitfilling.py:521-555,
+ * 872-891).  Synchronises `stream`; pinned `out` makes the copy direct.
+ * No counterpart in goSPL (its tables are host arrays already).
+ */
+int gospl_pit_tables(gospl_mesh_t* mesh, int64_t npits, int64_t cap, const int32_t* spill,
+                     const double* pitparams, const double* filled_lvl,
+                     const double* filled_vol, double* out, gospl_stream_t stream);
+
+/*
  * Flow accumulation solve: replaces FAMesh._solve_KSP on the flow matrix
  * (I - W)^T built by matrixFlow (gospl/flow/flowplex.py:323-478, 843) with an
  * exact sweep over the receiver DAG.  rcv/wgt (nrcv, N), b (N) -> q (N).

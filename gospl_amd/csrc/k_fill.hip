@@ -136,10 +136,34 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_fill_relax_list(
   const bool own = t < n;
   const int u = own ? list[t] : 0;
   const int ru = own ? root[u] : 0;
-  const double zu = own ? dem[u] : 0.0;
-  const int dg = own ? deg[u] : 0;
+  // the neighbours in other basins, found once: their basin roots and the
+  // key of the edge's crossing height max(z_u, z_v) (position p: root or -1)
+  int rn[GOSPL_MAXW];
+  unsigned long long kn[GOSPL_MAXW];
+  {
+    const double zu = own ? dem[u] : 0.0;
+    const int dg = own ? deg[u] : 0;
+#pragma unroll
+    for (int p = 0; p < GOSPL_MAXW; ++p) {
+      int rv = -1;
+      unsigned long long kw = 0;
+      if (p < W && p < dg) {
+        const int v = nbr[(int64_t)p * N + u];
+        if (v >= 0 && state[v] >= 0) {
+          rv = root[v];
+          if (rv == ru) rv = -1;
+          const double w = dem[v] > zu ? dem[v] : zu;
+          kw = dkey(w);
+        }
+      }
+      rn[p] = rv;
+      kn[p] = kw;
+    }
+  }
   // a few rounds per launch, the workgroup's cells (consecutive boundary
-  // cells, a compact patch) seeing each other's lowered levels between rounds
+  // cells, a compact patch) seeing each other's lowered levels between
+  // rounds; a round is one load of this basin's level and of each listed
+  // neighbour basin's
   bool changed = false;
   for (int round = 0; round < FILL_REPEAT; ++round) {
     if (own) {
@@ -147,17 +171,13 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_fill_relax_list(
           __hip_atomic_load(&S[ru], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (cur != KEY_NEG_INF) {
         unsigned long long best = cur;
-        for (int p = 0; p < W && p < dg; ++p) {
-          const int v = nbr[(int64_t)p * N + u];
-          if (v < 0 || state[v] < 0) continue;
-          const int rv = root[v];
-          if (rv == ru) continue;
+#pragma unroll
+        for (int p = 0; p < GOSPL_MAXW; ++p) {
+          if (rn[p] < 0) continue;
           const unsigned long long sv =
-              __hip_atomic_load(&S[rv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              __hip_atomic_load(&S[rn[p]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (sv == KEY_POS_INF) continue;
-          const double w = dem[v] > zu ? dem[v] : zu;
-          const unsigned long long kw = dkey(w);
-          const unsigned long long c = kw > sv ? kw : sv;
+          const unsigned long long c = kn[p] > sv ? kn[p] : sv;
           if (c < best) best = c;
         }
         if (best < cur) {
@@ -397,29 +417,47 @@ __global__ __launch_bounds__(PTDIR_BLOCK) void k_ptdir_relax(
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // whole workgroup past the list
   const bool own = t < n;
   const int c = own ? cells[t] : 0;
-  const double fc = own ? fill[c] : 0.0;
-  const int dg = own ? deg[c] : 0;
+  // only this thread writes ptdir[c]: its current value stays in a register
+  int cur = own ? ptdir[c] : 1;
+  const bool active = own && cur != 1;
+  // the neighbours a distance can come from (fill not above this cell's),
+  // found once: position p holds the neighbour or -1 (fixed positions keep
+  // the list in registers)
+  int nb[GOSPL_MAXW];
+  {
+    const double fc = active ? fill[c] : 0.0;
+    const int dg = active ? deg[c] : 0;
+#pragma unroll
+    for (int p = 0; p < GOSPL_MAXW; ++p) {
+      int i = -1;
+      if (p < W && p < dg) {
+        i = nbr[(int64_t)p * N + c];
+        if (i >= 0 && !(fc >= fill[i])) i = -1;
+      }
+      nb[p] = i;
+    }
+  }
   // a few rounds per launch, the workgroup's cells (consecutive pit cells, a
-  // compact patch) seeing each other's updates between rounds
+  // compact patch) seeing each other's updates between rounds; a round is
+  // one load of each listed neighbour's distance
   bool changed = false;
   for (int round = 0; round < PTDIR_REPEAT; ++round) {
-    if (own) {
-      const int cur = ptdir[c];
-      if (cur != 1) {
-        int best = cur;
-        for (int p = 0; p < W && p < dg; ++p) {
-          const int i = nbr[(int64_t)p * N + c];
-          if (i < 0) continue;
-          const int di = ptdir[i];
-          if (di > -1 && fc >= fill[i]) {
+    if (active) {
+      int best = cur;
+#pragma unroll
+      for (int p = 0; p < GOSPL_MAXW; ++p) {
+        if (nb[p] >= 0) {
+          const int di = ptdir[nb[p]];
+          if (di > -1) {
             const int cand = di + 1;
             if (best < 0 || cand < best) best = cand;
           }
         }
-        if (best != cur) {
-          ptdir[c] = best;
-          changed = true;
-        }
+      }
+      if (best != cur) {
+        ptdir[c] = best;
+        cur = best;
+        changed = true;
       }
     }
     __syncthreads();

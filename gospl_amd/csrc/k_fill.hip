@@ -406,7 +406,7 @@ __global__ void k_ptdir_init(int64_t N, const int8_t* __restrict__ ls, int32_t* 
 
 // Breadth-first distance from the spill cells over pit cells along
 // not-descending edges (fill_dir + nghb_dir, functions.F90:1845-1969).
-#define PTDIR_REPEAT 4
+#define PTDIR_REPEAT 8
 #define PTDIR_BLOCK 256
 __global__ __launch_bounds__(PTDIR_BLOCK) void k_ptdir_relax(
     int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,

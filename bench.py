@@ -30,7 +30,7 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic (compulsory, single-pass) bytes per vertex per step of each
 # phase at FP64 state, int32 indices, 6 neighbours, 3 receivers.
 ALG_BYTES = {
-    "gospl_fill_depressions": 44,
+    "gospl_fill_elevation": 44,
     "gospl_pit_info": 64,
     "gospl_flow_direction": 156,
     "gospl_flowacc": 68,

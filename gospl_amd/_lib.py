@@ -29,6 +29,7 @@ SIGNATURES = {
     "gospl_mfdreceivers": (I32, [P, I32, F64, P, F64, P, P, P, P]),
     "gospl_flow_direction": (I32, [P, I32, F64, P, F64, P, I32, P, P, P, P, P, P, P]),
     "gospl_fill_depressions": (I32, [P, P, F64, F64, P, P]),
+    "gospl_fill_elevation": (I32, [P, P, I32, F64, F64, P, P]),
     "gospl_pit_info": (I32, [P, P, P, F64, P, P, P, P, I64, P, P, P, P, ctypes.POINTER(I64), P]),
     "gospl_pit_tables": (I32, [P, I64, I64, P, P, P, P, P, P]),
     "gospl_flowacc": (I32, [P, I32, P, P, P, P, ctypes.POINTER(I32), ctypes.POINTER(I64), P]),

@@ -37,13 +37,53 @@ namespace gospl {
 constexpr unsigned long long KEY_POS_INF = 0xFFF0000000000000ull;  // dkey(+inf)
 constexpr unsigned long long KEY_NEG_INF = 0x000FFFFFFFFFFFFFull;  // dkey(-inf)
 
+// The fill level: the given one, or, when `lv` is set, fillElevation's
+// choice from the minimum elevation the device found (pitfilling.py:860-866):
+// minh = min(h) (+ 1e-3 when lv->add_eps), level = max(minh, lv->floor) with
+// Python's max (the first argument unless the second is greater).
+struct FillLevel {
+  const unsigned long long* minkey;  // dkey of min(h)
+  int add_eps;
+  double floor;
+};
+__device__ __forceinline__ double fill_level(double level, const FillLevel& lv) {
+  if (!lv.minkey) return level;
+  double mh = dkey_inv(*lv.minkey);
+  if (lv.add_eps) mh += 1.0e-3;
+  return lv.floor > mh ? lv.floor : mh;
+}
+
+// dkey of min(h): a fixed grid strides over the mesh, one atomic per
+// workgroup (exact and independent of order); *key starts at ~0.
+#define FILL_MIN_BLOCKS 512
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_min_key(int64_t N, const double* __restrict__ h,
+                                                        unsigned long long* __restrict__ key) {
+  __shared__ unsigned long long sh[GOSPL_BLOCK / 64];
+  unsigned long long k = ~0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long c = dkey(h[i]);
+    k = c < k ? c : k;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long c = __shfl_xor(k, o);
+    k = c < k ? c : k;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) k = sh[w] < k ? sh[w] : k;
+    atomicMin(key, k);
+  }
+}
+
 // state: 1 seed, 0 included, -1 excluded (edge_tile, functions.F90:2036-2080)
 __global__ void k_fill_init(int64_t N, int W, const int32_t* __restrict__ nbr,
                             const int32_t* __restrict__ deg, const int8_t* __restrict__ outlet,
-                            int flat, const double* __restrict__ h, double level,
+                            int flat, const double* __restrict__ h, double level_in, FillLevel lv,
                             double* __restrict__ dem, int32_t* __restrict__ state) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const double level = fill_level(level_in, lv);
   const double z = h[i] - level;
   dem[i] = z;
   const bool border = flat && outlet[i];
@@ -193,10 +233,11 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_fill_relax_list(
 
 __global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
                              const int32_t* __restrict__ state, const int32_t* __restrict__ root,
-                             const unsigned long long* __restrict__ S, double level, double sl,
-                             double* __restrict__ lfill) {
+                             const unsigned long long* __restrict__ S, double level_in, FillLevel lv,
+                             double sl, double* __restrict__ lfill) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  const double level = fill_level(level_in, lv);
   const double z = dem[i];
   double f = z;
   if (state[i] >= 0) {
@@ -547,15 +588,15 @@ using namespace gospl;
 
 extern "C" {
 
-int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, double sealevel,
-                           double* lfill, gospl_stream_t s) {
+static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel lv, double sealevel,
+                     double* lfill, hipStream_t s) {
   const int64_t N = m->N;
   double* dem = m->wd[0];
   int32_t* state = m->wi[3];
   int32_t* root = m->wi[4];
   unsigned long long* S = m->wu[0];
   const int g = nblocks(N);
-  k_fill_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->outlet, m->flat, h, level, dem,
+  k_fill_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->outlet, m->flat, h, level, lv, dem,
                                          state);
   k_fill_descent<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S);
   GOSPL_CHECK(hipGetLastError());
@@ -582,9 +623,23 @@ int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, doubl
                                                       bcount, f, pass);
       }, &it))
     return 1;
-  k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, sealevel, lfill);
+  k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, lv, sealevel, lfill);
   GOSPL_CHECK(hipGetLastError());
   return 0;
+}
+
+int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, double sealevel,
+                           double* lfill, gospl_stream_t s) {
+  return fill_body(m, h, level, FillLevel{nullptr, 0, 0.0}, sealevel, lfill, s);
+}
+
+int gospl_fill_elevation(gospl_mesh_t* m, const double* h, int32_t add_eps, double floor_level,
+                         double sealevel, double* lfill, gospl_stream_t s) {
+  unsigned long long* key = m->wu[1] + 8;  // free while the fill runs
+  GOSPL_CHECK(hipMemsetAsync(key, 0xff, sizeof(unsigned long long), s));
+  k_min_key<<<FILL_MIN_BLOCKS, GOSPL_BLOCK, 0, s>>>(m->N, h, key);
+  GOSPL_CHECK(hipGetLastError());
+  return fill_body(m, h, 0.0, FillLevel{key, add_eps ? 1 : 0, floor_level}, sealevel, lfill, s);
 }
 
 int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double sealevel,

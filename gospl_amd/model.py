@@ -244,14 +244,11 @@ class Model:
     # -------------------------------------------------------------- pits
     def fillElevation(self, sed=False):
         """pitfilling.py:849-903 (water filling, sed=False)."""
-        minh = float(self.hLocal.min().item())
-        if not self.flatModel:
-            minh += 1.0e-3
-        level = max(minh, self.sealevel + self.oFill)
-        self.level = level
+        # level = max(min h (+1e-3 on the sphere), sealevel + oFill), chosen
+        # on the device (no read-back)
         S = self.stream
-        self._call("gospl_fill_depressions", self.H, _ptr(self.hLocal), level, self.sealevel,
-                   _ptr(self.lFill), S)
+        self._call("gospl_fill_elevation", self.H, _ptr(self.hLocal), 0 if self.flatModel else 1,
+                   self.sealevel + self.oFill, self.sealevel, _ptr(self.lFill), S)
         npits = ctypes.c_int64(0)
         self._call("gospl_pit_info", self.H, _ptr(self.hLocal), _ptr(self.lFill), self.sealevel,
                    _ptr(self.pitIDs), _ptr(self.flatDirs), _ptr(self.ptdir), _ptr(self.lspill),

@@ -91,6 +91,15 @@ int gospl_fill_depressions(gospl_mesh_t* mesh, const double* h, double level,
                            double sealevel, double* lfill, gospl_stream_t stream);
 
 /*
+ * fillElevation (gospl/flow/pitfilling.py:849-870) with the level chosen on
+ * the device, so nothing is read back: level = max(min(h) (+ 1e-3 when
+ * add_eps, goSPL's sphere case), floor_level) with Python's max, floor_level
+ * = sealevel + oFill as goSPL computes it; then as gospl_fill_depressions.
+ */
+int gospl_fill_elevation(gospl_mesh_t* mesh, const double* h, int32_t add_eps, double floor_level,
+                         double sealevel, double* lfill, gospl_stream_t stream);
+
+/*
  * Depression information: replaces PITFill._pitInformation and the
  * depth-volume curves of fillElevation (gospl/flow/pitfilling.py:704-847,
  * 872-891), i.e. the Fortran `label_pits`, `pits_cons`, `fill_dir`,

@@ -392,3 +392,31 @@ def test_cascade_seed_matches_pit_water_and_scatter(nspill):
     assert np.array_equal(rhs2.cpu().numpy(), rhs1)
     assert not torch.equal(w1, water0)  # the case moves some water levels
     dm.close()
+
+
+@pytest.mark.parametrize("kind,size", [("sphere", 5), ("planar", 30)])
+def test_fill_elevation_level_on_device(kind, size):
+    """The fill with its level chosen on the device equals the fill given
+    goSPL's host-side level max(min h (+1e-3 on the sphere), sealevel +
+    oFill) (pitfilling.py:862-868), bit for bit, with the floor both below
+    and above the minimum."""
+    import torch
+
+    from gospl_amd.model import DeviceMesh, _ptr
+
+    m, h0, p = case(kind, size)
+    dm = DeviceMesh(m, torch.device("cuda:0"))
+    lib = dm.lib
+    s = torch.cuda.current_stream().cuda_stream
+    h = torch.as_tensor(np.ascontiguousarray(h0), dtype=torch.float64, device="cuda")
+    add_eps = 0 if kind == "planar" else 1
+    for floor in (float(h0.min()) - 5000.0, float(np.median(h0))):
+        minh = float(h0.min()) + (1.0e-3 if add_eps else 0.0)
+        level = max(minh, floor)
+        a = torch.empty_like(h)
+        b = torch.empty_like(h)
+        assert lib.gospl_fill_depressions(dm.handle, _ptr(h), level, 0.0, _ptr(a), s) == 0
+        assert lib.gospl_fill_elevation(dm.handle, _ptr(h), add_eps, floor, 0.0, _ptr(b), s) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), floor
+    dm.close()

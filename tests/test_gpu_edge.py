@@ -420,3 +420,32 @@ def test_fill_elevation_level_on_device(kind, size):
         torch.cuda.synchronize()
         assert torch.equal(a, b), floor
     dm.close()
+
+
+def test_pit_tables_read_back_matches_device_tables():
+    """The packed read-back of the per-pit tables holds rows 0..npits of the
+    spill, parameter and depth-volume tables exactly as they are on the
+    device."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    m, h0, p = case("sphere", 6)
+    g = Model(m, h0, tend=1e12, **p)
+    g.initialise()
+    n1 = g.npits + 1
+    assert n1 > 1
+    host = torch.empty(14 * n1 + 7, dtype=torch.float64, pin_memory=True)
+    host.fill_(123.0)
+    assert g.lib.gospl_pit_tables(g.H, g.npits, g._cap, g._spill.data_ptr(), g._pitparams.data_ptr(),
+                                  g._flvl.data_ptr(), g._fvol.data_ptr(), host.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream) == 0
+    ref = torch.cat([g._spill[:n1].to(torch.float64), g._pitparams[:, :n1].reshape(-1),
+                     g._flvl[:, :n1].reshape(-1), g._fvol[:, :n1].reshape(-1)]).cpu()
+    assert torch.equal(host[:14 * n1], ref)
+    assert torch.all(host[14 * n1:] == 123.0)  # nothing written past the tables
+    # a bad pit count is refused
+    assert g.lib.gospl_pit_tables(g.H, g._cap, g._cap, g._spill.data_ptr(), g._pitparams.data_ptr(),
+                                  g._flvl.data_ptr(), g._fvol.data_ptr(), host.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream) != 0
+    g.dm.close()

@@ -253,6 +253,9 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->pg_keys[1]);
   dfree(m->pg_seg);
   dfree(m->pack_buf);
+  dfree(m->hcoef);
+  dfree(m->hdiag);
+  dfree(m->hsea);
   delete m;
   return 0;
 }

@@ -62,6 +62,11 @@ struct gospl_mesh {
   const int32_t* pg_ids;      // pit-id buffer the grouping was built from
   int64_t pg_npits;
   int64_t pg_n;               // grouped cells
+  double* hcoef;              // hillslope operator rows (W, N), kept between calls
+  double* hdiag;              // its diagonal
+  int8_t* hsea;               // the sea flags the rows were built from
+  double hpar[3];             // cda, cdm, dt they were built with
+  int hvalid;
   double* pack_buf;           // per-pit tables packed for one read-back
   int64_t pack_cap;           // its capacity (doubles)
   int timing;                 // record HIP events around the DAG sweep kernels

@@ -5,20 +5,23 @@
 // off = -v; flat-mesh outlets are identity rows (hillslope.py:253-255).
 // M h' = h is solved by Jacobi sweeps: M is strictly diagonally dominant with
 // contraction factor max_k sum v / (1 + sum v) (~1e-5 at the benchmark
-// parameters), so a handful of sweeps reach a fixed point in FP64.  Rows are
-// rebuilt every call from the sea mask of the current elevation.
+// parameters), so a handful of sweeps reach a fixed point in FP64.  Row k
+// depends on the elevation only through the sea flags (h <= sealevel) of k and
+// its neighbours, so the rows are kept in the handle with the flags they were
+// built from, and a call rebuilds only the rows where one of those flags
+// changed (goSPL keeps its operator until the coastline moves,
+// hillslope.py:184-200); new diffusivities or dt rebuild every row.
 #include "common.h"
 
 namespace gospl {
 
-__global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
-                            const int32_t* __restrict__ deg, const double* __restrict__ area,
-                            const double* __restrict__ elgt, const double* __restrict__ vdist,
-                            const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
-                            double sl, double cda, double cdm, double dt, double* __restrict__ coef,
-                            double* __restrict__ diag) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+// One operator row; the sea flags decide each vertex's diffusivity.
+__device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int32_t* __restrict__ nbr,
+                                         const int32_t* __restrict__ deg, const double* __restrict__ area,
+                                         const double* __restrict__ elgt, const double* __restrict__ vdist,
+                                         const int8_t* __restrict__ outlet, int flat,
+                                         const double* __restrict__ h, double sl, double cda, double cdm,
+                                         double dt, double* __restrict__ coef, double* __restrict__ diag) {
   const double a = area[i];
   const bool ident = flat && outlet[i];
   const double ck = (h[i] <= sl ? cdm : cda) * dt;
@@ -40,6 +43,49 @@ __global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
     coef[(int64_t)p * N + i] = c_out;
   }
   diag[i] = ident ? 1.0 : (a > 0.0 ? 1.0 + s1 : 0.0);
+}
+
+// Every row, and the sea flags they were built from.
+__global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
+                            const int32_t* __restrict__ deg, const double* __restrict__ area,
+                            const double* __restrict__ elgt, const double* __restrict__ vdist,
+                            const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
+                            double sl, double cda, double cdm, double dt, double* __restrict__ coef,
+                            double* __restrict__ diag, int8_t* __restrict__ sea) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag);
+  sea[i] = h[i] <= sl;
+}
+
+// The sea flags of the current elevation against the kept ones: chg = 1
+// where a vertex changed side; the kept flags are updated.
+__global__ void k_hill_sea(int64_t N, const double* __restrict__ h, double sl, int8_t* __restrict__ sea,
+                           int8_t* __restrict__ chg) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int8_t f = h[i] <= sl;
+  chg[i] = f != sea[i];
+  sea[i] = f;
+}
+
+// The rows whose vertex or a neighbour changed side, rebuilt.
+__global__ void k_hill_coef_changed(int64_t N, int W, const int32_t* __restrict__ nbr,
+                                    const int32_t* __restrict__ deg, const double* __restrict__ area,
+                                    const double* __restrict__ elgt, const double* __restrict__ vdist,
+                                    const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
+                                    double sl, double cda, double cdm, double dt,
+                                    const int8_t* __restrict__ chg, double* __restrict__ coef,
+                                    double* __restrict__ diag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  bool need = chg[i] != 0;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg && !need; ++p) {
+    const int n = nbr[(int64_t)p * N + i];
+    if (n >= 0 && chg[n]) need = true;
+  }
+  if (need) hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag);
 }
 
 __global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
@@ -115,15 +161,34 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
                                int32_t sweeps, double* h, double* cumed, double* eblocal,
                                gospl_stream_t s) {
 
-  if (gospl_ensure_coef(m, m->W)) return 1;
   const int64_t N = m->N;
   const int g = nblocks(N);
-  double* coef = m->wcoef;
-  double* diag = m->wd[3];
+  if (!m->hcoef) {
+    GOSPL_CHECK(hipMalloc(&m->hcoef, sizeof(double) * (int64_t)m->W * (N + 64)));
+    GOSPL_CHECK(hipMalloc(&m->hdiag, sizeof(double) * (N + 64)));
+    GOSPL_CHECK(hipMalloc(&m->hsea, N + 64));
+    m->hvalid = 0;
+  }
+  double* coef = m->hcoef;
+  double* diag = m->hdiag;
   double* xa = m->wd[1];
   double* xb = m->wd[2];
-  k_hill_coef<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, m->outlet,
-                                         m->flat, h, sealevel, cda, cdm, dt, coef, diag);
+  if (!m->hvalid || m->hpar[0] != cda || m->hpar[1] != cdm || m->hpar[2] != dt) {
+    k_hill_coef<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, m->outlet,
+                                           m->flat, h, sealevel, cda, cdm, dt, coef, diag, m->hsea);
+    GOSPL_CHECK(hipGetLastError());
+    m->hpar[0] = cda;
+    m->hpar[1] = cdm;
+    m->hpar[2] = dt;
+    m->hvalid = 1;
+  } else {
+    int8_t* chg = reinterpret_cast<int8_t*>(m->wi[0]);
+    k_hill_sea<<<g, GOSPL_BLOCK, 0, s>>>(N, h, sealevel, m->hsea, chg);
+    k_hill_coef_changed<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist,
+                                                   m->outlet, m->flat, h, sealevel, cda, cdm, dt, chg, coef,
+                                                   diag);
+    GOSPL_CHECK(hipGetLastError());
+  }
   if (sweeps < 1) sweeps = 1;
   const double* xin = h;
   double* xout = xa;

@@ -176,6 +176,10 @@ int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const doubl
  * solve.  Diffusivity cda on land, cdm at sea (h <= sealevel), times dt.
  * Updates h, cumed and eblocal (+= dh/dt).  `sweeps` Jacobi sweeps (the
  * Python host derives the count from the operator's contraction bound).
+ * The operator rows are kept in the handle between calls: a call rebuilds
+ * the rows where the sea flag of the vertex or of a neighbour changed, all
+ * of them when cda, cdm or dt change (goSPL keeps its operator until the
+ * coastline moves, hillslope.py:184-200); results equal a full rebuild's.
  */
 int gospl_hillslope(gospl_mesh_t* mesh, double cda, double cdm, double dt, double sealevel,
                     int32_t sweeps, double* h, double* cumed, double* eblocal,

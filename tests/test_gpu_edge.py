@@ -449,3 +449,35 @@ def test_pit_tables_read_back_matches_device_tables():
                                   g._flvl.data_ptr(), g._fvol.data_ptr(), host.data_ptr(),
                                   torch.cuda.current_stream().cuda_stream) != 0
     g.dm.close()
+
+
+def test_hillslope_operator_kept_between_calls():
+    """A handle that keeps its hillslope rows from an earlier call (other
+    sea flags, then other diffusivities) gives bit for bit what a fresh
+    handle gives on the same elevation."""
+    import torch
+
+    from gospl_amd.model import DeviceMesh, _ptr
+
+    m, h0, p = case("sphere", 5)
+    dev = torch.device("cuda:0")
+    kept = DeviceMesh(m, dev)
+    s = torch.cuda.current_stream().cuda_stream
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+    def run(dm, h, cda, cdm, sl):
+        hh, ce, eb = t(h), torch.zeros(m.N, dtype=torch.float64, device="cuda"), \
+            torch.zeros(m.N, dtype=torch.float64, device="cuda")
+        assert dm.lib.gospl_hillslope(dm.handle, cda, cdm, 5000.0, sl, 8, _ptr(hh), _ptr(ce), _ptr(eb), s) == 0
+        torch.cuda.synchronize()
+        return hh.cpu().numpy(), ce.cpu().numpy()
+
+    # first call: sea level 0; second: a higher sea level moves the coastline;
+    # third: other diffusivities
+    for (cda, cdm, sl) in [(0.5, 0.5, 0.0), (0.5, 2.0, 800.0), (0.5, 2.0, -300.0), (0.1, 3.0, -300.0)]:
+        a = run(kept, h0, cda, cdm, sl)
+        fresh = DeviceMesh(m, dev)
+        b = run(fresh, h0, cda, cdm, sl)
+        fresh.close()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (cda, cdm, sl)
+    kept.close()

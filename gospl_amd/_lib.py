@@ -26,6 +26,7 @@ SIGNATURES = {
     "gospl_fv_geometry": (I32, [P, I64, P, P, I64, P, I64, P, I32, P, P, P, P, P]),
     "gospl_mesh_create": (I32, [I64, I32, I32, P, P, P, P, P, P, P, ctypes.POINTER(P)]),
     "gospl_mesh_destroy": (I32, [P]),
+    "gospl_mesh_touch": (I32, [P]),
     "gospl_mfdreceivers": (I32, [P, I32, F64, P, F64, P, P, P, P]),
     "gospl_flow_direction": (I32, [P, I32, F64, P, F64, P, I32, P, P, P, P, P, P, P]),
     "gospl_fill_depressions": (I32, [P, P, F64, F64, P, P]),

@@ -224,6 +224,15 @@ int gospl_ensure_coef(gospl_mesh* m, int rows) {
   return 0;
 }
 
+int gospl_mesh_touch(gospl_mesh_t* m) {
+  if (!m) {
+    set_error("gospl_mesh_touch: null handle");
+    return 1;
+  }
+  m->hvalid = 0;
+  return 0;
+}
+
 int gospl_mesh_destroy(gospl_mesh_t* m) {
   if (!m) return 0;
   // best effort: release everything even if one call fails

@@ -85,6 +85,7 @@ def updatearea(narea):
     dev = _state.get("dev")
     if dev is not None:
         dev["area"].copy_(_torch().as_tensor(a))
+        _lib.check(_lib.load().gospl_mesh_touch(dev["handle"]), "gospl_mesh_touch")
 
 
 def _torch():

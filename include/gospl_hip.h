@@ -59,6 +59,14 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr,
 int gospl_mesh_destroy(gospl_mesh_t* mesh);
 
 /*
+ * The caller changed, in place, mesh arrays it lent to the handle (e.g. the
+ * areas, as goSPL's `updatearea`, functions.F90:2699-2713): drop what the
+ * handle derived from them and keeps between calls (the hillslope rows).
+ * No counterpart in goSPL (its operator is rebuilt from its own arrays).
+ */
+int gospl_mesh_touch(gospl_mesh_t* mesh);
+
+/*
  * Multiple-flow-direction receivers: replaces `mfdreceivers`
  * (functions.pyf:146-156, functions.F90:1544-1671).  elev (N) -> rcv, dist,
  * wgt (nrcv, N).  nrcv <= 12.

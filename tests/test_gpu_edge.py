@@ -480,4 +480,18 @@ def test_hillslope_operator_kept_between_calls():
         b = run(fresh, h0, cda, cdm, sl)
         fresh.close()
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (cda, cdm, sl)
+    # areas changed in place (goSPL's updatearea) and announced: the kept rows
+    # are dropped, as a fresh handle on the new areas shows
+    before = run(kept, h0, 0.1, 3.0, -300.0)
+    kept.area.mul_(1.5)
+    assert kept.lib.gospl_mesh_touch(kept.handle) == 0
+    a = run(kept, h0, 0.1, 3.0, -300.0)
+    import copy
+    m2 = copy.copy(m)
+    m2.area = m.area * 1.5
+    fresh = DeviceMesh(m2, dev)
+    b = run(fresh, h0, 0.1, 3.0, -300.0)
+    fresh.close()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert not np.array_equal(a[0], before[0])
     kept.close()

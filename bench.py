@@ -97,7 +97,8 @@ def cpu_baseline(m, h0, p, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed steps (the workload is a 100-step run at dt = 5000 yr)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--level", type=int, default=9)
     ap.add_argument("--seed", type=int, default=0)

@@ -221,16 +221,16 @@ def test_csr_rows_match_packed_records():
 @pytest.mark.parametrize("level", [6, 8])
 def test_dataflow_sweep_matches_level_sweep(level):
     """The dataflow sweeps (the default for flow accumulation and stream
-    power), with their normal pools and with small pools that hand most
-    releases to the global queue, give the bits of the level-by-level
-    sweeps."""
+    power) give the bits of the level-by-level sweeps: with the default
+    hand-off (a batch goes to the global queue as soon as a pool holds one),
+    with pools that keep 128 vertices, and with small pools."""
     import os
 
     from gospl_amd.model import Model
 
     m, h0, p = case("sphere", level)
     out = []
-    for chain, pool, share in (("0", "512", "128"), ("1", "512", "128"), ("1", "128", "64")):
+    for chain, pool, share in (("0", "512", "0"), ("1", "512", "0"), ("1", "512", "128"), ("1", "128", "64")):
         os.environ["GOSPL_DAG_CHAIN"] = chain
         os.environ["GOSPL_CHAIN_POOL"] = pool
         os.environ["GOSPL_CHAIN_SHARE"] = share

@@ -822,7 +822,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
 #define CHAIN_THREADS 256
 #define CHAIN_POOL 512
 #define CHAIN_SPIN_MAX (1 << 21)
-#define CHAIN_POLLERS 256  // idle wavefronts that stay to poll the queue
+#define CHAIN_POLLERS 32  // idle wavefronts that stay to poll the queue (256: a few % slower)
 // control words, one 128-byte line each
 #define CQ_QUEUE 0    // 64-bit: batches reserved << 32 | batches claimed
 #define CQ_ACTIVE 32  // wavefronts holding work
@@ -1602,10 +1602,12 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   }
   if (per_cu < 1) per_cu = 1;
   const int maxb = per_cu * m->num_cu;
-  // dataflow sweeps (GOSPL_DAG_CHAIN=0: level by level); two workgroups per
-  // CU, within what the GPU holds at once (GOSPL_CHAIN_BLOCKS overrides).
-  // More resident wavefronts do not shorten the dependency chains and only
-  // share the SIMDs with the chain that is on the longest path.
+  // dataflow sweeps (GOSPL_DAG_CHAIN=0: level by level); one workgroup per
+  // CU (GOSPL_CHAIN_BLOCKS overrides).  More resident wavefronts do not
+  // shorten the dependency chains: they share the SIMDs and the memory
+  // pipeline with the chain that is on the longest path (two per CU: flow
+  // 1.27 / 0.99 ms, erosion 1.37 / 1.65 ms on early / late steps of the
+  // 2.6M-vertex sphere, against 1.19 / 0.86 and 1.27 / 1.60 with one).
   const char* te_chain = getenv("GOSPL_DAG_CHAIN");
   m->dag_chain = !(te_chain && atoi(te_chain) == 0);
   const void* kc[2] = {(const void*)k_chain<0>, (const void*)k_chain<1>};
@@ -1613,7 +1615,11 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
   for (int i = 0; i < 2; ++i) {
     int b = 0;
     GOSPL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kc[i], CHAIN_THREADS, 0));
-    m->chain_blocks[i] = (b < 1 ? 1 : b > 2 ? 2 : b) * m->num_cu;
+    if (b < 1) {
+      set_error("the dataflow kernel does not fit on a compute unit");
+      return 1;
+    }
+    m->chain_blocks[i] = m->num_cu;
     if (te_cb && atoi(te_cb) > 0) m->chain_blocks[i] = atoi(te_cb);
   }
   // pool capacity per wavefront (GOSPL_CHAIN_POOL, 128..512) and the pool

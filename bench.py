@@ -51,8 +51,8 @@ SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_chain<0>",
 # kernels crash the counter-collecting profiler.
 PMC_SCALE = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
 PMC_KIB = {
-    "gospl_flowacc": {"FETCH_SIZE": 213866.4 + 50847.2, "WRITE_SIZE": 15296.6 + 140025.0},
-    "gospl_spl": {"FETCH_SIZE": 196744.1 + 46317.3, "WRITE_SIZE": 17259.2 + 132100.2},
+    "gospl_flowacc": {"FETCH_SIZE": 213481.8 + 50784.1, "WRITE_SIZE": 15297.1 + 139938.3},
+    "gospl_spl": {"FETCH_SIZE": 196818.8 + 46179.1, "WRITE_SIZE": 17258.1 + 132079.6},
 }
 
 

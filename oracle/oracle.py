@@ -223,6 +223,10 @@ class OracleModel:
         self.rainVal[self.rainVal < 0] = 0.0
         self.upsub = None if upsub is None else np.broadcast_to(np.asarray(upsub, np.float64), (N,)).copy()
         self.sedfacVal = None if sedfac is None else np.broadcast_to(np.asarray(sedfac, np.float64), (N,)).copy()
+        if self.sedfacVal is not None:
+            # erodibility factors below 0.1 are raised to 0.1 when goSPL loads the
+            # map (_updateEroFactor, gospl/mesher/unstructuredmesh.py:1089)
+            self.sedfacVal[self.sedfacVal < 0.1] = 0.1
         self.h = np.array(h, dtype=np.float64)
         self.cumED = np.zeros(N)
         self.EbLocal = np.zeros(N)

@@ -495,3 +495,17 @@ def test_hillslope_operator_kept_between_calls():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert not np.array_equal(a[0], before[0])
     kept.close()
+
+
+def test_erodibility_factor_and_precipitation_exponent():
+    """A per-vertex erodibility factor (goSPL's sedfactor map, clipped at 0.1
+    on loading, unstructuredmesh.py:1089; the map config 4's log-K field
+    feeds) and a precipitation exponent d (Kbr = K * factor * P^d,
+    SPL.py:65-70) over lognormal rain, step by step against the oracle."""
+    m, h0, p = case("sphere", 5)
+    rng = np.random.default_rng(7)
+    q = dict(p)
+    q["sedfac"] = m.to_device_order(np.exp(rng.uniform(np.log(0.02), np.log(3.0), m.N)))
+    q["d"] = 0.5
+    assert (q["sedfac"] < 0.1).any()
+    _compare_steps(m, h0, q, nsteps=3)

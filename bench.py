@@ -237,6 +237,7 @@ def main():
                              "workload"},
         "cpu_baseline": cpu,
     }
+    g.close()
     print(json.dumps(out), flush=True)
 
 

@@ -5,7 +5,6 @@
 #include <cstdio>
 #include <cstring>
 
-#include <hipcub/hipcub.hpp>
 
 #include "common.h"
 
@@ -154,6 +153,67 @@ extern "C" {
 
 const char* gospl_last_error(void) { return g_err; }
 
+}  // extern "C"
+
+namespace gospl {
+
+// Counts the neighbour relations of the (W, N) table that are not mutual; the
+// receiver-DAG sweeps find a donor's row entry through the receiver's
+// neighbour list, so every relation must be mutual (as in a triangulation
+// whose vertices have at most GOSPL_MAXW neighbours).
+static int check_symmetry(int64_t N, int W, const int32_t* nbr, const int32_t* deg, const char* who) {
+  int32_t* asym = nullptr;
+  GOSPL_CHECK(hipMalloc(&asym, sizeof(int32_t)));
+  int32_t h = 0;
+  hipError_t e = hipMemset(asym, 0, sizeof(int32_t));
+  if (e == hipSuccess) {
+    k_nbr_symmetry<<<nblocks(N), GOSPL_BLOCK>>>(N, W, nbr, deg, asym);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&h, asym, sizeof(int32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(asym);
+  if (e != hipSuccess) {
+    set_error("%s: neighbour check failed: %s", who, hipGetErrorString(e));
+    return 1;
+  }
+  if (h) {
+    set_error("%s: %d neighbour relations are not mutual (a vertex with more than %d neighbours?)", who,
+              (int)h, GOSPL_MAXW);
+    return 1;
+  }
+  return 0;
+}
+
+// Device workspace of a new handle (every pointer starts null, so a failure
+// part way leaves a handle gospl_mesh_destroy can release).
+static int mesh_alloc(gospl_mesh* m) {
+  const int64_t N = m->N;
+  GOSPL_CHECK(hipGetDevice(&m->device));
+  hipDeviceProp_t prop;
+  GOSPL_CHECK(hipGetDeviceProperties(&prop, m->device));
+  m->num_cu = prop.multiProcessorCount;
+  for (int k = 0; k < 8; ++k) GOSPL_CHECK(hipMalloc(&m->wi[k], sizeof(int32_t) * (N + 64)));
+  for (int k = 0; k < 2; ++k) GOSPL_CHECK(hipMalloc(&m->wu[k], sizeof(unsigned long long) * (N + 64)));
+  for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
+  GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
+  GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
+  GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
+  // rocPRIM workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
+  size_t bytes = 0, b2 = 0;
+  if (scan_tmp_bytes(N + 1, &bytes)) return 1;
+  if (pit_tmp_bytes(N, &b2)) return 1;
+  if (b2 > bytes) bytes = b2;
+  m->cub_tmp_bytes = bytes + 256;
+  GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+  if (gospl_dag_init_occupancy(m)) return 1;
+  if (gospl_ensure_coef(m, 3)) return 1;
+  return 0;
+}
+
+}  // namespace gospl
+
+extern "C" {
+
 int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, const int32_t* deg,
                       const double* elgt, const double* vdist, const double* area,
                       const int32_t* gid, const int8_t* outlet, gospl_mesh_t** out) {
@@ -161,24 +221,7 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
     set_error("gospl_mesh_create: bad arguments (N=%lld, W=%d)", (long long)N, (int)W);
     return 1;
   }
-  // The receiver-DAG sweeps find a donor's row entry through the receiver's
-  // neighbour list, so every neighbour relation must be mutual (as in a
-  // triangulation whose vertices have at most GOSPL_MAXW neighbours).
-  {
-    int32_t* asym = nullptr;
-    GOSPL_CHECK(hipMalloc(&asym, sizeof(int32_t)));
-    GOSPL_CHECK(hipMemset(asym, 0, sizeof(int32_t)));
-    k_nbr_symmetry<<<nblocks(N), GOSPL_BLOCK>>>(N, W, nbr, deg, asym);
-    GOSPL_CHECK(hipGetLastError());
-    int32_t h = 0;
-    GOSPL_CHECK(hipMemcpy(&h, asym, sizeof(int32_t), hipMemcpyDeviceToHost));
-    GOSPL_CHECK(hipFree(asym));
-    if (h) {
-      set_error("gospl_mesh_create: %d neighbour relations are not mutual (a vertex with more than "
-                "%d neighbours?)", (int)h, GOSPL_MAXW);
-      return 1;
-    }
-  }
+  if (check_symmetry(N, W, nbr, deg, "gospl_mesh_create")) return 1;
   gospl_mesh* m = new gospl_mesh();
   std::memset(m, 0, sizeof(*m));
   m->N = N;
@@ -191,26 +234,10 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
   m->area = area;
   m->gid = gid;
   m->outlet = outlet;
-  GOSPL_CHECK(hipGetDevice(&m->device));
-  hipDeviceProp_t prop;
-  GOSPL_CHECK(hipGetDeviceProperties(&prop, m->device));
-  m->num_cu = prop.multiProcessorCount;
-  for (int k = 0; k < 8; ++k) GOSPL_CHECK(hipMalloc(&m->wi[k], sizeof(int32_t) * (N + 64)));
-  for (int k = 0; k < 2; ++k) GOSPL_CHECK(hipMalloc(&m->wu[k], sizeof(unsigned long long) * (N + 64)));
-  for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
-  GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
-  GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
-  GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
-  // hipcub workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
-  size_t bytes = 0, b2 = 0;
-  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, m->wi[0], m->wi[1], (int)(N + 1)));
-  if (pit_tmp_bytes(N, &b2)) return 1;
-  if (b2 > bytes) bytes = b2;
-  m->cub_tmp_bytes = bytes + 256;
-  GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
-
-  if (gospl_dag_init_occupancy(m)) return 1;
-  if (gospl_ensure_coef(m, 3)) return 1;
+  if (mesh_alloc(m)) {
+    gospl_mesh_destroy(m);  // releases what was allocated before the failure
+    return 1;
+  }
   *out = m;
   return 0;
 }
@@ -228,6 +255,13 @@ int gospl_mesh_touch(gospl_mesh_t* m) {
   if (!m) {
     set_error("gospl_mesh_touch: null handle");
     return 1;
+  }
+  // the lent neighbour table may have changed: check it again, and drop the
+  // reverse-neighbour positions built from it (rebuilt by the next sweep)
+  if (check_symmetry(m->N, m->W, m->nbr, m->deg, "gospl_mesh_touch")) return 1;
+  if (m->rev) {
+    GOSPL_CHECK(hipFree(m->rev));
+    m->rev = nullptr;
   }
   m->hvalid = 0;
   return 0;

@@ -73,7 +73,7 @@ struct gospl_mesh {
   hipEvent_t tev[2];
   double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep
   int64_t t_calls[2];
-  void* cub_tmp;
+  void* cub_tmp;              // rocPRIM workspace (scans, selects, radix sorts)
   size_t cub_tmp_bytes;
 };
 
@@ -82,7 +82,8 @@ namespace gospl {
 void set_error(const char* fmt, ...);
 
 // depression grouping (k_pits.hip)
-int pit_tmp_bytes(int64_t N, size_t* bytes);  // hipcub workspace of the grouping
+int pit_tmp_bytes(int64_t N, size_t* bytes);  // rocPRIM workspace of the grouping
+int scan_tmp_bytes(int64_t n, size_t* bytes);  // rocPRIM workspace of an int32 exclusive scan
 int pit_cells(gospl_mesh* m, const int32_t* pitids, int32_t* cells, int32_t* ncells, hipStream_t s);
 int ensure_groups(gospl_mesh* m, int64_t rows);  // grouping buffers for `rows` pits
 // `cleared`: the caller has already zeroed the segment table (2 * (pg_cap + 64) ints)
@@ -128,6 +129,15 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k) {
   unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
   return __longlong_as_double((long long)b);
 }
+inline double dkey_inv_host(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double d;
+  __builtin_memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+// bound of the hillslope solve's iterate-to-convergence mode
+#define GOSPL_HILL_MAX_SWEEPS 1000000
 
 // Append `value` for every lane with `pred` set, one atomicAdd per wavefront.
 // Must be reached by all lanes of the wavefront.

@@ -25,7 +25,7 @@
 #include <cstdlib>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "common.h"
 
@@ -78,11 +78,21 @@ __global__ void k_dep_fill(int64_t N, int W, int NR, const int32_t* __restrict__
 
 // Pending counters.  Level 0 (pend == 0) is not listed: k_dag_level0 scans
 // all vertices, which avoids a million appends on one counter.
-__global__ void k_pending(int mode, int64_t N, int NR, const int32_t* __restrict__ rcv,
+__global__ void k_pending(int mode, int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                          const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
                           const double* __restrict__ coef, const int32_t* __restrict__ doff,
-                          int32_t* __restrict__ pend) {
+                          int32_t* __restrict__ pend, int32_t* __restrict__ nonnbr) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
+  // a dependency on a vertex outside the neighbour list has no row entry
+  const int dg = deg[i];
+  for (int k = 0; k < NR; ++k) {
+    const int r = rcv[(int64_t)k * N + i];
+    if (r == (int)i || r < 0 || coef[(int64_t)k * N + i] == 0.0) continue;
+    bool found = false;
+    for (int p = 0; p < W && p < dg; ++p) found = found || nbr[(int64_t)p * N + i] == r;
+    if (!found) atomicAdd(nonnbr, 1);
+  }
   int c = 0;
   if (mode == 0) {
     c = doff[i + 1] - doff[i];
@@ -446,7 +456,8 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
                              const int32_t* __restrict__ rcv, const double* __restrict__ coef,
                              const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
-                             double* __restrict__ oacc, int32_t* __restrict__ pend) {
+                             double* __restrict__ oacc, int32_t* __restrict__ pend,
+                             int32_t* __restrict__ nonnbr) {
   __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
@@ -464,7 +475,11 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
     w[k] = coef[(int64_t)k * N + i];
     const int rk = r[k];
     const int p = rpos[(int64_t)k * N + i];
-    if (p < 0) continue;
+    if (p < 0) {
+      // a dependency with no row entry: its receiver is not a neighbour
+      if (rk != (int)i && rk >= 0 && w[k] != 0.0) atomicAdd(nonnbr, 1);
+      continue;
+    }
     const int c = __popcll(rowmask[rk] & ((1ull << (p * NR + k)) - 1ull));
     sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
                          : (int)(-2 - ((int64_t)(c - REC_NACC) * N + rk));
@@ -491,10 +506,11 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
 // entries past the record go to `odon`).
 // pend[i] = number of receivers i depends on.
 __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
-                             const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-                             const double* __restrict__ diag, const double* __restrict__ rhs,
-                             const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
-                             int32_t* __restrict__ odon, int32_t* __restrict__ pend) {
+                             const int8_t* __restrict__ rpos, const int32_t* __restrict__ rcv,
+                             const double* __restrict__ coef, const double* __restrict__ diag,
+                             const double* __restrict__ rhs, const unsigned long long* __restrict__ rowmask,
+                             double* __restrict__ rec, int32_t* __restrict__ odon,
+                             int32_t* __restrict__ pend, int32_t* __restrict__ nonnbr) {
   __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
@@ -505,7 +521,12 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
   double o[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < NR && k < 3; ++k) {
     o[k] = coef[(int64_t)k * N + i];
-    if (rcv[(int64_t)k * N + i] != (int)i && o[k] != 0.0) mask |= 1 << k;
+    const int rk = rcv[(int64_t)k * N + i];
+    if (rk != (int)i && o[k] != 0.0) {
+      mask |= 1 << k;
+      // its value would never be pushed: the receiver is not a neighbour
+      if (rpos[(int64_t)k * N + i] < 0) atomicAdd(nonnbr, 1);
+    }
   }
   int t[REC_NTGT];
 #pragma unroll
@@ -1378,10 +1399,11 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos,
                                          zl);
     if (mode == 0)
-      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc, pend);
+      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc, pend,
+                                              dev + 3);
     else
-      k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rcv, coef, diag, rhs, rowmask, m->wrec,
-                                              m->wdon, pend);
+      k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rpos, rcv, coef, diag, rhs, rowmask, m->wrec,
+                                              m->wdon, pend, dev + 3);
   } else {
     // CSR donor rows
     if (ensure_donors(m, (int64_t)nrcv * N)) return 1;
@@ -1389,9 +1411,10 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
     k_dep_count<<<nblocks(N + 1), GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, dcnt);
     size_t bytes = m->cub_tmp_bytes;
-    GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, dcnt, doff, (int)(N + 1), s));
+    GOSPL_CHECK(rocprim::exclusive_scan(m->cub_tmp, bytes, dcnt, doff, 0, (size_t)(N + 1),
+                                        rocprim::plus<int32_t>(), s));
     k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
-    k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, nrcv, rcv, coef, doff, pend);
+    k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, pend, dev + 3);
   }
   GOSPL_CHECK(hipGetLastError());
   DagArgs a{N, nrcv, rcv, coef, diag, rhs, x, doff, m->wdon, m->wacc, slot, pend,
@@ -1459,7 +1482,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* left = fa;  // the frontier lists are free again
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
-  GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
+  GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
   // abort word
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, chain ? m->chain_ctl + CQ_ABORT : (int32_t*)(bar + 2),
                              sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -1469,6 +1492,11 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (chain) GOSPL_CHECK(hipMemsetAsync(gq, 0xff, sizeof(int32_t) * gq_cap * 64, s));
     set_error(chain ? "dataflow sweep aborted: a bounded wait ran out"
                     : "receiver-DAG sweep aborted: a grid barrier did not complete");
+    return 1;
+  }
+  if (m->hflag[3] > 0) {
+    set_error("%d receivers are not neighbours of their donor (each receiver must be in the donor's "
+              "neighbour list)", m->hflag[3]);
     return 1;
   }
   if (mode == 0 && m->hflag[1] > 0) {

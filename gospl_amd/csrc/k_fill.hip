@@ -28,7 +28,9 @@
 // breadth-first distance; per-pit sums in a fixed order (k_pits.hip).
 #include <climits>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "common.h"
 
@@ -613,8 +615,8 @@ static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel l
     k_basin_boundary<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, state, root, bflag);
     GOSPL_CHECK(hipGetLastError());
     size_t bytes = m->cub_tmp_bytes;
-    hipcub::CountingInputIterator<int32_t> cit(0);
-    GOSPL_CHECK(hipcub::DeviceSelect::Flagged(m->cub_tmp, bytes, cit, bflag, blist, bcount, (int)N, s));
+    rocprim::counting_iterator<int32_t> cit(0);
+    GOSPL_CHECK(rocprim::select(m->cub_tmp, bytes, cit, bflag, blist, bcount, (size_t)N, s));
   }
   // one list entry per thread; the length stays on the device (workgroups past it return)
   const int gb = g;
@@ -665,7 +667,7 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   k_pit_mark<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, cmin, cnon, mark);
   GOSPL_CHECK(hipGetLastError());
   size_t bytes = m->cub_tmp_bytes;
-  GOSPL_CHECK(hipcub::DeviceScan::ExclusiveSum(m->cub_tmp, bytes, mark, rank, (int)N, s));
+  GOSPL_CHECK(rocprim::exclusive_scan(m->cub_tmp, bytes, mark, rank, 0, (size_t)N, rocprim::plus<int32_t>(), s));
   // npits = rank[N-1] + mark[N-1]
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 16, rank + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 17, mark + (N - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -106,6 +106,65 @@ __global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
   xout[i] = s / d;
 }
 
+// The sweep of k_hill_sweep that also records max |xout - xin| and max
+// |xout| (order-preserving keys, one atomic per workgroup): the convergence
+// test of the iterate-to-convergence mode.
+__global__ void k_hill_sweep_chk(int64_t N, int W, const int32_t* __restrict__ nbr,
+                                 const int32_t* __restrict__ deg, const double* __restrict__ coef,
+                                 const double* __restrict__ diag, const double* __restrict__ rhs,
+                                 const double* __restrict__ xin, double* __restrict__ xout,
+                                 unsigned long long* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double chg = 0.0, mag = 0.0;
+  if (i < N) {
+    const double d = diag[i];
+    const double r = rhs[i];
+    double x = r;
+    if (d != 0.0) {
+      double s = r;
+      const int dg = deg[i];
+      for (int p = 0; p < W && p < dg; ++p) {
+        const double c = coef[(int64_t)p * N + i];
+        if (c != 0.0) s = s - c * xin[nbr[(int64_t)p * N + i]];
+      }
+      x = s / d;
+    }
+    xout[i] = x;
+    chg = fabs(x - xin[i]);
+    mag = fabs(x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    chg = fmax(chg, __shfl_xor(chg, o, 64));
+    mag = fmax(mag, __shfl_xor(mag, o, 64));
+  }
+  __shared__ double sh[2][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = chg;
+    sh[1][threadIdx.x >> 6] = mag;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      chg = fmax(chg, sh[0][w]);
+      mag = fmax(mag, sh[1][w]);
+    }
+    atomicMax(&keys[0], dkey(chg));
+    atomicMax(&keys[1], dkey(mag));
+  }
+}
+
+// max over the rows of the Jacobi contraction (diag - 1) / diag = s / (1 + s).
+__global__ void k_hill_rho(int64_t N, const double* __restrict__ diag, unsigned long long* __restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (i < N) {
+    const double d = diag[i];
+    if (d > 0.0) r = (d - 1.0) / d;
+  }
+  for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+  if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(key, dkey(r));
+}
+
 __global__ void k_hill_update(int64_t N, const double* __restrict__ x, double dt, double* __restrict__ h,
                               double* __restrict__ cumed, double* __restrict__ eblocal) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -189,13 +248,54 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
                                                    diag);
     GOSPL_CHECK(hipGetLastError());
   }
-  if (sweeps < 1) sweeps = 1;
   const double* xin = h;
   double* xout = xa;
-  for (int k = 0; k < sweeps; ++k) {
-    k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout);
-    xin = xout;
-    xout = (xout == xa) ? xb : xa;
+  if (sweeps > 0) {
+    // the count the caller derived from the contraction bound
+    for (int k = 0; k < sweeps; ++k) {
+      k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout);
+      xin = xout;
+      xout = (xout == xa) ? xb : xa;
+    }
+  } else {
+    // iterate to convergence: with contraction rho the error after a sweep
+    // is at most rho / (1 - rho) times its change, so stop once that bound
+    // is below 2^-52 of the largest value; chunks of 32 sweeps, the last one
+    // measuring (one read-back per chunk)
+    unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
+    GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 3, s));
+    k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, keys + 2);
+    GOSPL_CHECK(hipGetLastError());
+    unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
+    GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    const double rho = hk[2] ? dkey_inv_host(hk[2]) : 0.0;
+    const double factor = rho < 1.0 ? rho / (1.0 - rho) : 1.0e300;
+    int64_t done = 0;
+    bool conv = false;
+    while (!conv && done < GOSPL_HILL_MAX_SWEEPS) {
+      for (int k = 0; k < 31; ++k) {
+        k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout);
+        xin = xout;
+        xout = (xout == xa) ? xb : xa;
+      }
+      GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
+      k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout, keys);
+      GOSPL_CHECK(hipGetLastError());
+      xin = xout;
+      xout = (xout == xa) ? xb : xa;
+      done += 32;
+      GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+      GOSPL_CHECK(hipStreamSynchronize(s));
+      const double chg = dkey_inv_host(hk[0]), mag = dkey_inv_host(hk[1]);
+      if (!(chg == chg) || !(mag == mag)) break;  // NaN: not converging
+      conv = chg * factor <= 0x1p-52 * mag;
+    }
+    if (!conv) {
+      set_error("gospl_hillslope: Jacobi sweeps did not converge within %lld sweeps (contraction %.6g)",
+                (long long)GOSPL_HILL_MAX_SWEEPS, rho);
+      return 1;
+    }
   }
   k_hill_update<<<g, GOSPL_BLOCK, 0, s>>>(N, xin, dt, h, cumed, eblocal);
   GOSPL_CHECK(hipGetLastError());

@@ -10,7 +10,10 @@
 // per-thread partial sums followed by a fixed LDS tree.  The result does not
 // depend on scheduling (no floating-point atomics), so runs are reproducible
 // bit for bit.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "common.h"
 
@@ -169,17 +172,24 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_pit_reduce_inflow(
   if (threadIdx.x == 0) inv[p] = v[0];
 }
 
+int scan_tmp_bytes(int64_t n, size_t* bytes) {
+  *bytes = 0;
+  GOSPL_CHECK(rocprim::exclusive_scan(nullptr, *bytes, (int32_t*)nullptr, (int32_t*)nullptr, 0, (size_t)n,
+                                      rocprim::plus<int32_t>()));
+  return 0;
+}
+
 int pit_tmp_bytes(int64_t N, size_t* bytes) {
   size_t b1 = 0, b2 = 0;
-  hipcub::CountingInputIterator<int32_t> it(0);
+  rocprim::counting_iterator<int32_t> it(0);
   // (the select's workspace does not depend on its predicate)
-  GOSPL_CHECK(hipcub::DeviceSelect::If(nullptr, b1, it, (int32_t*)nullptr, (int32_t*)nullptr, (int)N,
-                                       IsPitCell{nullptr}));
-  GOSPL_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, (unsigned long long*)nullptr,
-                                                (unsigned long long*)nullptr, (int)N, 32, 64));
+  GOSPL_CHECK(rocprim::select(nullptr, b1, it, (int32_t*)nullptr, (int32_t*)nullptr, (size_t)N,
+                              IsPitCell{nullptr}));
+  GOSPL_CHECK(rocprim::radix_sort_keys(nullptr, b2, (unsigned long long*)nullptr,
+                                       (unsigned long long*)nullptr, (size_t)N, 32, 64));
   size_t b3 = 0;
-  GOSPL_CHECK(hipcub::DeviceSelect::Flagged(nullptr, b3, it, (const int8_t*)nullptr, (int32_t*)nullptr,
-                                            (int32_t*)nullptr, (int)N));
+  GOSPL_CHECK(rocprim::select(nullptr, b3, it, (const int8_t*)nullptr, (int32_t*)nullptr,
+                              (int32_t*)nullptr, (size_t)N));
   *bytes = b1 > b2 ? b1 : b2;
   if (b3 > *bytes) *bytes = b3;
   return 0;
@@ -187,9 +197,8 @@ int pit_tmp_bytes(int64_t N, size_t* bytes) {
 
 int pit_cells(gospl_mesh* m, const int32_t* pitids, int32_t* cells, int32_t* ncells, hipStream_t s) {
   size_t bytes = m->cub_tmp_bytes;
-  hipcub::CountingInputIterator<int32_t> it(0);
-  GOSPL_CHECK(hipcub::DeviceSelect::If(m->cub_tmp, bytes, it, cells, ncells, (int)m->N,
-                                       IsPitCell{pitids}, s));
+  rocprim::counting_iterator<int32_t> it(0);
+  GOSPL_CHECK(rocprim::select(m->cub_tmp, bytes, it, cells, ncells, (size_t)m->N, IsPitCell{pitids}, s));
   return 0;
 }
 
@@ -256,8 +265,8 @@ int pit_group(gospl_mesh* m, const int32_t* pitids, int64_t npits, const int32_t
   int bits = 1;
   while (bits < 31 && ((int64_t)1 << bits) <= npits) ++bits;
   size_t bytes = m->cub_tmp_bytes;
-  GOSPL_CHECK(hipcub::DeviceRadixSort::SortKeys(m->cub_tmp, bytes, m->pg_keys[1], m->pg_keys[0],
-                                                (int)ncells, 32, 32 + bits, s));
+  GOSPL_CHECK(rocprim::radix_sort_keys(m->cub_tmp, bytes, m->pg_keys[1], m->pg_keys[0], (size_t)ncells, 32,
+                                       32 + bits, s));
   k_pg_segments<<<g, GOSPL_BLOCK, 0, s>>>(ncells, m->pg_keys[0], start, end);
   GOSPL_CHECK(hipGetLastError());
   return 0;

@@ -145,8 +145,15 @@ def mfdreceivers(nrcv, exp, elev, sl, gid):
     dev = _device()
     N = _state["N"]
     P = dev["P"]
-    e = torch.as_tensor(np.ascontiguousarray(elev, dtype=np.float64), device="cuda")
-    dev["gid"].copy_(torch.as_tensor(np.ascontiguousarray(gid, dtype=np.int32)))
+    elev = np.ascontiguousarray(elev, dtype=np.float64).ravel()
+    gid = np.ascontiguousarray(gid, dtype=np.int32).ravel()
+    # the kernel reads N entries of each: refuse shorter arrays before any launch
+    if len(elev) != N or len(gid) != N:
+        raise ValueError("mfdreceivers: len(elev)=%d, len(gid)=%d for a %d-vertex mesh" % (len(elev), len(gid), N))
+    if not 1 <= int(nrcv) <= 12:
+        raise ValueError("mfdreceivers: nrcv=%d outside 1..12" % int(nrcv))
+    e = torch.as_tensor(elev, device="cuda")
+    dev["gid"].copy_(torch.as_tensor(gid))
     rcv = torch.empty((nrcv, N), dtype=torch.int32, device="cuda")
     dist = torch.empty((nrcv, N), dtype=torch.float64, device="cuda")
     wgt = torch.empty_like(dist)
@@ -163,9 +170,10 @@ def sethillslopecoeff(nb, kd):
     torch = _torch()
     dev = _device()
     N = _state["N"]
-    if nb != N:
-        raise ValueError("sethillslopecoeff: nb=%d for a %d-vertex mesh" % (nb, N))
-    k = torch.as_tensor(np.ascontiguousarray(kd, dtype=np.float64), device="cuda")
+    kd = np.ascontiguousarray(kd, dtype=np.float64).ravel()
+    if nb != N or len(kd) != N:
+        raise ValueError("sethillslopecoeff: nb=%d, len(kd)=%d for a %d-vertex mesh" % (nb, len(kd), N))
+    k = torch.as_tensor(kd, device="cuda")
     out = torch.empty((13, N), dtype=torch.float64, device="cuda")
     s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     _lib.check(_lib.load().gospl_hillslope_coeff(dev["handle"], dev["P"](k), dev["P"](out), s),

@@ -233,13 +233,17 @@ class Model:
         return k * self.rainVal ** self.coeffd
 
     def hillslope_sweeps(self):
+        """Jacobi sweeps that bring the error below 2^-60 of max|h| by the
+        contraction bound rho = s/(1+s), s = Kd*dt*gmax; 0 (the library then
+        iterates to a device-checked convergence) when that takes more than
+        64 sweeps."""
         kd = max(self.Cda, self.Cdm) * self.dt
         s = kd * self.dm.gmax
         if s <= 0.0:
             return 1
         rho = s / (1.0 + s)
         n = int(math.ceil(math.log(2.0 ** -60) / math.log(rho))) + 1
-        return max(2, min(n, 500))
+        return max(2, n) if n <= 64 else 0
 
     # -------------------------------------------------------------- pits
     def fillElevation(self, sed=False):
@@ -438,6 +442,13 @@ class Model:
             if self.tNow == self.tEnd:
                 break
             self.step()
+
+    def close(self):
+        """Release the library handle now (its device and pinned buffers and
+        events) instead of at interpreter exit, when the HIP runtime may be
+        shutting down already."""
+        torch.cuda.synchronize(self.device)
+        self.dm.close()
 
     # -------------------------------------------------------------- views
     def input_order(self, t):

@@ -182,8 +182,12 @@ int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const doubl
  * (gospl/sed/hillslope.py:171-233, 614-649) with `sethillslopecoeff`
  * (functions.pyf:10-14, functions.F90:901-938) and its Richardson/ILU(0)
  * solve.  Diffusivity cda on land, cdm at sea (h <= sealevel), times dt.
- * Updates h, cumed and eblocal (+= dh/dt).  `sweeps` Jacobi sweeps (the
- * Python host derives the count from the operator's contraction bound).
+ * Updates h, cumed and eblocal (+= dh/dt).  sweeps > 0: that many Jacobi
+ * sweeps (the Python host derives the count from the operator's contraction
+ * bound rho = max s/(1+s)); sweeps <= 0: sweeps until the error bound
+ * rho/(1-rho) * max|change| is below 2^-52 * max|h| (checked on the device
+ * every 32 sweeps, one read-back each), failing past 10^6 sweeps, where goSPL
+ * stops its Richardson/ILU(0) solve at rtol 1e-8 (hillslope.py:347-371).
  * The operator rows are kept in the handle between calls: a call rebuilds
  * the rows where the sea flag of the vertex or of a neighbour changed, all
  * of them when cda, cdm or dt change (goSPL keeps its operator until the

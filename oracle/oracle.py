@@ -171,11 +171,15 @@ def spl_solve(rcv, off, diag, rhs):
     return x, int(bad)
 
 
-def hillslope_solve(coeff, rhs, ngb, nnb, maxit=200):
+def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
+    """Jacobi sweeps to the bitwise fixed point; raises when ``maxit`` sweeps
+    do not reach it (goSPL's Richardson solve would stop at rtol 1e-8)."""
     N = len(rhs)
     x = np.empty(N)
     it = lib().or_hillslope_solve(N, _p(ngb), _p(nnb), _p(_c(coeff, np.float64)), _p(_c(rhs, np.float64)),
                                   _p(x), int(maxit))
+    if it > maxit:
+        raise RuntimeError("oracle hillslope Jacobi did not reach its fixed point in %d sweeps" % maxit)
     return x, int(it)
 
 

@@ -44,6 +44,7 @@ SIGNATURES = {
     "gospl_cascade_seed": (I32, [P, P, I64, I64, P, P, P, F64, P, P]),
     "gospl_cascade_add": (I32, [P, P, P, P, F64, P, P, P]),
     "gospl_flow_finalize": (I32, [P, P, P, P, I32, P, P, P]),
+    "gospl_flow_finalize_noalloc": (I32, [I64, P, P, P, I32, P, P, P]),
     "gospl_sweep_timing": (I32, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64),
                                  ctypes.POINTER(F64), ctypes.POINTER(I64)]),
 }

@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 
 #include "common.h"
@@ -127,6 +128,83 @@ __global__ void k_finalize_b(int64_t N, const double* __restrict__ hl, const dou
   if (has_pits && hl[i] <= water[i] && pit[i] > -1) g = dkey_inv(*maxkey) * 0.1;
   fal[i] = f < 0.0 ? 0.0 : f;
   fillfal[i] = g < 0.0 ? 0.0 : g;
+}
+
+// gospl_flow_finalize without a handle or any allocation: the discharge
+// maximum goes through `scratch`, one partial maximum (order-preserving key)
+// per workgroup of k_fin_partial, each workgroup striding over the mesh; every
+// call writes all the slots it reads, so nothing needs clearing.
+#define NOALLOC_PARTIALS 1024
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_PARTIALS];
+
+__global__ void k_fin_partial(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
+                              const int32_t* __restrict__ pit, int has_pits, double* __restrict__ fal,
+                              unsigned long long* __restrict__ part) {
+  double v = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    double f = fal[i];
+    if (has_pits && hl[i] <= water[i] && pit[i] > -1) f = 0.0;
+    fal[i] = f;
+    v = v > f ? v : f;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o, 64);
+    v = v > w ? v : w;
+  }
+  __shared__ double s_max[GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_max[0];
+    for (int k = 1; k < GOSPL_BLOCK / 64; ++k) b = b > s_max[k] ? b : s_max[k];
+    part[blockIdx.x] = dkey(b);
+  }
+}
+
+__global__ void k_fin_apply(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
+                            const int32_t* __restrict__ pit, int has_pits,
+                            const unsigned long long* __restrict__ part, int nparts,
+                            double* __restrict__ fal, double* __restrict__ fillfal) {
+  __shared__ unsigned long long s_key[GOSPL_BLOCK / 64];
+  unsigned long long k = 0ull;
+  for (int t = threadIdx.x; t < nparts; t += blockDim.x) k = k > part[t] ? k : part[t];
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(k, o, 64);
+    k = k > w ? k : w;
+  }
+  if ((threadIdx.x & 63) == 0) s_key[threadIdx.x >> 6] = k;
+  __syncthreads();
+  for (int w = 0; w < GOSPL_BLOCK / 64; ++w) k = k > s_key[w] ? k : s_key[w];
+  const double fmaxv = dkey_inv(k);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double f = fal[i];
+  double g = f;
+  if (has_pits && hl[i] <= water[i] && pit[i] > -1) g = fmaxv * 0.1;
+  fal[i] = f < 0.0 ? 0.0 : f;
+  fillfal[i] = g < 0.0 ? 0.0 : g;
+}
+
+// Address of `scratch` on each device, looked up once per device with that
+// device current (the host-side name is not a device address).
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static int scratch_on_device(void** out) {
+  int dev = 0;
+  GOSPL_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("gospl_flow_finalize_noalloc: device %d outside 0..63", dev);
+    return 1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GOSPL_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
 }
 
 // Counts in *asym the neighbour relations that are not mutual.
@@ -335,6 +413,24 @@ int gospl_cascade_seed(gospl_mesh_t* m, const double* stage, int64_t npit, int64
 int gospl_cascade_add(gospl_mesh_t* m, const double* q, const double* hl, const double* water,
                       double dt, double* nfa, double* fal, gospl_stream_t stream) {
   k_cascade_add<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, q, hl, water, dt, nfa, fal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_flow_finalize_noalloc(int64_t N, const double* hl, const double* water, const int32_t* pitids,
+                                int32_t has_pits, double* fal, double* fillfal, gospl_stream_t stream) {
+  if (N <= 0 || N > (int64_t)INT32_MAX) {
+    set_error("gospl_flow_finalize_noalloc: N=%lld outside 1..%d", (long long)N, INT32_MAX);
+    return 1;
+  }
+  void* part = nullptr;
+  if (scratch_on_device(&part)) return 1;
+  const int g = nblocks(N) < NOALLOC_PARTIALS ? nblocks(N) : NOALLOC_PARTIALS;
+  k_fin_partial<<<g, GOSPL_BLOCK, 0, stream>>>(N, hl, water, pitids, has_pits, fal,
+                                               static_cast<unsigned long long*>(part));
+  k_fin_apply<<<nblocks(N), GOSPL_BLOCK, 0, stream>>>(N, hl, water, pitids, has_pits,
+                                                      static_cast<const unsigned long long*>(part), g, fal,
+                                                      fillfal);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

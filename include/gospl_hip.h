@@ -254,6 +254,20 @@ int gospl_flow_finalize(gospl_mesh_t* mesh, const double* hl, const double* wate
                         double* fillfal, gospl_stream_t stream);
 
 /*
+ * gospl_flow_finalize for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops): no mesh handle, and no hipMalloc, hipFree or
+ * allocating library call.  No counterpart in goSPL.  Same inputs, outputs and
+ * results (bit for bit) as gospl_flow_finalize on an N-vertex mesh; N must be
+ * in 1..2^31-1 (the call fails otherwise).  Its scratch is a fixed 8 KiB
+ * device array (1024 partial maxima) that each GPU holds until the process
+ * ends and that every call on that GPU shares: calls that overlap on one GPU
+ * (different streams or threads) race.
+ */
+int gospl_flow_finalize_noalloc(int64_t N, const double* hl, const double* water,
+                                const int32_t* pitids, int32_t has_pits, double* fal,
+                                double* fillfal, gospl_stream_t stream);
+
+/*
  * Instrumentation (no counterpart in goSPL): when enabled, HIP events on the
  * launch stream bracket every DAG sweep kernel (flow accumulation and SPL).
  * Returns the accumulated milliseconds and call counts since the previous

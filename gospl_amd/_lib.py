@@ -36,6 +36,10 @@ SIGNATURES = {
     "gospl_flowacc": (I32, [P, I32, P, P, P, P, ctypes.POINTER(I32), ctypes.POINTER(I64), P]),
     "gospl_spl": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P,
                         ctypes.POINTER(I32), P]),
+    "gospl_flowacc_refine": (I32, [P, I32, P, P, P, P, F64, I32, ctypes.POINTER(I32), ctypes.POINTER(F64),
+                                   ctypes.POINTER(I64), P]),
+    "gospl_spl_refine": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P, F64, I32,
+                               ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
     "gospl_hillslope": (I32, [P, F64, F64, F64, F64, I32, P, P, P, P]),
     "gospl_hillslope_coeff": (I32, [P, P, P, P]),
     "gospl_rain_source": (I32, [P, P, P, F64, P, P]),

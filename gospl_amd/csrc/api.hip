@@ -375,6 +375,8 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->pg_seg);
   dfree(m->pack_buf);
   dfree(m->hcoef);
+  dfree(m->rx_ctl);
+  if (m->rx_hctl) (void)hipHostFree(m->rx_hctl);
   dfree(m->hdiag);
   dfree(m->hsea);
   delete m;

@@ -71,9 +71,12 @@ struct gospl_mesh {
   int64_t pack_cap;           // its capacity (doubles)
   int timing;                 // record HIP events around the DAG sweep kernels
   hipEvent_t tev[2];
-  double t_ms[2];             // [0] flow-accumulation sweep, [1] SPL sweep
-  int64_t t_calls[2];
+  double t_ms[4];             // [0] flow-accumulation sweep, [1] SPL sweep, [2], [3] their refined solves
+  int64_t t_calls[4];
   void* cub_tmp;              // rocPRIM workspace (scans, selects, radix sorts)
+  double* rx_ctl;             // refined solves: control block (device, 64 doubles)
+  double* rx_hctl;            // its pinned host mirror
+  int rx_part_cap;            // partial-sum slots available in wd[0]
   size_t cub_tmp_bytes;
 };
 
@@ -91,6 +94,26 @@ int pit_group(gospl_mesh* m, const int32_t* pitids, int64_t npits, const int32_t
               const int32_t* ncells_dev, int64_t ncells, hipStream_t s, bool cleared = false);
 int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double sl, int64_t cap,
                      double* pitparams, double* filled_lvl, double* filled_vol, hipStream_t s);
+
+// receiver-DAG workspace (k_dag.hip): packed records + reverse neighbour
+// positions; donor rows / overflow entries of capacity `cap`
+int ensure_records(gospl_mesh* m, hipStream_t s);
+int ensure_donors(gospl_mesh* m, int64_t cap);
+
+// Refined solve on the receiver DAG (k_refine.hip): x (in: initial guess,
+// out: solution) of the flow system (mode 0: (I - W^T) x = rhs, coef = W) or
+// the stream-power system (mode 1: diag x + off x_rcv = rhs, coef = off) by
+// correction sweeps in 64-bit fixed point and FP64 residuals, until
+// ||rhs - A x||_2 <= rtol ||rhs||_2 or maxpass corrections.
+struct RefineStatus {
+  int passes;        // correction sweeps run
+  double relres;     // final ||r||_2 / ||rhs||_2
+  int fallback;      // the sweep left vertices unsolved (receiver cycle) or a value overflowed
+  int aborted;       // a bounded wait of the dataflow kernel ran out
+};
+int refine_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
+                 const double* diag, const double* rhs, double* x, double rtol, int maxpass,
+                 RefineStatus* st, hipStream_t s);
 }  // namespace gospl
 
 extern "C" {

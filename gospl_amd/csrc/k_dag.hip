@@ -27,7 +27,7 @@
 
 #include <rocprim/device/device_scan.hpp>
 
-#include "common.h"
+#include "dag_common.h"
 
 namespace gospl {
 
@@ -381,13 +381,6 @@ __global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
   }
 }
 
-// Small int32 arrays cleared by the first workgroup of a launch.
-struct ZeroList {
-  int32_t* p[4];
-  int len[4];
-  int n;
-};
-
 // Row masks: receiver r's row is the bitmask over (neighbour position p,
 // receiver slot k), bit p * NR + k, of the donors that send to r through
 // their slot k; bit order is row order (FV neighbour order, then slot).  Set
@@ -662,19 +655,6 @@ __device__ __forceinline__ void solve_vertex(bool valid, int v, const DagArgs& a
   else process_vertex<MODE, Append, WG>(valid, v, a, app);
 }
 
-// Appends into a frontier list in global memory; `blk` (LDS, may be null)
-// accumulates the workgroup's appended count.
-struct GlobalAppend {
-  int32_t* cnt;
-  int32_t* out;
-  int32_t* blk;
-  __device__ __forceinline__ int reserve(int n) const {
-    if (blk) atomicAdd(blk, n);
-    return atomicAdd(cnt, n);
-  }
-  __device__ __forceinline__ void write(int idx, int value) const { out[idx] = value; }
-};
-
 // Level 0 (vertices without dependencies, typically a third of the mesh) on
 // the whole GPU before the persistent sweep takes over at level 1.  The
 // dependency count is recomputed from the rows (pend[] is being decremented
@@ -840,39 +820,6 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_dag_tail(
 // after CHAIN_SPIN_MAX idle passes); work is only ever held by a running
 // wavefront, so leaving early costs parallelism, never vertices.
 // ---------------------------------------------------------------------------
-#define CHAIN_THREADS 256
-#define CHAIN_POOL 512
-#define CHAIN_SPIN_MAX (1 << 21)
-#define CHAIN_POLLERS 32  // idle wavefronts that stay to poll the queue (256: a few % slower)
-// control words, one 128-byte line each
-#define CQ_QUEUE 0    // 64-bit: batches reserved << 32 | batches claimed
-#define CQ_ACTIVE 32  // wavefronts holding work
-#define CQ_IDLERS 64  // idle wavefronts polling the queue
-#define CQ_ABORT 96
-#define CQ_WORDS 128
-#define CQ_RETRIES 112  // traced sweeps: passes that found a row incomplete
-
-__device__ __forceinline__ double ld_agent(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int ld_agent_i(const int32_t* p) {
-  return __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent_i(int32_t* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// LDS written by some lanes of a wavefront and read by others: order the
-// accesses (one wavefront's LDS operations complete in issue order).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // One wavefront's work state.  Wave-uniform except v, ci, next.
 struct ChainWave {
   int32_t* pool;
@@ -1330,7 +1277,7 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
-static int ensure_records(gospl_mesh* m, hipStream_t s) {
+int ensure_records(gospl_mesh* m, hipStream_t s) {
   if (m->wrec) return 0;
   GOSPL_CHECK(hipMalloc(&m->wrec, sizeof(double) * REC_W * (m->N + 64)));
   GOSPL_CHECK(hipMalloc(&m->rev, sizeof(int8_t) * m->W * (m->N + 64)));
@@ -1339,7 +1286,7 @@ static int ensure_records(gospl_mesh* m, hipStream_t s) {
   return 0;
 }
 
-static int ensure_donors(gospl_mesh* m, int64_t cap) {
+int ensure_donors(gospl_mesh* m, int64_t cap) {
   if (m->wdon_cap >= cap) return 0;
   if (m->wdon) GOSPL_CHECK(hipFree(m->wdon));
   if (m->wacc) GOSPL_CHECK(hipFree(m->wacc));
@@ -1691,12 +1638,15 @@ int gospl_sweep_timing(gospl_mesh_t* m, int32_t enable, double* fa_ms, int64_t* 
     GOSPL_CHECK(hipEventCreate(&m->tev[0]));
     GOSPL_CHECK(hipEventCreate(&m->tev[1]));
   }
-  if (fa_ms) *fa_ms = m->t_ms[0];
-  if (fa_calls) *fa_calls = m->t_calls[0];
-  if (spl_ms) *spl_ms = m->t_ms[1];
-  if (spl_calls) *spl_calls = m->t_calls[1];
-  m->t_ms[0] = m->t_ms[1] = 0.0;
-  m->t_calls[0] = m->t_calls[1] = 0;
+  // exact sweeps and refined solves of the same system are reported together
+  if (fa_ms) *fa_ms = m->t_ms[0] + m->t_ms[2];
+  if (fa_calls) *fa_calls = m->t_calls[0] + m->t_calls[2];
+  if (spl_ms) *spl_ms = m->t_ms[1] + m->t_ms[3];
+  if (spl_calls) *spl_calls = m->t_calls[1] + m->t_calls[3];
+  for (int k = 0; k < 4; ++k) {
+    m->t_ms[k] = 0.0;
+    m->t_calls[k] = 0;
+  }
   m->timing = enable ? 1 : 0;
   return 0;
 }
@@ -1709,6 +1659,78 @@ int gospl_flowacc(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const doubl
     return 1;
   }
   return dag_solve(m, 0, nrcv, rcv, wgt, nullptr, b, q, nlevels, unsolved, stream);
+}
+
+int gospl_flowacc_refine(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const double* wgt,
+                         const double* b, double* q, double rtol, int32_t maxpass, int32_t* passes,
+                         double* relres, int64_t* unsolved, gospl_stream_t stream) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_flowacc_refine: nrcv must be in 1..12");
+    return 1;
+  }
+  if (passes) *passes = 0;
+  if (relres) *relres = 0.0;
+  if (unsolved) *unsolved = 0;
+  if (nrcv <= 3 && m->W * nrcv <= 64) {
+    RefineStatus st{};
+    if (refine_solve(m, 0, nrcv, rcv, wgt, nullptr, b, q, rtol, maxpass, &st, stream)) return 1;
+    if (st.aborted) {
+      set_error("refined flow solve aborted: a bounded wait ran out");
+      return 1;
+    }
+    if (!st.fallback) {
+      if (passes) *passes = st.passes;
+      if (relres) *relres = st.relres;
+      return 0;
+    }
+  }
+  // more than three receivers, a receiver cycle or an overflow: the exact sweep
+  if (passes) *passes = -1;
+  return dag_solve(m, 0, nrcv, rcv, wgt, nullptr, b, q, nullptr, unsolved, stream);
+}
+
+int gospl_spl_refine(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                     const double* disti, const double* fal, const double* kfac, double K, double mexp,
+                     double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+                     double* cumed, double* eb, double* eblocal, double rtol, int32_t maxpass,
+                     int32_t* passes, double* relres, gospl_stream_t stream) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_spl_refine: nrcv must be in 1..12");
+    return 1;
+  }
+  if (gospl_ensure_coef(m, nrcv)) return 1;
+  double* offc = m->wcoef;
+  double* diag = m->wd[3];
+  double* x = m->wd[2];
+  if (passes) *passes = 0;
+  if (relres) *relres = 0.0;
+  k_spl_coef<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, nrcv, rcvi, wgti, disti, fal, kfac, K,
+                                                        mexp, dt, sealevel, lfill, h, offc, diag);
+  GOSPL_CHECK(hipGetLastError());
+  // the iterate starts at the old surface
+  GOSPL_CHECK(hipMemcpyAsync(x, h, sizeof(double) * m->N, hipMemcpyDeviceToDevice, stream));
+  bool exact = true;
+  if (nrcv <= 3 && m->W * nrcv <= 64) {
+    RefineStatus st{};
+    if (refine_solve(m, 1, nrcv, rcvi, offc, diag, h, x, rtol, maxpass, &st, stream)) return 1;
+    if (st.aborted) {
+      set_error("refined stream-power solve aborted: a bounded wait ran out");
+      return 1;
+    }
+    exact = st.fallback != 0;
+    if (!exact) {
+      if (passes) *passes = st.passes;
+      if (relres) *relres = st.relres;
+    }
+  }
+  if (exact) {
+    if (passes) *passes = -1;
+    if (dag_solve(m, 1, nrcv, rcvi, offc, diag, h, x, nullptr, nullptr, stream)) return 1;
+  }
+  k_spl_update<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, x, m->outlet, m->flat, lsink, dt, h,
+                                                          cumed, eb, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
 }
 
 int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,

@@ -20,6 +20,7 @@ cascade (a few thousand numbers) runs on the host as in goSPL.
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import math
 import time
@@ -92,11 +93,20 @@ class Model:
     ``dt``, ``rain`` (scalar or per-vertex m/yr), ``upsub`` (per-vertex m/yr
     or None), ``sedfac`` (per-vertex K factor or None), ``sealevel``,
     ``ofill`` (oFill), ``tstart``/``tend``.
+
+    ``refine=True`` selects the refined solves for flow accumulation and
+    stream-power erosion (``gospl_flowacc_refine`` / ``gospl_spl_refine``:
+    fixed-point correction sweeps and FP64 residuals, iterated to
+    ``||r|| <= refine_rtol ||rhs||``, at most ``refine_maxpass`` corrections;
+    the flow solves warm-start from their previous solutions as goSPL's KSP
+    does) and, for hillslope diffusion, FP32 Jacobi corrections of an FP64
+    residual.  State, fill, pits and receivers are FP64 / exact either way.
     """
 
     def __init__(self, mesh, h0, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
-                 tend=None, device="cuda", verbose=False):
+                 tend=None, device="cuda", verbose=False, refine=False, refine_rtol=1.0e-12,
+                 refine_maxpass=4):
         self.dm = mesh if isinstance(mesh, DeviceMesh) else DeviceMesh(mesh, device)
         self.lib = self.dm.lib
         self.H = self.dm.handle
@@ -172,6 +182,14 @@ class Model:
         self.timers = {}
         self.profile = False
         self._events = []
+        self.refine = bool(refine)
+        self.refine_rtol = float(refine_rtol)
+        self.refine_maxpass = int(refine_maxpass)
+        # warm starts of the refined flow solves (main solve, cascade solves)
+        self._ws_main = torch.zeros(N, **f64) if self.refine else None
+        self._ws_casc = torch.zeros(N, **f64) if self.refine else None
+        # per solve: (entry point, correction passes (-1: exact fallback), relative residual)
+        self.solve_log = collections.deque(maxlen=4096)
 
     # -------------------------------------------------------------- helpers
     @property
@@ -283,14 +301,26 @@ class Model:
                    _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
                    self.stream)
 
-    def _solve_flow(self, b, out, fatal=True):
+    def _solve_flow(self, b, out, fatal=True, ws=None):
         """_solve_KSP (flowplex.py:323-427) on the flow matrix.  A receiver
         cycle that does not drain makes goSPL's solvers give up: the main
-        solve (fatal) raises, a cascade solve is zeroed (flowplex.py:304-320)."""
+        solve (fatal) raises, a cascade solve is zeroed (flowplex.py:304-320).
+        Refined mode: ``ws`` holds the warm start and receives the solution."""
         lev = ctypes.c_int32(0)
         bad = ctypes.c_int64(0)
-        self._call("gospl_flowacc", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal), _ptr(b),
-                   _ptr(out), ctypes.byref(lev), ctypes.byref(bad), self.stream)
+        if self.refine:
+            npass = ctypes.c_int32(0)
+            rr = ctypes.c_double(0.0)
+            out.copy_(ws)
+            self._call("gospl_flowacc_refine", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal),
+                       _ptr(b), _ptr(out), self.refine_rtol, self.refine_maxpass, ctypes.byref(npass),
+                       ctypes.byref(rr), ctypes.byref(bad), self.stream)
+            self.solve_log.append(("flowacc", int(npass.value), float(rr.value)))
+            if not bad.value:
+                ws.copy_(out)
+        else:
+            self._call("gospl_flowacc", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal), _ptr(b),
+                       _ptr(out), ctypes.byref(lev), ctypes.byref(bad), self.stream)
         self.undrained = int(bad.value)
         if bad.value:
             if fatal:
@@ -355,7 +385,7 @@ class Model:
             self._cascade_resid = resid
             if resid > self.maxarea[0]:
                 excess = True
-                self._solve_flow(self._rhs, self._q, fatal=False)
+                self._solve_flow(self._rhs, self._q, fatal=False, ws=self._ws_casc)
                 self._call("gospl_cascade_add", self.H, _ptr(self._q), _ptr(hl), _ptr(self.waterFilled),
                            self.dt, _ptr(self._nfa), _ptr(self.FAL), S)
                 nFA = self._nfa
@@ -370,7 +400,7 @@ class Model:
         S = self.stream
         self._call("gospl_rain_source", self.H, _ptr(self.rainVal), _ptr(self.lFill), self.sealevel,
                    _ptr(self._b), S)
-        self.fa_levels = self._solve_flow(self._b, self.FAL)
+        self.fa_levels = self._solve_flow(self._b, self.FAL, ws=self._ws_main)
         self.waterFilled = hl.clone()
         self.cascade_passes = 0
         has_pits = bool((pitVol > 0.0).any())
@@ -401,6 +431,17 @@ class Model:
     # -------------------------------------------------------------- erosion
     def erodepSPL(self):
         lev = ctypes.c_int32(0)
+        if self.refine:
+            npass = ctypes.c_int32(0)
+            rr = ctypes.c_double(0.0)
+            self._call("gospl_spl_refine", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
+                       _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.dt,
+                       self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),
+                       _ptr(self.Eb), _ptr(self.EbLocal), self.refine_rtol, self.refine_maxpass,
+                       ctypes.byref(npass), ctypes.byref(rr), self.stream)
+            self.solve_log.append(("spl", int(npass.value), float(rr.value)))
+            self.spl_levels = 0
+            return
         self._call("gospl_spl", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
                    _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.dt,
                    self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),

@@ -178,6 +178,33 @@ int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const doubl
               gospl_stream_t stream);
 
 /*
+ * Refined solves (the reduced-cost mode; no counterpart in goSPL, which
+ * solves these systems with FGMRES + ILU(0) to rtol 1e-8, flowplex.py:341-408).
+ * Iterative refinement: each pass takes the FP64 residual r = rhs - A x of the
+ * FP64 iterate x and solves the correction A d = r by one dataflow sweep that
+ * accumulates in 64-bit fixed point (one integer atomic per dependency, order
+ * independent, so runs are bit-reproducible), x += d; passes stop once
+ * ||r||_2 <= rtol ||rhs||_2 or after maxpass corrections.  *passes (HOST, may
+ * be NULL) receives the corrections run, or -1 when the solve fell back to
+ * the exact sweep of gospl_flowacc / gospl_spl (more than three receivers, a
+ * receiver cycle or a fixed-point overflow); *relres (HOST) the final
+ * relative residual.  Synchronises `stream`.
+ *  - gospl_flowacc_refine: as gospl_flowacc; q is IN/OUT: the initial guess
+ *    (goSPL warm-starts from the previous solution, flowplex.py:395-396).
+ *  - gospl_spl_refine: as gospl_spl, the iterate starting at the old h.
+ */
+int gospl_flowacc_refine(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const double* wgt,
+                         const double* b, double* q, double rtol, int32_t maxpass,
+                         int32_t* passes, double* relres, int64_t* unsolved,
+                         gospl_stream_t stream);
+int gospl_spl_refine(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                     const double* disti, const double* fal, const double* kfac, double K,
+                     double m, double dt, double sealevel, const double* lfill,
+                     const int8_t* lsink, double* h, double* cumed, double* eb, double* eblocal,
+                     double rtol, int32_t maxpass, int32_t* passes, double* relres,
+                     gospl_stream_t stream);
+
+/*
  * Linear hillslope diffusion: replaces hillSLP.getHillslope/_hillSlope(0)
  * (gospl/sed/hillslope.py:171-233, 614-649) with `sethillslopecoeff`
  * (functions.pyf:10-14, functions.F90:901-938) and its Richardson/ILU(0)

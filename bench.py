@@ -3,19 +3,35 @@
 Workload: the 2.6M-vertex icosahedral sphere (refinement level 9, 2,621,442
 vertices, radius 6371 km), SH initial surface (degree <= 64, spectrum l^-2,
 scaled to [-6000, 5000] m, sea level 0), lognormal rain, K=5e-6, m=0.5, n=1,
-3 MFD receivers, diffusivity 0.5 m2/yr, dt = 5000 yr.  One "step" is one pass
-of goSPL's runProcesses body over the whole mesh: flow accumulation (fill,
-pit information, receivers, flow solve, overflow cascade), stream-power
-erosion, hillslope diffusion and forcing.  Inputs are resident in HBM before
-the timed region.
+3 MFD receivers, diffusivity 0.5 m2/yr, dt = 5000 yr.  ``--level 10`` runs the
+10,485,762-vertex sphere (SH degree <= 256, spectrum l^-2.5, m = 0.45,
+dt = 2000 yr).  One "step" is one pass of goSPL's runProcesses body over the
+whole mesh: flow accumulation (fill, pit information, receivers, flow solve,
+overflow cascade), stream-power erosion, hillslope diffusion and forcing.
+Inputs are resident in HBM before the timed region.
+
+Three legs of the same workload, each timed over the same window (W untimed
+warm-up steps, then K timed steps between synchronisations):
+* refined (``value``): hillslope diffusion by FP32 Jacobi corrections of an
+  FP64 residual (rtol 1e-12); the triangular receiver-DAG systems by their
+  exact FP64 sweeps (a direct solve: one pass, nothing left to refine);
+* refined_dag: in addition the DAG systems by refinement -- 64-bit
+  fixed-point correction sweeps with FP64 residuals (rtol 1e-12);
+* fp64: exact FP64 sweeps and FP64 Jacobi hillslope (the plain path).
+Then every leg runs ``--acc-steps`` steps from the initial surface, beside
+FP64 runs from the surface nudged by +-1 ulp (three seeds), and each refined
+trajectory's relative elevation error against the FP64 one is reported next
+to the FP64 path's own divergence under that nudge.
 
 Prints one JSON line (the driver's contract) with the roofline of the
-dominant kernel and the CPU oracle timed on a bounded sample beside it.
+dominant kernel of the reported leg and the CPU oracle timed on a bounded
+sample beside it.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -36,19 +52,23 @@ ALG_BYTES = {
     "gospl_flowacc": 68,
     "gospl_spl": 116,
     "gospl_hillslope": 116,
+    "gospl_flowacc_refine": 68,
+    "gospl_spl_refine": 116,
+    "gospl_hillslope_refine": 116,
 }
 METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel err"
-STEP_BYTES = 564  # sum of ALG_BYTES: compulsory bytes per vertex and step
+STEP_BYTES = 564  # compulsory bytes per vertex and step (sum over the phases)
 # the sweep kernels between the library's timing events, per entry point
-SWEEP_KERNELS = {"gospl_flowacc": "k_dag_level0<0> + k_chain<0>",
-                 "gospl_spl": "k_dag_level0<1> + k_chain<1>"}
-# HBM traffic per sweep (level 0 + dataflow kernel) on this workload, from two
-# rocprofv3 --pmc passes over `bench.py --steps 2 --warmup 1 --no-cpu
-# --surface z9.npy` (KiB per dispatch, mean over the passes' solves), each
-# scaled by what the counter reads for a 256 MiB device copy
-# (`--copy-probe 256`): FETCH_SIZE reads 0.5x the bytes moved (scale 2),
-# WRITE_SIZE 1.0x.  Counter runs need --surface: the SH construction's torch
-# kernels crash the counter-collecting profiler.
+SWEEP_KERNELS = {"gospl_flowacc": ("k_dag_level0<0> + k_chain<0>", 0),
+                 "gospl_spl": ("k_dag_level0<1> + k_chain<1>", 1),
+                 "gospl_flowacc_refine": ("k_rx_level0<0> + k_rx_chain<0>", 0),
+                 "gospl_spl_refine": ("k_rx_level0<1> + k_rx_chain<1>", 1)}
+# HBM traffic per sweep (level 0 + dataflow kernel) on the level-9 workload,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch, mean over the passes' sweeps), each scaled by what the counter
+# reads for a 256 MiB device copy (`--copy-probe 256`): FETCH_SIZE reads 0.5x
+# the bytes moved (scale 2), WRITE_SIZE 1.0x.  The runs and their commands are
+# in profiles/*_pmc_traffic.md.
 PMC_SCALE = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
 PMC_KIB = {
     "gospl_flowacc": {"FETCH_SIZE": 213481.8 + 50784.1, "WRITE_SIZE": 15297.1 + 139938.3},
@@ -72,7 +92,12 @@ def build_workload(level, seed, device, surface=None, save_surface=None):
 
     t0 = time.time()
     z = np.load(surface) if surface else None
-    m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device, z=z)
+    if level >= 10:
+        m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device, z=z, lmax=256,
+                                         slope=2.5)
+        p.update(m=0.45, dt=2000.0)
+    else:
+        m, h0, p = workloads.sphere_case(_lib.load(), level=level, seed=seed, device=device, z=z)
     if save_surface:
         np.save(save_surface, m.to_input_order(h0))
     log("[bench] mesh level %d: N=%d built in %.1fs" % (level, m.N, time.time() - t0))
@@ -94,14 +119,144 @@ def cpu_baseline(m, h0, p, steps):
     return steps / dt, dt
 
 
+LEGS = {"refined": dict(refine=True), "refined_dag": dict(refine=True, refine_dag=True), "fp64": {}}
+
+
+def run_leg(m, h0, p, dev, name, steps, warmup):
+    """W warm-up steps, K timed steps, then K profiled steps (HIP events on the
+    launch stream around every entry point and every sweep)."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    g = Model(m, h0, tend=1e12, device=dev, **LEGS[name], **p)
+    t0 = time.time()
+    g.initialise()
+    torch.cuda.synchronize()
+    log("[bench] %s: initialise %.2fs" % (name, time.time() - t0))
+    for i in range(warmup):
+        g.step()
+        torch.cuda.synchronize()
+        log("[bench] %s: warmup %d done (cascade %d, pits %d)" % (name, i, g.cascade_passes, g.npits))
+    g.solve_log.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        g.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sps = steps / elapsed
+    log("[bench] %s: %d steps in %.3fs -> %.2f steps/s" % (name, steps, elapsed, sps))
+    solves = list(g.solve_log)
+    lib = g.lib
+    f64p, i64p = ctypes.c_double, ctypes.c_int64
+    z = [f64p(0), i64p(0), f64p(0), i64p(0)]
+    lib.gospl_sweep_timing(g.H, 1, *[ctypes.byref(v) for v in z])
+    g.enable_profiling(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        g.step()
+    torch.cuda.synchronize()
+    prof_elapsed = time.perf_counter() - t1
+    ph = g.phase_times()
+    g.enable_profiling(False)
+    sw = [f64p(0), i64p(0), f64p(0), i64p(0)]
+    lib.gospl_sweep_timing(g.H, 0, *[ctypes.byref(v) for v in sw])
+    g.close()
+    phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
+    log("[bench] %s phases (ms per step): " % name + ", ".join(
+        "%s=%.3f" % (k, v["ms_total"] / steps) for k, v in sorted(phases.items(), key=lambda x: -x[1]["ms_total"])))
+    return dict(sps=sps, elapsed=elapsed, phases=phases, prof_elapsed=prof_elapsed, solves=solves,
+                sweep=dict(fa_ms=sw[0].value, fa_calls=sw[1].value, spl_ms=sw[2].value, spl_calls=sw[3].value))
+
+
+def dominant_kernel(leg, N, level):
+    """The entry point with the most time among the per-vertex phases, and its
+    sweep kernels' duration per launch (HIP events inside the library)."""
+    phases = leg["phases"]
+    dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms_total"])
+    sw = leg["sweep"]
+    if dom in SWEEP_KERNELS:
+        kname, slot = SWEEP_KERNELS[dom]
+        kms_tot, kcalls = (sw["fa_ms"], sw["fa_calls"]) if slot == 0 else (sw["spl_ms"], sw["spl_calls"])
+        kms = kms_tot / kcalls if kcalls else phases[dom]["ms_per_call"]
+    else:
+        kname, kms, kcalls = dom, phases[dom]["ms_per_call"], phases[dom]["calls"]
+    achieved = ALG_BYTES[dom] * N / (kms * 1e-3) / 1e9
+    traffic = pmc_traffic(dom, level) if dom in SWEEP_KERNELS else None
+    return dict(bound="hbm", kernel=kname, entry_point=dom, achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=achieved / PEAK_HBM_GBS, traffic=traffic, alg_bytes_per_vertex=ALG_BYTES[dom],
+                ms_per_launch=kms, launches=kcalls)
+
+
+def accuracy(m, h0, p, dev, nsteps, legs):
+    """Relative elevation error of each refined trajectory after nsteps
+    against the FP64 one, and the FP64 path's own divergence from surfaces
+    nudged by +-1 ulp at every vertex (three seeds: the envelope a change of
+    rounding is read against, as goSPL's step is discontinuous at the last bit
+    of its inputs)."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    cases = [("fp64", h0, {})] + [(n, h0, LEGS[n]) for n in legs if n != "fp64"]
+    for seed in (7, 8, 9):
+        rng = np.random.default_rng(seed)
+        h1 = np.where(rng.random(m.N) < 0.5, np.nextafter(h0, np.inf), np.nextafter(h0, -np.inf))
+        cases.append(("fp64_ulp%d" % seed, h1, {}))
+    runs = {}
+    t0 = time.time()
+    for name, h, kw in cases:
+        g = Model(m, h, tend=1e12, device=dev, **kw, **p)
+        g.refine_stats = bool(kw)
+        g.initialise()
+        for _ in range(nsteps):
+            g.step()
+        torch.cuda.synchronize()
+        runs[name] = (g.hLocal.cpu().numpy(), list(g.solve_log))
+        g.close()
+        log("[bench] accuracy: %s %d steps done (%.1fs)" % (name, nsteps, time.time() - t0))
+    a = m.area
+    hf = runs["fp64"][0]
+
+    def err(h):
+        d = h - hf
+        return dict(l2_area=float(np.sqrt((a * d * d).sum() / (a * hf * hf).sum())),
+                    max_rel=float(np.abs(d).max() / np.abs(hf).max()),
+                    frac_vertices_over_1e6=float(np.mean(np.abs(d) > 1e-6 * np.abs(hf).max())),
+                    mean_m=float(abs((a * d).sum() / a.sum())))
+
+    out = dict(steps=nsteps)
+    ulp = [err(runs[k][0]) for k in runs if k.startswith("fp64_ulp")]
+    out["fp64_ulp_vs_fp64"] = {k: max(u[k] for u in ulp) for k in ulp[0]}  # worst of the three seeds
+    for n in legs:
+        if n == "fp64":
+            continue
+        stats = {}
+        solves = runs[n][1]
+        for kind in ("flowacc", "spl", "hillslope"):
+            ps = [s[1] for s in solves if s[0] == kind]
+            rr = [s[2] for s in solves if s[0] == kind and s[1] >= 0]
+            if ps:
+                stats[kind] = dict(solves=len(ps), passes_mean=float(np.mean([x for x in ps if x >= 0] or [0])),
+                                   passes_max=int(max(ps)), exact_fallbacks=int(sum(1 for x in ps if x < 0)),
+                                   relres_max=float(max(rr)) if rr else None)
+        out[n + "_vs_fp64"] = err(runs[n][0])
+        out[n + "_solves"] = stats
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100,
-                    help="timed steps (the workload is a 100-step run at dt = 5000 yr)")
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps per leg")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--level", type=int, default=9)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--legs", default="refined,refined_dag,fp64", help="comma list of " + ", ".join(LEGS))
+    ap.add_argument("--acc-steps", type=int, default=100,
+                    help="steps of the accuracy comparison (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--surface", default=None, metavar="NPY",
@@ -129,71 +284,30 @@ def main():
 
     import torch
 
-    from gospl_amd.model import Model
-
     dev = torch.device("cuda:0")
     m, h0, p = build_workload(args.level, args.seed, dev, args.surface, args.save_surface)
     if args.save_surface and args.steps == 0:
         return
-    g = Model(m, h0, tend=1e12, device=dev, **p)
-    t0 = time.time()
-    g.initialise()
-    torch.cuda.synchronize()
-    log("[bench] initialise %.2fs" % (time.time() - t0))
-    for i in range(args.warmup):
-        g.step()
-        torch.cuda.synchronize()
-        log("[bench] warmup %d done (fa levels %d, spl levels %d, cascade %d, pits %d)" %
-            (i, g.fa_levels, g.spl_levels, g.cascade_passes, g.npits))
-
-    # timed region: K steps
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        g.step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    sps = args.steps / elapsed
-    log("[bench] %d steps in %.3fs -> %.2f steps/s" % (args.steps, elapsed, sps))
-
-    # per-entry-point durations (HIP events on the launch stream) over a second
-    # timed pass of the same length
-    import ctypes
-
-    lib = g.lib
-    f64p, i64p = ctypes.c_double, ctypes.c_int64
-    z = [f64p(0), i64p(0), f64p(0), i64p(0)]
-    lib.gospl_sweep_timing(g.H, 1, *[ctypes.byref(v) for v in z])
-    g.enable_profiling(True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        g.step()
-    torch.cuda.synchronize()
-    prof_elapsed = time.perf_counter() - t1
-    ph = g.phase_times()
-    g.enable_profiling(False)
-    fa_ms, fa_calls, spl_ms, spl_calls = f64p(0), i64p(0), f64p(0), i64p(0)
-    lib.gospl_sweep_timing(g.H, 0, ctypes.byref(fa_ms), ctypes.byref(fa_calls), ctypes.byref(spl_ms),
-                           ctypes.byref(spl_calls))
     N = m.N
-    phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
-    dom = max((k for k in phases if k in ALG_BYTES), key=lambda k: phases[k]["ms_total"])
-    # the dominant entry point's dominant kernel: the DAG sweep, i.e. its
-    # back-to-back kernels (level 0 on the full grid, then the dataflow
-    # kernel), timed by HIP events on their launch stream inside the library
-    kern = {"gospl_flowacc": (SWEEP_KERNELS["gospl_flowacc"], fa_ms.value, fa_calls.value),
-            "gospl_spl": (SWEEP_KERNELS["gospl_spl"], spl_ms.value, spl_calls.value)}.get(dom)
-    if kern and kern[2] > 0:
-        kname, kms, kcalls = kern[0], kern[1] / kern[2], kern[2]
-    else:
-        kname, kms, kcalls = dom, phases[dom]["ms_per_call"], phases[dom]["calls"]
-    bytes_per_launch = ALG_BYTES[dom] * N
-    traffic = pmc_traffic(dom, args.level) if kern and kern[2] > 0 else None
-    achieved = bytes_per_launch / (kms * 1e-3) / 1e9
-    log("[bench] dominant kernel %s: %.3f ms per launch over %d launches" % (kname, kms, kcalls))
-    log("[bench] phases (ms per step): " + ", ".join(
-        "%s=%.3f" % (k, v["ms_total"] / args.steps) for k, v in sorted(phases.items(), key=lambda x: -x[1]["ms_total"])))
+    legs = {}
+    for name in [x.strip() for x in args.legs.split(",") if x.strip()]:
+        legs[name] = run_leg(m, h0, p, dev, name, args.steps, args.warmup)
+    main_leg = "refined" if "refined" in legs else next(iter(legs))
+    L = legs[main_leg]
+    sps = L["sps"]
+    roof = dominant_kernel(L, N, args.level)
+    roof["window"] = "steps %d-%d of the %s leg (after %d warm-up and %d timed steps)" % (
+        args.warmup + args.steps + 1, args.warmup + 2 * args.steps, main_leg, args.warmup, args.steps)
+    roof["note"] = ("latency-bound: ~600 dependent hops per sweep (the longest flow path); achieved = "
+                    "algorithmic bytes per launch / mean launch duration (HIP events on the launch stream)")
+    log("[bench] dominant kernel %s: %.3f ms per launch over %d launches (%.1f GB/s)" %
+        (roof["kernel"], roof["ms_per_launch"], roof["launches"], roof["achieved"]))
+
+    acc = None
+    if args.acc_steps > 0 and any(n != "fp64" for n in legs):
+        acc = accuracy(m, h0, p, dev, args.acc_steps, list(legs))
+        log("[bench] accuracy after %d steps: %s" % (args.acc_steps, json.dumps(
+            {k: v for k, v in acc.items() if k.endswith("_vs_fp64")})))
 
     cpu = None
     if not args.no_cpu:
@@ -205,6 +319,29 @@ def main():
                "sample": "FP64 oracle (C + numpy, exact DAG solves), %d steps of the same %d-vertex "
                          "workload after its init flow accumulation, one host core" % (args.cpu_steps, N)}
 
+    leg_cfg = {}
+    for name, lg in legs.items():
+        leg_cfg[name] = dict(steps_per_s=lg["sps"], ms_per_step=1e3 / lg["sps"],
+                             phases_ms_per_step={k: v["ms_total"] / args.steps for k, v in lg["phases"].items()},
+                             profiled_pass_s=lg["prof_elapsed"],
+                             solves_in_timed_window=len(lg["solves"]),
+                             correction_passes_in_timed_window=int(sum(s[1] for s in lg["solves"] if s[1] > 0)))
+    cfg = {"workload": "icosahedral sphere level %d, %d vertices, %s leg reported" % (args.level, N, main_leg),
+           "vertices": N, "dt_yr": p["dt"], "flowdir": p["flowdir"], "spl_m": p["m"],
+           "vertex_steps_per_s": sps * N, "legs": leg_cfg,
+           "step_roofline": {"alg_bytes_per_vertex_step": STEP_BYTES,
+                             "achieved_GBps": STEP_BYTES * N * sps / 1e9,
+                             "frac": STEP_BYTES * N * sps / 1e9 / PEAK_HBM_GBS}}
+    if "fp64" in legs:
+        cfg["fp64_steps_per_s"] = legs["fp64"]["sps"]
+        for n in legs:
+            if n != "fp64":
+                cfg[n + "_over_fp64"] = legs[n]["sps"] / legs["fp64"]["sps"]
+    if acc is not None:
+        cfg["accuracy"] = acc
+    dtype = {"refined": "f64 (DAG sweeps, state, residuals) + f32 (hillslope corrections)",
+             "refined_dag": "f64 (state, residuals) + i64 fixed-point (DAG corrections) + f32 (hillslope)",
+             "fp64": "f64"}[main_leg]
     out = {
         "metric": METRIC,
         "value": sps,
@@ -216,28 +353,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (seeded SH surface, lognormal rain)",
-        "config": {"workload": "icosahedral sphere level %d, %d vertices, FP64 plain path" % (args.level, N),
-                   "vertices": N, "dt_yr": p["dt"], "flowdir": p["flowdir"],
-                   "vertex_steps_per_s": sps * N,
-                   "fp32_refine": "not implemented",
-                   "phases_ms_per_step": {k: v["ms_total"] / args.steps for k, v in phases.items()},
-                   "profiled_pass_s": prof_elapsed,
-                   "step_roofline": {"alg_bytes_per_vertex_step": STEP_BYTES,
-                                     "achieved_GBps": STEP_BYTES * N * sps / 1e9,
-                                     "frac": STEP_BYTES * N * sps / 1e9 / PEAK_HBM_GBS}},
-        "roofline": {"bound": "hbm", "kernel": kname, "entry_point": dom, "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": traffic, "alg_bytes_per_vertex": ALG_BYTES[dom],
-                     "ms_per_launch": kms, "launches": kcalls,
-                     "note": "latency-bound: ~600 dependent levels per solve (the longest flow "
-                             "path); traffic = HBM bytes per launch from rocprofv3 --pmc "
-                             "FETCH_SIZE x2 + WRITE_SIZE x1 (copy-calibrated), measured on this "
-                             "workload"},
+        "config": cfg,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
-    g.close()
     print(json.dumps(out), flush=True)
 
 

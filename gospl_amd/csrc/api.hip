@@ -378,6 +378,8 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->rx_ctl);
   if (m->rx_hctl) (void)hipHostFree(m->rx_hctl);
   dfree(m->hdiag);
+  dfree(m->hcoef32);
+  dfree(m->hdiag32);
   dfree(m->hsea);
   delete m;
   return 0;

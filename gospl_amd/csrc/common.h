@@ -64,6 +64,8 @@ struct gospl_mesh {
   int64_t pg_n;               // grouped cells
   double* hcoef;              // hillslope operator rows (W, N), kept between calls
   double* hdiag;              // its diagonal
+  float* hcoef32;             // FP32 copies of both (refined solve's corrections)
+  float* hdiag32;
   int8_t* hsea;               // the sea flags the rows were built from
   double hpar[3];             // cda, cdm, dt they were built with
   int hvalid;

@@ -21,7 +21,8 @@ __device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int3
                                          const double* __restrict__ elgt, const double* __restrict__ vdist,
                                          const int8_t* __restrict__ outlet, int flat,
                                          const double* __restrict__ h, double sl, double cda, double cdm,
-                                         double dt, double* __restrict__ coef, double* __restrict__ diag) {
+                                         double dt, double* __restrict__ coef, double* __restrict__ diag,
+                                         float* __restrict__ coef32, float* __restrict__ diag32) {
   const double a = area[i];
   const bool ident = flat && outlet[i];
   const double ck = (h[i] <= sl ? cdm : cda) * dt;
@@ -41,8 +42,11 @@ __device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int3
       }
     }
     coef[(int64_t)p * N + i] = c_out;
+    coef32[(int64_t)p * N + i] = (float)c_out;
   }
-  diag[i] = ident ? 1.0 : (a > 0.0 ? 1.0 + s1 : 0.0);
+  const double d = ident ? 1.0 : (a > 0.0 ? 1.0 + s1 : 0.0);
+  diag[i] = d;
+  diag32[i] = (float)d;
 }
 
 // Every row, and the sea flags they were built from.
@@ -51,10 +55,11 @@ __global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
                             const double* __restrict__ elgt, const double* __restrict__ vdist,
                             const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
                             double sl, double cda, double cdm, double dt, double* __restrict__ coef,
-                            double* __restrict__ diag, int8_t* __restrict__ sea) {
+                            double* __restrict__ diag, float* __restrict__ coef32, float* __restrict__ diag32,
+                            int8_t* __restrict__ sea) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag);
+  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag, coef32, diag32);
   sea[i] = h[i] <= sl;
 }
 
@@ -76,7 +81,8 @@ __global__ void k_hill_coef_changed(int64_t N, int W, const int32_t* __restrict_
                                     const int8_t* __restrict__ outlet, int flat, const double* __restrict__ h,
                                     double sl, double cda, double cdm, double dt,
                                     const int8_t* __restrict__ chg, double* __restrict__ coef,
-                                    double* __restrict__ diag) {
+                                    double* __restrict__ diag, float* __restrict__ coef32,
+                                    float* __restrict__ diag32) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   bool need = chg[i] != 0;
@@ -85,7 +91,8 @@ __global__ void k_hill_coef_changed(int64_t N, int W, const int32_t* __restrict_
     const int n = nbr[(int64_t)p * N + i];
     if (n >= 0 && chg[n]) need = true;
   }
-  if (need) hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag);
+  if (need)
+    hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag, coef32, diag32);
 }
 
 __global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
@@ -165,6 +172,129 @@ __global__ void k_hill_rho(int64_t N, const double* __restrict__ diag, unsigned 
   if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(key, dkey(r));
 }
 
+// Refined solve (FP32 corrections of an FP64 residual).  Control words in
+// device memory: [0] done, [1] passes, [2] relative residual, [3] its first value.
+struct HillCtl {
+  double done;
+  double pass;
+  double relres;
+  double relres0;
+};
+
+// r = h - M x in FP64 (rows with a zero diagonal hold x = h), stored as FP32
+// for the correction sweeps; per-workgroup sum r^2 and sum h^2.
+__global__ void k_hill_resid(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                             const double* __restrict__ coef, const double* __restrict__ diag,
+                             const double* __restrict__ h, const double* __restrict__ x, float* __restrict__ r32,
+                             double* __restrict__ part, const HillCtl* __restrict__ ctl) {
+  if (ctl->done != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, hh = 0.0;
+  if (i < N) {
+    const double d = diag[i];
+    const double hi = h[i];
+    double r = hi - x[i];
+    if (d != 0.0) {
+      double s = d * x[i];
+      const int dg = deg[i];
+      for (int p = 0; p < W && p < dg; ++p) {
+        const double c = coef[(int64_t)p * N + i];
+        if (c != 0.0) s = s + c * x[nbr[(int64_t)p * N + i]];
+      }
+      r = hi - s;
+    }
+    r32[i] = (float)r;
+    rr = r * r;
+    hh = hi * hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    hh += __shfl_xor(hh, o, 64);
+  }
+  __shared__ double sh[2][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = rr;
+    sh[1][threadIdx.x >> 6] = hh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = sh[0][0], b = sh[1][0];
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      a += sh[0][w];
+      b += sh[1][w];
+    }
+    part[2 * (int64_t)blockIdx.x] = a;
+    part[2 * (int64_t)blockIdx.x + 1] = b;
+  }
+}
+
+// Relative residual (one workgroup, fixed order) and the stop test.
+__global__ void k_hill_norm(int nblk, const double* __restrict__ part, double rtol, int pass, int maxpass,
+                            HillCtl* __restrict__ ctl) {
+  if (ctl->done != 0.0) return;
+  double rr = 0.0, hh = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    rr += part[2 * (int64_t)b];
+    hh += part[2 * (int64_t)b + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    hh += __shfl_xor(hh, o, 64);
+  }
+  __shared__ double sh[2][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = rr;
+    sh[1][threadIdx.x >> 6] = hh;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+    sh[0][0] += sh[0][w];
+    sh[1][0] += sh[1][w];
+  }
+  rr = sh[0][0];
+  hh = sh[1][0];
+  const double rel = hh > 0.0 ? sqrt(rr / hh) : 0.0;
+  ctl->relres = rel;
+  if (pass == 0) ctl->relres0 = rel;
+  if (!(rel > rtol) || pass >= maxpass) ctl->done = 1.0;  // a NaN residual stops too
+}
+
+// One FP32 Jacobi sweep of M d = r (first: from d = 0).
+__global__ void k_hill_sweep32(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                               const float* __restrict__ coef32, const float* __restrict__ diag32,
+                               const float* __restrict__ r32, const float* __restrict__ din, int first,
+                               float* __restrict__ dout, const HillCtl* __restrict__ ctl) {
+  if (ctl->done != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float d = diag32[i];
+  const float r = r32[i];
+  if (d == 0.0f) {
+    dout[i] = r;
+    return;
+  }
+  float s = r;
+  if (!first) {
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      const float c = coef32[(int64_t)p * N + i];
+      if (c != 0.0f) s = s - c * din[nbr[(int64_t)p * N + i]];
+    }
+  }
+  dout[i] = s / d;
+}
+
+// x += d (FP64 iterate) and count the pass.
+__global__ void k_hill_addcorr(int64_t N, const float* __restrict__ d, double* __restrict__ x,
+                               HillCtl* __restrict__ ctl) {
+  if (ctl->done != 0.0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) ctl->pass = ctl->pass + 1.0;
+  if (i >= N) return;
+  x[i] = x[i] + (double)d[i];
+}
+
 __global__ void k_hill_update(int64_t N, const double* __restrict__ x, double dt, double* __restrict__ h,
                               double* __restrict__ cumed, double* __restrict__ eblocal) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,25 +346,25 @@ extern "C" int gospl_hillslope_coeff(gospl_mesh_t* m, const double* kd, double* 
   return 0;
 }
 
-extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
-                               int32_t sweeps, double* h, double* cumed, double* eblocal,
-                               gospl_stream_t s) {
-
+// The operator rows kept in the handle, brought up to date for h: all rows
+// when the parameters changed, else the rows next to a vertex that changed
+// side of sea level.
+static int hill_operator(gospl_mesh* m, double cda, double cdm, double dt, double sealevel, const double* h,
+                         hipStream_t s) {
   const int64_t N = m->N;
   const int g = nblocks(N);
   if (!m->hcoef) {
     GOSPL_CHECK(hipMalloc(&m->hcoef, sizeof(double) * (int64_t)m->W * (N + 64)));
     GOSPL_CHECK(hipMalloc(&m->hdiag, sizeof(double) * (N + 64)));
+    GOSPL_CHECK(hipMalloc(&m->hcoef32, sizeof(float) * (int64_t)m->W * (N + 64)));
+    GOSPL_CHECK(hipMalloc(&m->hdiag32, sizeof(float) * (N + 64)));
     GOSPL_CHECK(hipMalloc(&m->hsea, N + 64));
     m->hvalid = 0;
   }
-  double* coef = m->hcoef;
-  double* diag = m->hdiag;
-  double* xa = m->wd[1];
-  double* xb = m->wd[2];
   if (!m->hvalid || m->hpar[0] != cda || m->hpar[1] != cdm || m->hpar[2] != dt) {
     k_hill_coef<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, m->outlet,
-                                           m->flat, h, sealevel, cda, cdm, dt, coef, diag, m->hsea);
+                                           m->flat, h, sealevel, cda, cdm, dt, m->hcoef, m->hdiag, m->hcoef32,
+                                           m->hdiag32, m->hsea);
     GOSPL_CHECK(hipGetLastError());
     m->hpar[0] = cda;
     m->hpar[1] = cdm;
@@ -244,10 +374,24 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
     int8_t* chg = reinterpret_cast<int8_t*>(m->wi[0]);
     k_hill_sea<<<g, GOSPL_BLOCK, 0, s>>>(N, h, sealevel, m->hsea, chg);
     k_hill_coef_changed<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist,
-                                                   m->outlet, m->flat, h, sealevel, cda, cdm, dt, chg, coef,
-                                                   diag);
+                                                   m->outlet, m->flat, h, sealevel, cda, cdm, dt, chg, m->hcoef,
+                                                   m->hdiag, m->hcoef32, m->hdiag32);
     GOSPL_CHECK(hipGetLastError());
   }
+  return 0;
+}
+
+extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
+                               int32_t sweeps, double* h, double* cumed, double* eblocal,
+                               gospl_stream_t s) {
+
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  if (hill_operator(m, cda, cdm, dt, sealevel, h, s)) return 1;
+  double* coef = m->hcoef;
+  double* diag = m->hdiag;
+  double* xa = m->wd[1];
+  double* xb = m->wd[2];
   const double* xin = h;
   double* xout = xa;
   if (sweeps > 0) {
@@ -299,5 +443,57 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
   }
   k_hill_update<<<g, GOSPL_BLOCK, 0, s>>>(N, xin, dt, h, cumed, eblocal);
   GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gospl_hillslope_refine(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
+                                      int32_t sweeps32, double rtol, int32_t maxpass, double* h, double* cumed,
+                                      double* eblocal, int32_t* passes, double* relres, gospl_stream_t s) {
+  if (sweeps32 <= 0) {
+    // too stiff for a fixed count of FP32 sweeps: the converged FP64 solve
+    if (passes) *passes = -1;
+    if (relres) *relres = 0.0;
+    return gospl_hillslope(m, cda, cdm, dt, sealevel, 0, h, cumed, eblocal, s);
+  }
+  if (maxpass < 1) maxpass = 1;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  if (hill_operator(m, cda, cdm, dt, sealevel, h, s)) return 1;
+  if (!m->rx_ctl) {
+    GOSPL_CHECK(hipMalloc(&m->rx_ctl, sizeof(double) * 64));
+    GOSPL_CHECK(hipHostMalloc((void**)&m->rx_hctl, sizeof(double) * 64, hipHostMallocDefault));
+  }
+  HillCtl* ctl = reinterpret_cast<HillCtl*>(m->rx_ctl + 32);
+  double* x = m->wd[1];
+  double* part = m->wd[0];  // 2 per workgroup <= N + 64
+  float* r32 = reinterpret_cast<float*>(m->wi[5]);
+  float* da = reinterpret_cast<float*>(m->wi[6]);
+  float* db = reinterpret_cast<float*>(m->wi[7]);
+  GOSPL_CHECK(hipMemsetAsync(ctl, 0, sizeof(HillCtl), s));
+  GOSPL_CHECK(hipMemcpyAsync(x, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  for (int pass = 0; pass <= maxpass; ++pass) {
+    k_hill_resid<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->hcoef, m->hdiag, h, x, r32, part, ctl);
+    k_hill_norm<<<1, GOSPL_BLOCK, 0, s>>>(g, part, rtol, pass, maxpass, ctl);
+    if (pass == maxpass) break;
+    const float* din = da;
+    float* dout = da;
+    for (int k = 0; k < sweeps32; ++k) {
+      k_hill_sweep32<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->hcoef32, m->hdiag32, r32, din, k == 0,
+                                               dout, ctl);
+      din = dout;
+      dout = (dout == da) ? db : da;
+    }
+    k_hill_addcorr<<<g, GOSPL_BLOCK, 0, s>>>(N, din, x, ctl);
+    GOSPL_CHECK(hipGetLastError());
+  }
+  k_hill_update<<<g, GOSPL_BLOCK, 0, s>>>(N, x, dt, h, cumed, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  if (passes || relres) {
+    GOSPL_CHECK(hipMemcpyAsync(m->rx_hctl + 32, ctl, sizeof(HillCtl), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    const HillCtl* hc = reinterpret_cast<const HillCtl*>(m->rx_hctl + 32);
+    if (passes) *passes = (int32_t)hc->pass;
+    if (relres) *relres = hc->relres;
+  }
   return 0;
 }

@@ -22,11 +22,11 @@
 // Records (64 bytes, 8 words per vertex):
 //   flow:         w0 accumulator, w1 rhs, w2..4 weights (f64),
 //                 w5 receivers 0 | 1 << 32, w6 receiver 2
-//   stream power: w0 accumulator, w1 rhs, w2 diag (f32) | entries << 32,
-//                 w3..7 the first five dependents (donor id | coefficient
-//                 (f32) << 32), later ones in (odon, ocoef).
-// Stream-power coefficients are rounded to FP32 (the FP64 residual uses the
-// FP64 matrix, so this only slows refinement by a factor 2^-24 per pass).
+//   stream power: w0 accumulator, w1 rhs, w2 diag, w3 entries | dependent 0
+//                 << 32, w4 its coefficient, w5 dependents 1 | 2 << 32, w6, w7
+//                 their coefficients (f64); later ones in (odon, ocoef).
+// A sweep writes its correction d to dx; the leftover check adds it to x
+// (no read of x on the dependent path).
 #include <cmath>
 
 #include "dag_common.h"
@@ -40,8 +40,8 @@ namespace gospl {
 #define RX_MASK ((1ll << RX_SHIFT) - 1)
 #define RX_UNIT_BITS 48               // unit = bound / 2^48
 #define RX_QMAX 562949953421312.0     // 2^49: a larger push is an overflow
-#define RX_SD 5                       // dependents held in a stream-power record
-#define RX_PF 4                       // dependents whose records a stream-power hop prefetches
+#define RX_SD 3                       // dependents held in a stream-power record
+#define RX_PF 3                       // dependents whose records a stream-power hop prefetches
 #define RX_PARTS 4                    // partial sums per workgroup of the residual kernel
 
 // Control block of a refined solve (device memory, 512 bytes).
@@ -61,9 +61,11 @@ struct RxCtl {
 struct RxArgs {
   int64_t N;
   long long* rec;        // (N, RX_W) records
-  double* x;             // FP64 iterate, corrected in place
+  double* x;             // FP64 iterate
+  double* dx;            // the sweep's correction (added to x by k_rx_unsolved)
   const int32_t* odon;   // stream power: dependents past the record, (c - RX_SD) * N + v
-  const float* ocoef;    // and their coefficients
+  const double* ocoef;   // and their coefficients
+  const int32_t* cnt0;   // dependencies of each vertex at the start of the sweep
   RxCtl* ctl;
 };
 
@@ -73,7 +75,6 @@ __device__ __forceinline__ double rx_f64(long long w) { return __longlong_as_dou
 __device__ __forceinline__ long long rx_bits(double d) { return __double_as_longlong(d); }
 __device__ __forceinline__ int rx_lo(long long w) { return (int)(unsigned int)((unsigned long long)w & 0xffffffffull); }
 __device__ __forceinline__ int rx_hi(long long w) { return (int)(unsigned int)((unsigned long long)w >> 32); }
-__device__ __forceinline__ float rx_hif(long long w) { return __int_as_float(rx_hi(w)); }
 __device__ __forceinline__ long long rx_pack(int lo, int hi) {
   return (long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned long long)(unsigned int)lo);
 }
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_resid(
     int64_t N, int NR, const int32_t* __restrict__ nbr, const unsigned long long* __restrict__ rowmask,
     const int32_t* __restrict__ rcv, const double* __restrict__ coef, const double* __restrict__ diag,
     const double* __restrict__ rhs, const double* __restrict__ x, int build, long long* __restrict__ rec,
-    int32_t* __restrict__ odon, float* __restrict__ ocoef, double* __restrict__ part,
+    int32_t* __restrict__ odon, double* __restrict__ ocoef, int32_t* __restrict__ cnt0, double* __restrict__ part,
     const RxCtl* __restrict__ ctl, ZeroList zl) {
   if (ctl->done) return;
   if (blockIdx.x == 0)
@@ -220,6 +221,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_resid(
           w[k] = coef[(int64_t)k * N + i];
         }
         longlong2* q = reinterpret_cast<longlong2*>(rp);
+        cnt0[i] = __popcll(mk);
         q[0] = make_longlong2(((long long)__popcll(mk) << RX_SHIFT) + RX_BIAS, rx_bits(r));
         q[1] = make_longlong2(rx_bits(w[0]), rx_bits(w[1]));
         q[2] = make_longlong2(rx_bits(w[2]), rx_pack(rr[0], rr[1]));
@@ -231,7 +233,8 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_resid(
           if (rk != (int)i && rk >= 0 && coef[(int64_t)k * N + i] != 0.0) ndep++;
         }
         // dependents: donors j (slot k) with coefficient -off_jk >= 0
-        long long e[RX_SD] = {0, 0, 0, 0, 0};
+        int dj[RX_SD] = {0, 0, 0};
+        double dc[RX_SD] = {0.0, 0.0, 0.0};
         unsigned long long m = mk;
         const int nd = __popcll(mk);
         for (int c = 0; m; ++c) {
@@ -239,21 +242,25 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_resid(
           m &= m - 1ull;
           const int p = bit / NR, k = bit - p * NR;
           const int j = nbr[(int64_t)p * N + i];
-          const float cf = (float)(-coef[(int64_t)k * N + j]);
+          const double cf = -coef[(int64_t)k * N + j];
           if (c < RX_SD) {
 #pragma unroll
             for (int t = 0; t < RX_SD; ++t)
-              if (t == c) e[t] = rx_pack(j, __float_as_int(cf));
+              if (t == c) {
+                dj[t] = j;
+                dc[t] = cf;
+              }
           } else {
             odon[(int64_t)(c - RX_SD) * N + i] = j;
             ocoef[(int64_t)(c - RX_SD) * N + i] = cf;
           }
         }
         longlong2* q = reinterpret_cast<longlong2*>(rp);
+        cnt0[i] = ndep;
         q[0] = make_longlong2(((long long)ndep << RX_SHIFT) + RX_BIAS, rx_bits(r));
-        q[1] = make_longlong2(rx_pack(__float_as_int((float)diag[i]), nd), e[0]);
-        q[2] = make_longlong2(e[1], e[2]);
-        q[3] = make_longlong2(e[3], e[4]);
+        q[1] = make_longlong2(rx_bits(diag[i]), rx_pack(nd, dj[0]));
+        q[2] = make_longlong2(rx_bits(dc[0]), rx_pack(dj[1], dj[2]));
+        q[3] = make_longlong2(rx_bits(dc[1]), rx_bits(dc[2]));
       }
     }
   }
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_norm(int nblk, const double*
 template <int MODE>
 __device__ __forceinline__ double rx_value(const RxRec& rc, long long S, double unit) {
   const double s = rx_f64(rc.w[0]) + (double)S * unit;
-  return MODE == 0 ? s : s / (double)__int_as_float(rx_lo(rc.w[1]));
+  return MODE == 0 ? s : s / rx_f64(rc.w[1]);
 }
 
 // Dependent c of a record: its vertex (-1 if none) and push coefficient.
@@ -320,15 +327,11 @@ __device__ __forceinline__ void rx_dep(const RxRec& rc, int v, int c, int& d, do
       cf = w;
     }
   } else {
-    const int nd = rx_hi(rc.w[1]);
+    const int nd = rx_lo(rc.w[2]);
     if (c < nd && c < RX_SD) {
       // constant indices only (a runtime index would put the record in scratch)
-      long long e = rc.w[2];
-#pragma unroll
-      for (int t = 1; t < RX_SD; ++t)
-        if (c == t) e = rc.w[2 + t];
-      d = rx_lo(e);
-      cf = (double)rx_hif(e);
+      d = c == 0 ? rx_hi(rc.w[2]) : c == 1 ? rx_lo(rc.w[4]) : rx_hi(rc.w[4]);
+      cf = rx_f64(c == 0 ? rc.w[3] : c == 1 ? rc.w[5] : rc.w[6]);
     }
   }
 }
@@ -350,17 +353,19 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_level0(RxArgs a, int32_t* __
   RxRec rc;
   double d = 0.0;
   if (t < N) {
-    const long long acc = a.rec[(int64_t)v * RX_W];
-    src = rx_cnt(acc) == 0;
+    // sources by their initial count: the live accumulator of a vertex that
+    // another source releases here reaches zero too, and that vertex goes to
+    // the start list, not through this kernel
+    src = a.cnt0[v] == 0;
     if (src) {
       rx_load(a.rec, v, rc);
-      d = rx_value<MODE>(rc, rx_sum(acc), unit);
-      a.x[v] = a.x[v] + d;
+      d = rx_value<MODE>(rc, 0, unit);
+      a.dx[v] = d;
     }
   }
   // dependents: the three receivers (flow) or the record's dependents and
   // the overflow entries (stream power), one ballot round each
-  const int nd_lane = !src ? 0 : (MODE == 0 ? 3 : rx_hi(rc.w[1]));
+  const int nd_lane = !src ? 0 : (MODE == 0 ? 3 : rx_lo(rc.w[2]));
   int ndmax = nd_lane;
   for (int o = 32; o > 0; o >>= 1) ndmax = max(ndmax, __shfl_xor(ndmax, o, 64));
   for (int c = 0; c < ndmax; ++c) {
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_rx_level0(RxArgs a, int32_t* __
         rx_dep<MODE>(rc, v, c, dv, cf);
       } else {
         dv = a.odon[(int64_t)(c - RX_SD) * N + v];
-        cf = (double)a.ocoef[(int64_t)(c - RX_SD) * N + v];
+        cf = a.ocoef[(int64_t)(c - RX_SD) * N + v];
       }
       if (dv >= 0) {
         const long long q = rx_quant(cf * d, inv, ctl);
@@ -548,8 +553,8 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_rx_chain(RxArgs a, const int3
     int nd = 0;
     if (solved) {
       d = rx_value<MODE>(rc, S, unit);
-      a.x[v] = a.x[v] + d;
-      nd = MODE == 0 ? 3 : rx_hi(rc.w[1]);
+      a.dx[v] = d;
+      nd = MODE == 0 ? 3 : rx_lo(rc.w[2]);
 #pragma unroll
       for (int c = 0; c < PF; ++c) {
         double cf = 0.0;
@@ -607,7 +612,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_rx_chain(RxArgs a, const int3
             rx_dep<MODE>(rc, v, c, dc, cf);
           } else {
             dc = a.odon[(int64_t)(c - RX_SD) * N + v];
-            cf = (double)a.ocoef[(int64_t)(c - RX_SD) * N + v];
+            cf = a.ocoef[(int64_t)(c - RX_SD) * N + v];
           }
           if (dc >= 0) {
             const long long q = rx_quant(cf * d, inv, ctl) - RX_ONE;
@@ -655,12 +660,15 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_rx_chain(RxArgs a, const int3
   }
 }
 
-// Vertices a sweep did not reach (a receiver cycle and what lies below it):
-// counted; any makes the solve fall back to the exact FP64 sweep.
-__global__ void k_rx_unsolved(int64_t N, const long long* __restrict__ rec, RxCtl* __restrict__ ctl) {
+// x += dx at the vertices the sweep solved; the ones it did not reach (a
+// receiver cycle and what lies below it) are counted, and any makes the solve
+// fall back to the exact FP64 sweep.
+__global__ void k_rx_unsolved(int64_t N, const long long* __restrict__ rec, const double* __restrict__ dx,
+                              double* __restrict__ x, RxCtl* __restrict__ ctl) {
   if (ctl->done) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool left = i < N && rx_cnt(rec[i * RX_W]) != 0;
+  if (i < N && !left) x[i] = x[i] + dx[i];
   const unsigned long long m = __ballot(left);
   if (m != 0ull && __lane_id() == 0) {
     atomicAdd(reinterpret_cast<unsigned long long*>(&ctl->unsolved), (unsigned long long)__popcll(m));
@@ -702,26 +710,34 @@ int refine_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const do
   int32_t* counts = m->counts;
   long long* rec = reinterpret_cast<long long*>(m->wrec);
   int32_t* odon = m->wdon;
-  float* ocoef = reinterpret_cast<float*>(m->wacc);
+  double* ocoef = m->wacc;
+  double* dx = m->wd[1];
   GOSPL_CHECK(hipMemsetAsync(ctl, 0, sizeof(RxCtl), s));
   GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
   k_rx_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, &ctl->nonnbr);
   GOSPL_CHECK(hipGetLastError());
   ZeroList zl{{counts, m->chain_ctl, nullptr, nullptr}, {2, CQ_WORDS, 0, 0}, 2};
-  RxArgs a{N, rec, x, odon, ocoef, ctl};
-  if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
-  for (int pass = 0; pass <= maxpass; ++pass) {
+  int32_t* cnt0 = m->wi[0];
+  RxArgs a{N, rec, x, dx, odon, ocoef, cnt0, ctl};
+  auto residual = [&](int pass) {
     const int build = pass < maxpass;
-    if (mode == 0)
+    if (mode == 0) {
       k_rx_resid<0><<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rowmask, rcv, coef, diag, rhs, x, build, rec, odon,
-                                              ocoef, part, ctl, zl);
-    else
+                                              ocoef, cnt0, part, ctl, zl);
+      k_rx_norm<0><<<1, GOSPL_BLOCK, 0, s>>>(g, part, rtol, pass, maxpass, ctl);
+    } else {
       k_rx_resid<1><<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rowmask, rcv, coef, diag, rhs, x, build, rec, odon,
-                                              ocoef, part, ctl, zl);
-    if (mode == 0) k_rx_norm<0><<<1, GOSPL_BLOCK, 0, s>>>(g, part, rtol, pass, maxpass, ctl);
-    else k_rx_norm<1><<<1, GOSPL_BLOCK, 0, s>>>(g, part, rtol, pass, maxpass, ctl);
-    GOSPL_CHECK(hipGetLastError());
-    if (pass == maxpass) break;
+                                              ocoef, cnt0, part, ctl, zl);
+      k_rx_norm<1><<<1, GOSPL_BLOCK, 0, s>>>(g, part, rtol, pass, maxpass, ctl);
+    }
+  };
+  // pass 0's residual decides whether the first correction runs; after each
+  // correction the next residual is read back (one synchronisation per pass
+  // run), so only the passes that run are launched and timed
+  residual(0);
+  GOSPL_CHECK(hipGetLastError());
+  for (int pass = 0; pass < maxpass; ++pass) {
+    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
     if (mode == 0) {
       k_rx_level0<0><<<g, GOSPL_BLOCK, 0, s>>>(a, list, counts + 1);
       k_rx_chain<0><<<m->chain_blocks[0], CHAIN_THREADS, 0, s>>>(a, list, counts + 1, m->chain_gq, gq_cap,
@@ -733,21 +749,24 @@ int refine_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const do
                                                                 m->chain_ctl, m->chain_pool, m->chain_pollers,
                                                                 m->chain_spin);
     }
-    k_rx_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, rec, ctl);
+    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
+    k_rx_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, rec, dx, x, ctl);
     k_rx_check<<<1, 64, 0, s>>>(ctl);
+    residual(pass + 1);
     GOSPL_CHECK(hipGetLastError());
+    GOSPL_CHECK(hipMemcpyAsync(m->rx_hctl, ctl, sizeof(RxCtl), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, m->chain_ctl + CQ_ABORT, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    const RxCtl* hc = reinterpret_cast<const RxCtl*>(m->rx_hctl);
+    if (m->timing && hc->pass > pass) {
+      float ms = 0.0f;
+      GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
+      m->t_ms[2 + mode] += ms;
+      m->t_calls[2 + mode] += 1;
+    }
+    if (hc->done || m->hflag[34]) break;
   }
-  if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[1], s));
-  GOSPL_CHECK(hipMemcpyAsync(m->rx_hctl, ctl, sizeof(RxCtl), hipMemcpyDeviceToHost, s));
-  GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, m->chain_ctl + CQ_ABORT, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  GOSPL_CHECK(hipStreamSynchronize(s));
   const RxCtl* h = reinterpret_cast<const RxCtl*>(m->rx_hctl);
-  if (m->timing) {
-    float ms = 0.0f;
-    GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
-    m->t_ms[2 + mode] += ms;
-    m->t_calls[2 + mode] += 1;
-  }
   if (h->nonnbr > 0) {
     set_error("%lld receivers are not neighbours of their donor (each receiver must be in the donor's "
               "neighbour list)", (long long)h->nonnbr);

@@ -94,19 +94,21 @@ class Model:
     or None), ``sedfac`` (per-vertex K factor or None), ``sealevel``,
     ``ofill`` (oFill), ``tstart``/``tend``.
 
-    ``refine=True`` selects the refined solves for flow accumulation and
-    stream-power erosion (``gospl_flowacc_refine`` / ``gospl_spl_refine``:
-    fixed-point correction sweeps and FP64 residuals, iterated to
-    ``||r|| <= refine_rtol ||rhs||``, at most ``refine_maxpass`` corrections;
-    the flow solves warm-start from their previous solutions as goSPL's KSP
-    does) and, for hillslope diffusion, FP32 Jacobi corrections of an FP64
-    residual.  State, fill, pits and receivers are FP64 / exact either way.
+    ``refine=True`` solves hillslope diffusion by FP32 Jacobi corrections of
+    an FP64 residual (``gospl_hillslope_refine``, to ``||r|| <= refine_rtol
+    ||h||``); ``refine_dag=True`` also solves flow accumulation and
+    stream-power erosion by refinement (``gospl_flowacc_refine`` /
+    ``gospl_spl_refine``: fixed-point correction sweeps and FP64 residuals,
+    at most ``refine_maxpass`` corrections; the flow solves warm-start from
+    their previous solutions as goSPL's KSP does) instead of the exact FP64
+    sweeps, which are direct solves.  State, fill, pits and receivers are
+    FP64 / exact either way.
     """
 
     def __init__(self, mesh, h0, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
-                 tend=None, device="cuda", verbose=False, refine=False, refine_rtol=1.0e-12,
-                 refine_maxpass=4):
+                 tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
+                 refine_rtol=1.0e-12, refine_maxpass=4):
         self.dm = mesh if isinstance(mesh, DeviceMesh) else DeviceMesh(mesh, device)
         self.lib = self.dm.lib
         self.H = self.dm.handle
@@ -182,14 +184,16 @@ class Model:
         self.timers = {}
         self.profile = False
         self._events = []
-        self.refine = bool(refine)
+        self.refine_dag = bool(refine_dag)
+        self.refine = bool(refine) or self.refine_dag
         self.refine_rtol = float(refine_rtol)
         self.refine_maxpass = int(refine_maxpass)
         # warm starts of the refined flow solves (main solve, cascade solves)
-        self._ws_main = torch.zeros(N, **f64) if self.refine else None
-        self._ws_casc = torch.zeros(N, **f64) if self.refine else None
+        self._ws_main = torch.zeros(N, **f64) if self.refine_dag else None
+        self._ws_casc = torch.zeros(N, **f64) if self.refine_dag else None
         # per solve: (entry point, correction passes (-1: exact fallback), relative residual)
         self.solve_log = collections.deque(maxlen=4096)
+        self.refine_stats = False  # also log the hillslope passes (one read-back per call)
 
     # -------------------------------------------------------------- helpers
     @property
@@ -263,6 +267,18 @@ class Model:
         n = int(math.ceil(math.log(2.0 ** -60) / math.log(rho))) + 1
         return max(2, n) if n <= 64 else 0
 
+    def hillslope_sweeps32(self):
+        """FP32 Jacobi sweeps per correction of the refined hillslope solve:
+        the contraction bound brought below 2^-27 (FP32 rounding of the
+        correction); 0 (the converged FP64 solve) past 64 sweeps."""
+        kd = max(self.Cda, self.Cdm) * self.dt
+        s = kd * self.dm.gmax
+        if s <= 0.0:
+            return 1
+        rho = s / (1.0 + s)
+        n = int(math.ceil(math.log(2.0 ** -27) / math.log(rho))) + 1
+        return max(2, n) if n <= 64 else 0
+
     # -------------------------------------------------------------- pits
     def fillElevation(self, sed=False):
         """pitfilling.py:849-903 (water filling, sed=False)."""
@@ -308,7 +324,7 @@ class Model:
         Refined mode: ``ws`` holds the warm start and receives the solution."""
         lev = ctypes.c_int32(0)
         bad = ctypes.c_int64(0)
-        if self.refine:
+        if self.refine_dag:
             npass = ctypes.c_int32(0)
             rr = ctypes.c_double(0.0)
             out.copy_(ws)
@@ -431,7 +447,7 @@ class Model:
     # -------------------------------------------------------------- erosion
     def erodepSPL(self):
         lev = ctypes.c_int32(0)
-        if self.refine:
+        if self.refine_dag:
             npass = ctypes.c_int32(0)
             rr = ctypes.c_double(0.0)
             self._call("gospl_spl_refine", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
@@ -453,6 +469,19 @@ class Model:
         if self.Cda == 0.0 and self.Cdm == 0.0:
             # goSPL then adds the erosion step's thickness again (hillslope.py:638-643)
             self.EbLocal += self.Eb * self.dt / self.dt
+            return
+        if self.refine:
+            if self.refine_stats:
+                npass = ctypes.c_int32(0)
+                rr = ctypes.c_double(0.0)
+                pp, rp = ctypes.byref(npass), ctypes.byref(rr)
+            else:  # no read-back, no synchronisation
+                pp = rp = None
+            self._call("gospl_hillslope_refine", self.H, self.Cda, self.Cdm, self.dt, self.sealevel,
+                       self.hillslope_sweeps32(), self.refine_rtol, self.refine_maxpass, _ptr(self.hLocal),
+                       _ptr(self.cumED), _ptr(self.EbLocal), pp, rp, self.stream)
+            if self.refine_stats:
+                self.solve_log.append(("hillslope", int(npass.value), float(rr.value)))
             return
         self._call("gospl_hillslope", self.H, self.Cda, self.Cdm, self.dt, self.sealevel,
                    self.hillslope_sweeps(), _ptr(self.hLocal), _ptr(self.cumED), _ptr(self.EbLocal),

@@ -225,6 +225,20 @@ int gospl_hillslope(gospl_mesh_t* mesh, double cda, double cdm, double dt, doubl
                     gospl_stream_t stream);
 
 /*
+ * Refined hillslope solve (the reduced-cost mode of gospl_hillslope; no
+ * counterpart in goSPL): passes of r = h - M x in FP64 (x starts at h),
+ * `sweeps32` FP32 Jacobi sweeps of M d = r on an FP32 copy of the kept
+ * operator, x += d, until ||r||_2 <= rtol ||h||_2 or maxpass corrections.
+ * sweeps32 <= 0 runs the converged FP64 solve of gospl_hillslope instead
+ * (*passes = -1).  *passes / *relres (HOST, may both be NULL: then nothing is
+ * read back and the stream is not synchronised) receive the corrections run
+ * and the last relative residual.
+ */
+int gospl_hillslope_refine(gospl_mesh_t* mesh, double cda, double cdm, double dt, double sealevel,
+                           int32_t sweeps32, double rtol, int32_t maxpass, double* h, double* cumed,
+                           double* eblocal, int32_t* passes, double* relres, gospl_stream_t stream);
+
+/*
  * Hillslope operator coefficients: replaces `sethillslopecoeff`
  * (functions.pyf:10-14, functions.F90:901-938).  kd (N) = diffusivity * dt
  * per vertex; coeff (13, N): row 0 the diagonal 1 + sum v, row 1 + p the

@@ -152,6 +152,7 @@ def run_leg(m, h0, p, dev, name, steps, warmup):
     f64p, i64p = ctypes.c_double, ctypes.c_int64
     z = [f64p(0), i64p(0), f64p(0), i64p(0)]
     lib.gospl_sweep_timing(g.H, 1, *[ctypes.byref(v) for v in z])
+    first = dict(Model.sweeps)  # process-wide sweep launches before the profiled window
     g.enable_profiling(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -161,13 +162,14 @@ def run_leg(m, h0, p, dev, name, steps, warmup):
     prof_elapsed = time.perf_counter() - t1
     ph = g.phase_times()
     g.enable_profiling(False)
+    window = {k: (first[k], Model.sweeps[k]) for k in first}
     sw = [f64p(0), i64p(0), f64p(0), i64p(0)]
     lib.gospl_sweep_timing(g.H, 0, *[ctypes.byref(v) for v in sw])
     g.close()
     phases = {k: dict(ms_total=v[0], calls=v[1], ms_per_call=v[0] / v[1]) for k, v in ph.items()}
     log("[bench] %s phases (ms per step): " % name + ", ".join(
         "%s=%.3f" % (k, v["ms_total"] / steps) for k, v in sorted(phases.items(), key=lambda x: -x[1]["ms_total"])))
-    return dict(sps=sps, elapsed=elapsed, phases=phases, prof_elapsed=prof_elapsed, solves=solves,
+    return dict(sps=sps, elapsed=elapsed, phases=phases, prof_elapsed=prof_elapsed, solves=solves, window=window,
                 sweep=dict(fa_ms=sw[0].value, fa_calls=sw[1].value, spl_ms=sw[2].value, spl_calls=sw[3].value))
 
 
@@ -247,6 +249,37 @@ def accuracy(m, h0, p, dev, nsteps, legs):
     return out
 
 
+def rocpd_summary(db, first, count, mode, N):
+    """Markdown summary of a rocprofv3 database (--kernel-trace): kernels by
+    total time, and the exact sweep dispatches first .. first + count - 1
+    (level 0 + dataflow kernel of `mode`, in launch order) with the roofline
+    they give at the workload's algorithmic bytes."""
+    import sqlite3
+
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name "
+                       "order by sum(duration) desc").fetchall()
+    out = ["| kernel | calls | total ms | mean us |", "|---|---|---|---|"]
+    for name, n, tot, avg in rows[:30]:
+        out.append("| `%s` | %d | %.2f | %.1f |" % (name.split("(")[0].replace("void ", "")[:70], n, tot / 1e6,
+                                                  avg / 1e3))
+    key = {0: ("k_dag_level0<0, true>", "k_chain<0>"), 1: ("k_dag_level0<1, true>", "k_chain<1>")}[mode]
+    lv = [r[0] for r in cur.execute("select duration from kernels where name like ? order by start",
+                                    ("%gospl::" + key[0] + "%",))]
+    ch = [r[0] for r in cur.execute("select duration from kernels where name like ? order by start",
+                                    ("%gospl::" + key[1] + "(%",))]
+    lv, ch = lv[first:first + count], ch[first:first + count]
+    if lv and ch:
+        ml, mc = np.mean(lv) / 1e3, np.mean(ch) / 1e3
+        ab = {0: 68, 1: 116}[mode] * N
+        gbs = ab / ((ml + mc) * 1e-6) / 1e9
+        out += ["", "Window: sweep dispatches %d-%d (%d launches): `%s` mean %.1f us + `%s` mean %.1f us = "
+                "%.1f us; algorithmic bytes %d x %d = %.1f MB -> %.1f GB/s = %.4f of %.0f GB/s." % (
+                    first, first + len(ch) - 1, len(ch), key[0], ml, key[1], mc, ml + mc, ab // N, N, ab / 1e6,
+                    gbs, gbs / PEAK_HBM_GBS, PEAK_HBM_GBS)]
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,9 +296,19 @@ def main():
                     help="initial surface (input order) saved by --save-surface instead of the SH field "
                          "(keeps the SH construction's kernels out of a counter run)")
     ap.add_argument("--save-surface", default=None, metavar="NPY")
+    ap.add_argument("--rocpd-summary", default=None, metavar="DB",
+                    help="print a markdown summary of a rocprofv3 --kernel-trace database and exit")
+    ap.add_argument("--window", default="0:0", metavar="FIRST:COUNT",
+                    help="sweep dispatches (launch order) whose mean the summary reports")
+    ap.add_argument("--sweep-mode", type=int, default=0, help="0: flow sweeps, 1: stream-power sweeps")
+    ap.add_argument("--vertices", type=int, default=2621442)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.rocpd_summary:
+        f, c = (int(x) for x in args.window.split(":"))
+        print(rocpd_summary(args.rocpd_summary, f, c, args.sweep_mode, args.vertices))
+        return
     if args.copy_probe:
         import torch
 
@@ -296,8 +339,12 @@ def main():
     L = legs[main_leg]
     sps = L["sps"]
     roof = dominant_kernel(L, N, args.level)
-    roof["window"] = "steps %d-%d of the %s leg (after %d warm-up and %d timed steps)" % (
-        args.warmup + args.steps + 1, args.warmup + 2 * args.steps, main_leg, args.warmup, args.steps)
+    sys_ = "flow" if roof["entry_point"] in ("gospl_flowacc", "gospl_flowacc_refine") else "spl"
+    a0, a1 = L["window"][sys_]
+    roof["window"] = ("steps %d-%d of the %s leg (after %d warm-up and %d timed steps); exact %s sweeps "
+                      "%d-%d of the process (0-based dispatch order of the sweep kernels)" % (
+                          args.warmup + args.steps + 1, args.warmup + 2 * args.steps, main_leg, args.warmup,
+                          args.steps, sys_, a0, a1 - 1))
     roof["note"] = ("latency-bound: ~600 dependent hops per sweep (the longest flow path); achieved = "
                     "algorithmic bytes per launch / mean launch duration (HIP events on the launch stream)")
     log("[bench] dominant kernel %s: %.3f ms per launch over %d launches (%.1f GB/s)" %

@@ -87,6 +87,10 @@ class DeviceMesh:
 class Model:
     """Per-step hot path of goSPL on one GPU (FP64 state).
 
+    ``Model.sweeps`` counts, process-wide, the exact sweeps launched by every
+    model (keys ``"flow"``, ``"spl"``): one launch each of the level-0 and the
+    dataflow kernel, so a profile's dispatches can be matched to a window.
+
     ``mesh`` is a ``gospl_amd.mesh.Mesh`` (or ``DeviceMesh``), ``h0`` the
     initial elevation in device order; keyword parameters follow goSPL's YAML
     keys: ``K, m, d`` (spl), ``flowdir``, ``cda/cdm`` (hillslopeKa/Km),
@@ -104,6 +108,8 @@ class Model:
     sweeps, which are direct solves.  State, fill, pits and receivers are
     FP64 / exact either way.
     """
+
+    sweeps = {"flow": 0, "spl": 0}
 
     def __init__(self, mesh, h0, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
@@ -337,6 +343,7 @@ class Model:
         else:
             self._call("gospl_flowacc", self.H, self.flowDir, _ptr(self.rcvID), _ptr(self.wghtVal), _ptr(b),
                        _ptr(out), ctypes.byref(lev), ctypes.byref(bad), self.stream)
+            Model.sweeps["flow"] += 1
         self.undrained = int(bad.value)
         if bad.value:
             if fatal:
@@ -458,6 +465,7 @@ class Model:
             self.solve_log.append(("spl", int(npass.value), float(rr.value)))
             self.spl_levels = 0
             return
+        Model.sweeps["spl"] += 1
         self._call("gospl_spl", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
                    _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.dt,
                    self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),

@@ -22,15 +22,18 @@ DAGs, Jacobi to a fixed point for the diffusion operator) where goSPL stops
 its Krylov solvers at rtol 1e-8; parity of the HIP path is defined against
 these exact solutions.
 
-Parity unpinned: the parts of this glue that goSPL implements with numpy,
-pandas, numpy_indexed or PETSc (absent here, so no golden vector can be
-written) — the spill-point choice and flat receivers of _pitInformation /
-_dirFlats, the per-pit sums of _getPitParams and of the cascade inflow, the
-cascade bookkeeping of _distributeDownstream, and the Krylov solutions
-themselves.  They are restated from source text; the flow and erosion glue is
-checked against goSPL's analytic stream-power steady state
-(benchmarks/test_spl.py:443-577), the rest by invariants only
-(tests/test_oracle_pins.py).
+The parts of this glue that goSPL implements with numpy, pandas,
+numpy_indexed or PETSc (absent here, so no golden vector can be written) are
+pinned by their definitions, restated with plain loops in
+tests/test_oracle_glue_pins.py: spill points and flat routing (brute force,
+BFS), per-pit tables (goSPL's own Fortran getpitvol, brute-force maxima), the
+cascade bookkeeping (per-pass inflows, capacities, water levels, spills and
+the whole call's water balance) and the end of flowAccumulation.
+
+Parity unpinned: the Krylov solutions themselves.  goSPL stops FGMRES /
+Richardson at rtol 1e-8 (flowplex.py:38, hillslope.py:362-371); the oracle
+solves exactly, so an actual goSPL run agrees with it only to that
+tolerance.
 """
 
 from __future__ import annotations

@@ -7,7 +7,12 @@ points are run on the same inputs and compared with those outputs without
 going through the oracle: receivers, distances, filled elevations, pit ids,
 flat-routing distances and hillslope coefficients bit for bit; receiver
 weights to the last bit or two (the device pow may differ from the host's by
-1 ulp); depth-volume sums to 1e-12 (summation order).
+1 ulp); depth-volume sums to 1e-12 (summation order).  One column is the
+oracle's: the spill cells (goSPL selects them in numpy, pitfilling.py:
+756-801, so make_golden.py writes the oracle's choice, itself checked
+against the definition by brute force in test_oracle_glue_pins.py); the
+Fortran fill_dir/nghb_dir distances stored beside them start from those
+cells.
 
 On a flat mesh goSPL replaces the Voronoi areas by the mesh's control volumes
 when they differ by more than 1e2 m2 (unstructuredmesh.py:172-188), and the

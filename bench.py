@@ -104,19 +104,45 @@ def build_workload(level, seed, device, surface=None, save_surface=None):
     return m, h0, p
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(m, h0, p, steps):
-    """The FP64 oracle (serial C + numpy, exact solves) timed on this host."""
+    """The CPU restatement of goSPL's path in reference-solver mode (one rank:
+    heap priority-flood fill, the mfdreceivers kernel, FGMRES(30) + ILU(0) at
+    rtol 1e-8 for the flow and stream-power systems, Richardson + ILU(0) for
+    hillslope, goSPL's cascade glue), timed on this host single-threaded and
+    with OpenMP on the host threads available (sparse products and vector
+    updates; the ILU factorisation and solves stay sequential, as PETSc's)."""
     from oracle import oracle as orc
     from gospl_amd import workloads
 
     conv = {k: (m.to_input_order(v) if isinstance(v, np.ndarray) else v) for k, v in p.items()}
-    o = orc.OracleModel(workloads.oracle_geometry(m), m.to_input_order(h0), tend=1e12, **conv)
-    o.initialise()
+    nthr = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = orc.ReferenceSolverModel(workloads.oracle_geometry(m), m.to_input_order(h0), tend=1e12, threads=1,
+                                 **conv)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        o.step()
-    dt = time.perf_counter() - t0
-    return steps / dt, dt
+    o.initialise()
+    log("[bench] cpu baseline: initialise %.1fs" % (time.perf_counter() - t0))
+    res = {}
+    for th in sorted({1, nthr}):
+        o.threads = th
+        o.solver_its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step()
+        dt = time.perf_counter() - t0
+        res[th] = dict(steps_per_s=steps / dt, s_per_step=dt / steps,
+                       solver_iterations=[(k, i) for k, i, _ in o.solver_its])
+        log("[bench] cpu baseline: %d step(s) on %d thread(s) in %.1fs" % (steps, th, dt))
+    return res, nthr
 
 
 LEGS = {"refined": dict(refine=True), "refined_dag": dict(refine=True, refine_dag=True), "fp64": {}}
@@ -290,7 +316,7 @@ def main():
     ap.add_argument("--legs", default="refined,refined_dag,fp64", help="comma list of " + ", ".join(LEGS))
     ap.add_argument("--acc-steps", type=int, default=100,
                     help="steps of the accuracy comparison (0: skip)")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="timed CPU-baseline steps per thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--surface", default=None, metavar="NPY",
                     help="initial surface (input order) saved by --save-surface instead of the SH field "
@@ -359,12 +385,21 @@ def main():
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
-        cps, cdt = cpu_baseline(m, h0, p, args.cpu_steps)
-        log("[bench] cpu oracle %d steps in %.1fs (%.3f steps/s), total %.1fs" %
-            (args.cpu_steps, cdt, cps, time.time() - t0))
-        cpu = {"value": cps, "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": "FP64 oracle (C + numpy, exact DAG solves), %d steps of the same %d-vertex "
-                         "workload after its init flow accumulation, one host core" % (args.cpu_steps, N)}
+        res, nthr = cpu_baseline(m, h0, p, args.cpu_steps)
+        log("[bench] cpu baseline total %.1fs" % (time.time() - t0))
+        one = res[1]
+        cpu = {"value": one["steps_per_s"], "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": ("CPU restatement of goSPL's path in reference-solver mode (1 rank: heap "
+                          "priority-flood fill, mfdreceivers, FGMRES(30)+ILU(0) rtol 1e-8 for flow and "
+                          "stream power, Richardson+ILU(0) rtol 1e-8 for hillslope, goSPL's cascade "
+                          "glue), %d step(s) of the same %d-vertex workload after its init flow "
+                          "accumulation, single-threaded; 100 steps extrapolate to %.0f s" %
+                          (args.cpu_steps, N, 100.0 * one["s_per_step"])),
+               "ranks": 1, "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+               "solver_iterations": one["solver_iterations"]}
+        if nthr > 1:
+            cpu["threaded"] = {"threads": nthr, "value": res[nthr]["steps_per_s"],
+                               "solver_iterations": res[nthr]["solver_iterations"]}
 
     leg_cfg = {}
     for name, lg in legs.items():

@@ -593,3 +593,111 @@ class OracleModel:
             if self.tNow == self.tEnd:
                 break
             self.step()
+
+
+# --------------------------------------------------------------------------
+# Reference-solver mode (CPU baseline): goSPL's solvers instead of exact solves
+# --------------------------------------------------------------------------
+
+def _rs_lib():
+    L = lib()
+    if not getattr(L, "_rs_declared", False):
+        P, I64, I32, F64 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.rs_flow_solve.argtypes = [I64, I32, P, P, P, P, F64, I32, I32, dp]
+        L.rs_flow_solve.restype = I32
+        L.rs_spl_solve.argtypes = [I64, I32, P, P, P, P, P, F64, I32, I32, dp]
+        L.rs_spl_solve.restype = I32
+        L.rs_hill_solve.argtypes = [I64, P, P, P, P, P, F64, I32, I32, dp]
+        L.rs_hill_solve.restype = I32
+        L._rs_declared = True
+    return L
+
+
+class ReferenceSolverModel(OracleModel):
+    """The oracle with goSPL's own solver choices in place of exact solves,
+    as goSPL runs on one MPI rank (PETSc block Jacobi = one ILU(0) block):
+    FGMRES(30) + ILU(0) at rtol 1e-8 for the flow systems, max_it 5000 (1000
+    for cascade solves), warm-started from the previous solution or seeded
+    with the right-hand side (flowplex.py:323-427); the same for the
+    stream-power system, warm-started from the previous solution
+    (SPL.py:260-366); Richardson + ILU(0) at rtol 1e-8 for hillslope
+    diffusion, started from h (hillslope.py:347-371).  Fill (heap
+    priority-flood) and receivers are the oracle's serial kernels.  `threads`
+    OpenMP threads run the sparse products and vector updates."""
+
+    def __init__(self, geom, h, *, threads=1, rtol=1.0e-8, **kw):
+        super().__init__(geom, h, **kw)
+        self.threads = int(threads)
+        self.rtol = float(rtol)
+        self._fa_prev = None
+        self._cq_prev = None
+        self._stepED = None
+        self.solver_its = []
+
+    def _solve_flow(self, b, fatal=True):
+        L = _rs_lib()
+        prev = self._fa_prev if fatal else self._cq_prev
+        x = np.array(b, dtype=np.float64, copy=True) if prev is None or not np.any(prev) else prev.copy()
+        rr = ctypes.c_double(0.0)
+        it = L.rs_flow_solve(self.lpoints, self.flowDir, _p(_c(self.rcvID, np.int32)),
+                             _p(_c(self.wghtVal, np.float64)), _p(_c(b, np.float64)), _p(x), self.rtol,
+                             5000 if fatal else 1000, self.threads, ctypes.byref(rr))
+        self.solver_its.append(("flow", int(it), float(rr.value)))
+        self.undrained = 0 if it >= 0 else 1
+        if it < 0:
+            if fatal:
+                raise RuntimeError("FGMRES did not converge on the flow system (relres %.3e)" % rr.value)
+            x = np.zeros_like(x)
+        if fatal:
+            self._fa_prev = x.copy()
+        else:
+            self._cq_prev = x.copy()
+        return x
+
+    def erodepSPL(self):
+        L = _rs_lib()
+        hOld = self.h.copy()
+        off, diag = self._eroMats(hOld)
+        x = (self._stepED if self._stepED is not None else hOld).copy()
+        rr = ctypes.c_double(0.0)
+        it = L.rs_spl_solve(self.lpoints, self.flowDir, _p(_c(self.rcvIDi, np.int32)), _p(_c(off, np.float64)),
+                            _p(_c(diag, np.float64)), _p(hOld), _p(x), self.rtol, 5000, self.threads,
+                            ctypes.byref(rr))
+        self.solver_its.append(("spl", int(it), float(rr.value)))
+        if it < 0:
+            raise RuntimeError("FGMRES did not converge on the stream-power system")
+        self._stepED = x.copy()
+        E = (x - hOld) / self.dt
+        if self.flatModel:
+            E[self.outletIDs] = 0.0
+        E[self.lsink] = 0.0
+        self.Eb = E
+        self.tmp = E * self.dt
+        self.cumED = self.cumED + self.tmp
+        self.h = self.h + self.tmp
+        self.EbLocal = self.tmp / self.dt
+
+    def getHillslope(self):
+        L = _rs_lib()
+        h = self.h.copy()
+        self.seaID = np.where(h <= self.sealevel)[0]
+        if not (self.Cda == 0.0 and self.Cdm == 0.0):
+            Cd = np.full(self.lpoints, self.Cda)
+            Cd[self.seaID] = self.Cdm
+            coeff = sethillslopecoeff(Cd * self.dt, self.ngb, self.nnb, self.larea, self.elgt, self.vdist)
+            if self.flatModel:
+                coeff[self.outletIDs, 1:] = 0.0
+                coeff[self.outletIDs, 0] = 1.0
+            hOld = self.h.copy()
+            x = hOld.copy()
+            rr = ctypes.c_double(0.0)
+            it = L.rs_hill_solve(self.lpoints, _p(self.ngb), _p(self.nnb), _p(_c(coeff, np.float64)), _p(hOld),
+                                 _p(x), self.rtol, 100000, self.threads, ctypes.byref(rr))
+            self.solver_its.append(("hillslope", int(it), float(rr.value)))
+            if it < 0:
+                raise RuntimeError("Richardson did not converge on the diffusion system")
+            self.h = x
+            self.tmp = self.h - hOld
+            self.cumED = self.cumED + self.tmp
+        self.EbLocal = self.EbLocal + self.tmp / self.dt

@@ -76,6 +76,12 @@ def load(path: str | None = None):
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # torch first: the library takes torch's streams and device pointers, so it
+    # must bind to the HIP runtime torch loads (its bundled libamdhip64.so.7).
+    # Loading this library first would bring in the system runtime of the
+    # same soname, and torch would then resolve to it, beside its own HSA
+    # runtime, and find no device.
+    import torch  # noqa: F401
     if not os.path.exists(p):
         raise RuntimeError(
             "libgospl_hip.so not found at %s: build it with "

@@ -23,6 +23,7 @@ from __future__ import annotations
 import collections
 import ctypes
 import math
+import os
 import time
 
 import numpy as np
@@ -111,10 +112,26 @@ class Model:
 
     sweeps = {"flow": 0, "spl": 0}
 
-    def __init__(self, mesh, h0, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
+    def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
                  tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
-                 refine_rtol=1.0e-12, refine_maxpass=4):
+                 refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None):
+        if isinstance(mesh, (str, os.PathLike)):
+            # goSPL's constructor, Model(filename, verbose, showlog, profile)
+            # (gospl/model.py:212-304): the YAML file and its npz mesh
+            flags = ([h0] if h0 is not None else []) + list(args)
+            verbose = bool(flags[0]) if len(flags) > 0 else verbose
+            showlog = bool(flags[1]) if len(flags) > 1 else showlog
+            profile = bool(flags[2]) if len(flags) > 2 else profile
+            (m_, h_), params = Model._from_file(mesh, device)
+            self.__init__(m_, h_, device=device, verbose=verbose, showlog=showlog, profile=profile,
+                          refine=refine, refine_dag=refine_dag, refine_rtol=refine_rtol,
+                          refine_maxpass=refine_maxpass, **params)
+            return
+        if h0 is None or K is None:
+            raise TypeError("Model(mesh, h0, K=..., ...) or Model(filename)")
+        self.showlog = bool(showlog)
+        self._schedule = schedule or {}
         self.dm = mesh if isinstance(mesh, DeviceMesh) else DeviceMesh(mesh, device)
         self.lib = self.dm.lib
         self.H = self.dm.handle
@@ -190,6 +207,8 @@ class Model:
         self.timers = {}
         self.profile = False
         self._events = []
+        if profile:
+            self.enable_profiling(True)
         self.refine_dag = bool(refine_dag)
         self.refine = bool(refine) or self.refine_dag
         self.refine_rtol = float(refine_rtol)
@@ -200,8 +219,69 @@ class Model:
         # per solve: (entry point, correction passes (-1: exact fallback), relative residual)
         self.solve_log = collections.deque(maxlen=4096)
         self.refine_stats = False  # also log the hillslope passes (one read-back per call)
+        self._sched_state = {}
+
+    @staticmethod
+    def _from_file(filename, device):
+        """(mesh, h0, parameters) of goSPL's YAML + npz input
+        (gospl_amd.inputs.read_input; the mesh in device order)."""
+        from . import _lib as lib_, inputs, mesh as mesh_
+
+        inp = inputs.read_input(filename)
+        lib = lib_.load()
+        bverts = mesh_.boundary_vertices(inp["cells"])
+        flat = len(bverts) > 0
+        outlet = inputs.border_outlets(inp["coords"], inp["cells"], inp["bc"], bverts) if flat else None
+        m = mesh_.build_mesh(inp["coords"], inp["cells"], lib, flat=flat, outlet_mask=outlet)
+        dev = lambda a: m.to_device_order(a) if isinstance(a, np.ndarray) else a
+        sched = dict(climate=[(t, dev(v)) for t, v in inp["climate"]],
+                     tectonics=[(t0, t1, dev(u) if u is not None else None) for t0, t1, u in inp["tectonics"]],
+                     sedfactor=[(t, dev(v)) for t, v in inp["sedfactor"]])
+        first = lambda evs, t, dflt: next((v for s, v in reversed(evs) if s <= t), dflt)
+        params = dict(K=inp["K"], m=inp["m"], d=inp["d"], flowdir=inp["flowdir"], cda=inp["cda"], cdm=inp["cdm"],
+                      dt=inp["dt"], rain=first(sched["climate"], inp["tstart"], 0.0),
+                      sedfac=first(sched["sedfactor"], inp["tstart"], None), sealevel=inp["sealevel"],
+                      ofill=inp["ofill"], tstart=inp["tstart"], tend=inp["tend"], schedule=sched)
+        return (m, m.to_device_order(inp["elev"])), params
+
+    def _apply_schedule(self):
+        """The forcing events active at tNow (goSPL's _updateRain, the
+        tectonics event table and _updateEroFactor, unstructuredmesh.py:
+        743-794, 903-972, 1063-1094): the latest climate / erodibility event
+        started, the tectonic event with start <= tNow < end."""
+        sch = self._schedule
+        if not sch:
+            return
+        N = self.dm.N
+        dev = self.device
+        vec = lambda v: torch.as_tensor(np.broadcast_to(np.asarray(v, np.float64), (N,)).copy()).to(dev)
+        ev = [v for s, v in sch.get("climate", []) if s <= self.tNow]
+        key = ("rain", len(ev))
+        if ev and self._sched_state.get("rain") != key:
+            self.rainVal = torch.clamp(vec(ev[-1]), min=0.0)
+            self._sched_state["rain"] = key
+            self._kfac = self._make_kfac()
+        ev = [v for s, v in sch.get("sedfactor", []) if s <= self.tNow]
+        key = ("sed", len(ev))
+        if ev and self._sched_state.get("sed") != key:
+            self.sedfacVal = vec(ev[-1])
+            self._sched_state["sed"] = key
+            self._kfac = self._make_kfac()
+        tec = [u for s, e, u in sch.get("tectonics", []) if s <= self.tNow < e]
+        self.upsub = vec(tec[-1]) if tec and tec[-1] is not None else None
 
     # -------------------------------------------------------------- helpers
+    @property
+    def FAG(self):
+        """Global discharge (on one rank the local vector, flowplex.py:906-933)."""
+        return self.FAL
+
+    @property
+    def seaID(self):
+        """Vertices whose filled elevation is at or below sea level, as the
+        last flow-direction build defines them (flowplex.py:493)."""
+        return torch.nonzero(self.lFill <= self.sealevel).flatten()
+
     @property
     def stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -322,6 +402,24 @@ class Model:
                    _ptr(self.lFill), 1 if down else 0, _ptr(self.pitIDs), _ptr(self.flatDirs),
                    _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
                    self.stream)
+
+    def _solve_KSP(self, guess, matrix, vector1, vector2, fatal=False, seed=False):
+        """goSPL's flow solve (flowplex.py:323-427) with its signature:
+        vector1 the right-hand side, vector2 the solution (and, with guess,
+        the initial guess of the refined solve; seed: a zero vector2 starts
+        from vector1).  The operator is the one built from the last flow
+        directions (rcvID, wghtVal); `matrix` is accepted for the signature
+        and must be None.  A receiver cycle that does not drain raises when
+        fatal, else zeroes vector2, as goSPL's fallback does."""
+        if matrix is not None:
+            raise ValueError("_solve_KSP builds its operator from the flow directions: pass matrix=None")
+        ws = None
+        if self.refine_dag:
+            ws = vector2.clone() if guess else torch.zeros_like(vector2)
+            if seed and not bool(ws.any()):
+                ws.copy_(vector1)
+        self._solve_flow(vector1, vector2, fatal=fatal, ws=ws)
+        return vector2
 
     def _solve_flow(self, b, out, fatal=True, ws=None):
         """_solve_KSP (flowplex.py:323-427) on the flow matrix.  A receiver
@@ -497,6 +595,7 @@ class Model:
 
     # -------------------------------------------------------------- driver
     def applyForces(self):
+        self._apply_schedule()
         self.bL = self.rainVal * self.dm.area
         if self.upsub is not None:
             self.hLocal += self.upsub * self.dt

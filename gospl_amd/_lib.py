@@ -44,6 +44,10 @@ SIGNATURES = {
     "gospl_hillslope_coeff": (I32, [P, P, P, P]),
     "gospl_hillslope_refine": (I32, [P, F64, F64, F64, F64, I32, F64, I32, P, P, P, ctypes.POINTER(I32),
                                      ctypes.POINTER(F64), P]),
+    "gospl_flowacc_adjoint": (I32, [P, I32, P, P, P, P, P]),
+    "gospl_spl_adjoint": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P, P, P, P]),
+    "gospl_hillslope_frozen": (I32, [P, F64, F64, F64, P, I32, P, P, P]),
+    "gospl_hillslope_adjoint": (I32, [P, F64, F64, F64, P, I32, P, P, P]),
     "gospl_rain_source": (I32, [P, P, P, F64, P, P]),
     "gospl_pit_inflow": (I32, [P, P, P, P, I64, P, P]),
     "gospl_pit_water": (I32, [P, P, P, P, P, P, P]),

@@ -379,6 +379,8 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   if (m->rx_hctl) (void)hipHostFree(m->rx_hctl);
   dfree(m->hdiag);
   dfree(m->hcoef32);
+  dfree(m->adj_hcoef);
+  dfree(m->adj_hdiag);
   dfree(m->hdiag32);
   dfree(m->hsea);
   delete m;

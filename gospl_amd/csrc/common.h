@@ -66,6 +66,8 @@ struct gospl_mesh {
   double* hdiag;              // its diagonal
   float* hcoef32;             // FP32 copies of both (refined solve's corrections)
   float* hdiag32;
+  double* adj_hcoef;          // hillslope rows from given sea flags (adjoint, frozen replay)
+  double* adj_hdiag;
   int8_t* hsea;               // the sea flags the rows were built from
   double hpar[3];             // cda, cdm, dt they were built with
   int hvalid;

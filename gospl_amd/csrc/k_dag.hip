@@ -1277,6 +1277,109 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
   eblocal[i] = t / dt;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoints of the receiver-DAG solves (gradients with receivers, weights,
+// discharge and sea/sink flags held at their forward values).
+// ---------------------------------------------------------------------------
+
+// (I - W) y = lam, the transpose of the flow system: off = -w (0 for self
+// receivers), diag = 1, solved bottom-up like the stream-power system.
+__global__ void k_fa_adj_prep(int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ wgt,
+                              double* __restrict__ off, double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int k = 0; k < NR; ++k) {
+    const int r = rcv[(int64_t)k * N + i];
+    const double w = wgt[(int64_t)k * N + i];
+    off[(int64_t)k * N + i] = (r != (int)i && r >= 0) ? -w : 0.0;
+  }
+  diag[i] = 1.0;
+}
+
+// Stream-power adjoint, step 1: the right-hand side of A^T mu = lam_x
+// (lam_x = 0 where the forward kept h: flat outlets and sinks, SPL.py:345-352)
+// and the transposed system's weights c_ik / diag_i, so that nu = diag * mu
+// solves nu_i = lam_x_i + sum over donors j of (c_jk / diag_j) nu_j, a
+// top-down sweep like flow accumulation.
+__global__ void k_spl_adj_prep(int64_t N, int NR, const double* __restrict__ off, const double* __restrict__ diag,
+                               const double* __restrict__ lam, const int8_t* __restrict__ outlet, int flat,
+                               const int8_t* __restrict__ lsink, double* __restrict__ wt, double* __restrict__ rhs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double d = diag[i];
+  for (int k = 0; k < NR; ++k) wt[(int64_t)k * N + i] = -off[(int64_t)k * N + i] / d;
+  const bool reset = (flat && outlet[i]) || lsink[i];
+  rhs[i] = reset ? 0.0 : lam[i];
+}
+
+// Step 2: mu = nu / diag, the local terms of each dependency (i, k) and the
+// erodibility gradient.  With c_ik = Kbr_i l(h_i - h_rk) w_ik / d_ik,
+// l(s) = s / (s + 0.01), Kbr_i = K kfac_i dt FAL_i^m (0 at sea), and the
+// forward residual (A x)_i - h_i = x_i + sum_k c_ik (x_i - x_rk) - h_i:
+//   e_ik = mu_i Kbr_i w_ik / d_ik l'(h_i - h_rk) (x_i - x_rk)
+//   dL/dlog10(kfac_i) = -ln(10) mu_i sum_k c_ik (x_i - x_rk)
+//   dL/dFAL_i        = -m / FAL_i mu_i sum_k c_ik (x_i - x_rk)
+__global__ void k_spl_adj_terms(int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ wgt,
+                                const double* __restrict__ dist, const double* __restrict__ fal,
+                                const double* __restrict__ kfac, double K, double m, double dt, double sl,
+                                const double* __restrict__ lfill, const double* __restrict__ h,
+                                const double* __restrict__ x, const double* __restrict__ off,
+                                const double* __restrict__ diag, double* __restrict__ mu, double* __restrict__ e,
+                                double* __restrict__ glogk, double* __restrict__ gfal) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double mi = mu[i] / diag[i];
+  mu[i] = mi;
+  const bool sea = lfill[i] <= sl;
+  double kbr = K * 1.0;
+  if (kfac) kbr = kbr * kfac[i];
+  kbr = kbr * (dt * pow(fal[i], m));
+  if (sea) kbr = 0.0;
+  double S = 0.0;
+  for (int k = 0; k < NR; ++k) {
+    const int r = rcv[(int64_t)k * N + i];
+    const double o = off[(int64_t)k * N + i];
+    const double dk = dist[(int64_t)k * N + i];
+    double ek = 0.0;
+    if (r != (int)i && r >= 0 && kbr != 0.0 && dk != 0.0) {
+      // l'(s) = 0.01 / (s + 0.01)^2, also at s = 0 where the forward's
+      // l(0) = 0 is taken without the division
+      const double dx = x[i] - x[r];
+      S = S + (-o) * dx;
+      const double dh = h[i] - h[r];
+      const double w = wgt[(int64_t)k * N + i];
+      const double t = dh + 1.0e-2;
+      ek = mi * (kbr * w / dk) * (1.0e-2 / (t * t)) * dx;
+    }
+    e[(int64_t)k * N + i] = ek;
+  }
+  if (glogk) glogk[i] = -2.302585092994046 * mi * S;
+  if (gfal) gfal[i] = fal[i] > 0.0 ? -m / fal[i] * mi * S : 0.0;
+}
+
+// Step 3: dL/dh_j = (lam_j where the forward kept h) + mu_j - sum_k e_jk +
+// sum over donors i of j (through their slot k) of e_ik, gathered through the
+// neighbour list in a fixed order (no floating-point atomics).
+__global__ void k_spl_adj_gather(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
+                                 const int32_t* __restrict__ deg, const int32_t* __restrict__ rcv,
+                                 const double* __restrict__ e, const double* __restrict__ mu,
+                                 const double* __restrict__ lam, const int8_t* __restrict__ outlet, int flat,
+                                 const int8_t* __restrict__ lsink, double* __restrict__ gh) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  double g = mu[j];
+  for (int k = 0; k < NR; ++k) g = g - e[(int64_t)k * N + j];
+  const int dg = deg[j];
+  for (int p = 0; p < W && p < dg; ++p) {
+    const int i = nbr[(int64_t)p * N + j];
+    if (i < 0) continue;
+    for (int k = 0; k < NR; ++k)
+      if (rcv[(int64_t)k * N + i] == (int)j) g = g + e[(int64_t)k * N + i];
+  }
+  const bool reset = (flat && outlet[j]) || lsink[j];
+  gh[j] = reset ? g + lam[j] : g;
+}
+
 int ensure_records(gospl_mesh* m, hipStream_t s) {
   if (m->wrec) return 0;
   GOSPL_CHECK(hipMalloc(&m->wrec, sizeof(double) * REC_W * (m->N + 64)));
@@ -1753,6 +1856,74 @@ int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* 
   if (rc) return rc;
   k_spl_update<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, x, m->outlet, m->flat, lsink, dt, h,
                                                           cumed, eb, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_flowacc_adjoint(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const double* wgt,
+                          const double* lam, double* y, gospl_stream_t stream) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_flowacc_adjoint: nrcv must be in 1..12");
+    return 1;
+  }
+  if (gospl_ensure_coef(m, nrcv)) return 1;
+  k_fa_adj_prep<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, nrcv, rcv, wgt, m->wcoef, m->wd[3]);
+  GOSPL_CHECK(hipGetLastError());
+  int64_t bad = 0;
+  if (dag_solve(m, 1, nrcv, rcv, m->wcoef, m->wd[3], lam, y, nullptr, &bad, stream)) return 1;
+  if (bad) {
+    set_error("gospl_flowacc_adjoint: the receivers hold %lld vertices on cycles", (long long)bad);
+    return 1;
+  }
+  return 0;
+}
+
+int gospl_spl_adjoint(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                      const double* disti, const double* fal, const double* kfac, double K, double mexp,
+                      double dt, double sealevel, const double* lfill, const int8_t* lsink, const double* h,
+                      const double* x, const double* lam, double* grad_h, double* grad_logk, double* grad_fal,
+                      gospl_stream_t stream) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_spl_adjoint: nrcv must be in 1..12");
+    return 1;
+  }
+  // coefficient rows: off (nrcv rows), transposed weights (nrcv rows), local
+  // terms (nrcv rows), mu (one row; wd[1] is the sweep's cycle scratch)
+  if (gospl_ensure_coef(m, 3 * nrcv + 1)) return 1;
+  const int64_t N = m->N;
+  double* off = m->wcoef;
+  double* wt = m->wcoef + (int64_t)nrcv * (N + 64);
+  double* e = m->wcoef + (int64_t)2 * nrcv * (N + 64);
+  double* mu = m->wcoef + (int64_t)3 * nrcv * (N + 64);
+  double* diag = m->wd[3];
+  double* rhs = m->wd[2];
+  const int g = nblocks(N);
+  k_spl_coef<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, wgti, disti, fal, kfac, K, mexp, dt, sealevel, lfill, h,
+                                           off, diag);
+  GOSPL_CHECK(hipGetLastError());
+  int64_t bad = 0;
+  if (!x) {
+    // the forward solution A x = h, solved again (the forward kept h at
+    // flat outlets and sinks, whose x still enters their donors' rows)
+    double* xs = m->wd[0];
+    if (dag_solve(m, 1, nrcv, rcvi, off, diag, h, xs, nullptr, &bad, stream)) return 1;
+    if (bad) {
+      set_error("gospl_spl_adjoint: the receivers hold %lld vertices on cycles", (long long)bad);
+      return 1;
+    }
+    x = xs;
+  }
+  k_spl_adj_prep<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, off, diag, lam, m->outlet, m->flat, lsink, wt, rhs);
+  GOSPL_CHECK(hipGetLastError());
+  if (dag_solve(m, 0, nrcv, rcvi, wt, nullptr, rhs, mu, nullptr, &bad, stream)) return 1;
+  if (bad) {
+    set_error("gospl_spl_adjoint: the receivers hold %lld vertices on cycles", (long long)bad);
+    return 1;
+  }
+  k_spl_adj_terms<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, wgti, disti, fal, kfac, K, mexp, dt, sealevel, lfill,
+                                                h, x, off, diag, mu, e, grad_logk, grad_fal);
+  k_spl_adj_gather<<<g, GOSPL_BLOCK, 0, stream>>>(N, m->W, nrcv, m->nbr, m->deg, rcvi, e, mu, lam, m->outlet,
+                                                 m->flat, lsink, grad_h);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

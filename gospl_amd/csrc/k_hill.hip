@@ -15,17 +15,19 @@
 
 namespace gospl {
 
-// One operator row; the sea flags decide each vertex's diffusivity.
+// One operator row; the sea flags (is_sea(v)) decide each vertex's
+// diffusivity.  The FP32 copies are written when given.
+template <class Sea>
 __device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int32_t* __restrict__ nbr,
                                          const int32_t* __restrict__ deg, const double* __restrict__ area,
                                          const double* __restrict__ elgt, const double* __restrict__ vdist,
-                                         const int8_t* __restrict__ outlet, int flat,
-                                         const double* __restrict__ h, double sl, double cda, double cdm,
-                                         double dt, double* __restrict__ coef, double* __restrict__ diag,
-                                         float* __restrict__ coef32, float* __restrict__ diag32) {
+                                         const int8_t* __restrict__ outlet, int flat, const Sea& is_sea,
+                                         double cda, double cdm, double dt, double* __restrict__ coef,
+                                         double* __restrict__ diag, float* __restrict__ coef32,
+                                         float* __restrict__ diag32) {
   const double a = area[i];
   const bool ident = flat && outlet[i];
-  const double ck = (h[i] <= sl ? cdm : cda) * dt;
+  const double ck = (is_sea(i) ? cdm : cda) * dt;
   double s1 = 0.0;
   const int dg = deg[i];
   for (int p = 0; p < W; ++p) {
@@ -34,7 +36,7 @@ __device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int3
       const double vd = vdist[(int64_t)p * N + i];
       if (vd > 0.0) {
         const int n = nbr[(int64_t)p * N + i];
-        const double cn = (h[n] <= sl ? cdm : cda) * dt;
+        const double cn = (is_sea(n) ? cdm : cda) * dt;
         const double c = 0.5 * (ck + cn) / a;
         const double v = c * vd / elgt[(int64_t)p * N + i];
         s1 = s1 + v;
@@ -42,12 +44,22 @@ __device__ __forceinline__ void hill_row(int64_t N, int W, int64_t i, const int3
       }
     }
     coef[(int64_t)p * N + i] = c_out;
-    coef32[(int64_t)p * N + i] = (float)c_out;
+    if (coef32) coef32[(int64_t)p * N + i] = (float)c_out;
   }
   const double d = ident ? 1.0 : (a > 0.0 ? 1.0 + s1 : 0.0);
   diag[i] = d;
-  diag32[i] = (float)d;
+  if (diag32) diag32[i] = (float)d;
 }
+
+struct SeaByLevel {
+  const double* h;
+  double sl;
+  __device__ __forceinline__ bool operator()(int64_t v) const { return h[v] <= sl; }
+};
+struct SeaByFlag {
+  const int8_t* f;
+  __device__ __forceinline__ bool operator()(int64_t v) const { return f[v] != 0; }
+};
 
 // Every row, and the sea flags they were built from.
 __global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
@@ -59,7 +71,8 @@ __global__ void k_hill_coef(int64_t N, int W, const int32_t* __restrict__ nbr,
                             int8_t* __restrict__ sea) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag, coef32, diag32);
+  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, SeaByLevel{h, sl}, cda, cdm, dt, coef, diag, coef32,
+           diag32);
   sea[i] = h[i] <= sl;
 }
 
@@ -92,7 +105,8 @@ __global__ void k_hill_coef_changed(int64_t N, int W, const int32_t* __restrict_
     if (n >= 0 && chg[n]) need = true;
   }
   if (need)
-    hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, h, sl, cda, cdm, dt, coef, diag, coef32, diag32);
+    hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, SeaByLevel{h, sl}, cda, cdm, dt, coef, diag, coef32,
+             diag32);
 }
 
 __global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
@@ -170,6 +184,42 @@ __global__ void k_hill_rho(int64_t N, const double* __restrict__ diag, unsigned 
   }
   for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
   if ((threadIdx.x & 63) == 0 && r > 0.0) atomicMax(key, dkey(r));
+}
+
+// Rows from given sea flags (the frozen operator of the adjoint and its
+// forward replay), in buffers of their own.
+__global__ void k_hill_rows_flags(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                                  const double* __restrict__ area, const double* __restrict__ elgt,
+                                  const double* __restrict__ vdist, const int8_t* __restrict__ outlet, int flat,
+                                  const int8_t* __restrict__ sea, double cda, double cdm, double dt,
+                                  double* __restrict__ coef, double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  hill_row(N, W, i, nbr, deg, area, elgt, vdist, outlet, flat, SeaByFlag{sea}, cda, cdm, dt, coef, diag,
+           (float*)nullptr, (float*)nullptr);
+}
+
+// One Jacobi sweep of the transposed operator M^T y = rhs: the coefficient
+// of y_j in row i of M^T is row j's coefficient at i, found through the
+// reverse neighbour positions (rev[q * N + i] = position of i in the list of
+// its neighbour q).  Rows with a zero diagonal are identity rows of M.
+__global__ void k_hill_sweepT(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                              const int8_t* __restrict__ rev, const double* __restrict__ coef,
+                              const double* __restrict__ diag, const double* __restrict__ rhs,
+                              const double* __restrict__ xin, double* __restrict__ xout) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = rhs[i];
+  const int dg = deg[i];
+  for (int q = 0; q < W && q < dg; ++q) {
+    const int j = nbr[(int64_t)q * N + i];
+    const int p = rev[(int64_t)q * N + i];
+    if (j < 0 || p < 0) continue;
+    const double c = coef[(int64_t)p * N + j];
+    if (c != 0.0) s = s - c * xin[j];
+  }
+  const double d = diag[i];
+  xout[i] = d != 0.0 ? s / d : s;
 }
 
 // Refined solve (FP32 corrections of an FP64 residual).  Control words in
@@ -495,5 +545,67 @@ extern "C" int gospl_hillslope_refine(gospl_mesh_t* m, double cda, double cdm, d
     if (passes) *passes = (int32_t)hc->pass;
     if (relres) *relres = hc->relres;
   }
+  return 0;
+}
+
+// The frozen operator's rows (sea flags given), in the adjoint's buffers.
+static int hill_frozen_rows(gospl_mesh* m, double cda, double cdm, double dt, const int8_t* sea, hipStream_t s) {
+  const int64_t N = m->N;
+  if (!m->adj_hcoef) {
+    GOSPL_CHECK(hipMalloc(&m->adj_hcoef, sizeof(double) * (int64_t)m->W * (N + 64)));
+    GOSPL_CHECK(hipMalloc(&m->adj_hdiag, sizeof(double) * (N + 64)));
+  }
+  k_hill_rows_flags<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist,
+                                                       m->outlet, m->flat, sea, cda, cdm, dt, m->adj_hcoef,
+                                                       m->adj_hdiag);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gospl_hillslope_frozen(gospl_mesh_t* m, double cda, double cdm, double dt, const int8_t* sea,
+                                      int32_t sweeps, const double* h_in, double* h_out, gospl_stream_t s) {
+  if (sweeps < 1) {
+    set_error("gospl_hillslope_frozen: sweeps must be >= 1 (a count from the contraction bound)");
+    return 1;
+  }
+  if (hill_frozen_rows(m, cda, cdm, dt, sea, s)) return 1;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  double* xa = m->wd[1];
+  double* xb = m->wd[2];
+  const double* xin = h_in;
+  double* xout = xa;
+  for (int k = 0; k < sweeps; ++k) {
+    k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->adj_hcoef, m->adj_hdiag, h_in, xin, xout);
+    xin = xout;
+    xout = (xout == xa) ? xb : xa;
+  }
+  GOSPL_CHECK(hipMemcpyAsync(h_out, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gospl_hillslope_adjoint(gospl_mesh_t* m, double cda, double cdm, double dt, const int8_t* sea,
+                                       int32_t sweeps, const double* lam, double* out, gospl_stream_t s) {
+  if (sweeps < 1) {
+    set_error("gospl_hillslope_adjoint: sweeps must be >= 1 (a count from the contraction bound)");
+    return 1;
+  }
+  if (ensure_records(m, s)) return 1;  // the reverse neighbour positions
+  if (hill_frozen_rows(m, cda, cdm, dt, sea, s)) return 1;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  double* xa = m->wd[1];
+  double* xb = m->wd[2];
+  const double* xin = lam;
+  double* xout = xa;
+  for (int k = 0; k < sweeps; ++k) {
+    k_hill_sweepT<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->rev, m->adj_hcoef, m->adj_hdiag, lam, xin,
+                                           xout);
+    xin = xout;
+    xout = (xout == xa) ? xb : xa;
+  }
+  GOSPL_CHECK(hipMemcpyAsync(out, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  GOSPL_CHECK(hipGetLastError());
   return 0;
 }

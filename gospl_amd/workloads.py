@@ -51,6 +51,25 @@ def sphere_case(lib, level: int = 9, seed: int = 0, lmax: int = 64, slope: float
     return m, m.to_device_order(z), params
 
 
+def adjoint_case(lib, level: int = 8, seed: int = 0, lmax: int = 64, device=None, order="morton"):
+    """The inversion workload: the sphere of ``sphere_case``; a target
+    surface from a run with per-vertex log10 K = -5.3 plus a seeded smooth
+    perturbation (SH degree <= 16, scaled to max |dlogK| = 0.3) and uplift
+    0.5 mm/yr; the inverse run starts from uniform log10 K = -5.3.  dt 5000
+    yr, 50 steps.  Returns (mesh, h0, params, logk_true, logk0, uplift),
+    per-vertex arrays in device order; params exclude K and upsub."""
+    m, h0, p = sphere_case(lib, level=level, seed=seed, lmax=lmax, device=device, order=order)
+    coords_in = m.to_input_order(m.coords)
+    pert = fields.sh_field(coords_in, 16, 2.0, seed=seed + 17, device=device)
+    pert = 0.3 * pert / np.abs(pert).max()
+    logk_true = m.to_device_order(-5.3 + pert)
+    logk0 = np.full(m.N, -5.3)
+    uplift = np.full(m.N, 5.0e-4)
+    p = {k: v for k, v in p.items() if k not in ("K", "upsub", "d")}
+    p["nsteps"] = 50
+    return m, h0, p, logk_true, logk0, uplift
+
+
 def oracle_geometry(m: "_mesh.Mesh"):
     """Input-order geometry dict for ``oracle.OracleModel``."""
     ex = m.extras

@@ -239,6 +239,45 @@ int gospl_hillslope_refine(gospl_mesh_t* mesh, double cda, double cdm, double dt
                            double* eblocal, int32_t* passes, double* relres, gospl_stream_t stream);
 
 /*
+ * Adjoints (reverse-mode derivatives) of one step's solves with the step's
+ * receivers, weights, distances, discharge, fill and sea/sink flags held at
+ * their forward values.  No counterpart in goSPL, which has no derivatives;
+ * they are the transposes of the systems goSPL assembles in
+ * FAMesh.matrixFlow (flowplex.py:429-478), SPL._eroMats (SPL.py:45-114) and
+ * hillSLP._hillSlope (hillslope.py:235-293).  All arrays are on the device.
+ *  - gospl_flowacc_adjoint: y solves (I - W) y = lam, the transpose of the
+ *    flow system (I - W^T) q = b of gospl_flowacc (self receivers dropped as
+ *    there): one sweep from the outlets up.  Errors on a receiver cycle.
+ *  - gospl_spl_adjoint: reverse of gospl_spl.  With A(h) x = h the forward
+ *    system (h the pre-erosion elevation) and h' = x except where the forward
+ *    kept h (flat-mesh outlets, lsink cells), given lam = dL/dh' it writes
+ *    grad_h = dL/dh (through x and through A's dependence on h_i - h_r),
+ *    grad_logk (may be NULL) = dL/dlog10(kfac) per vertex and grad_fal (may
+ *    be NULL) = dL/dfal.  x (may be NULL: then solved again) is the forward
+ *    solution.  One sweep from the sources down for A^T mu = lam (plus one
+ *    the other way when x is NULL).
+ *  - gospl_hillslope_frozen: h_out = the `sweeps` Jacobi sweeps of
+ *    gospl_hillslope on h_in with the sea flags given per vertex (sea[i] != 0:
+ *    diffusivity cdm) instead of taken from h_in; the handle's kept operator
+ *    is not touched.
+ *  - gospl_hillslope_adjoint: out = M^-T lam by `sweeps` Jacobi sweeps of the
+ *    transposed operator (same spectrum, so the forward's count applies),
+ *    with M the operator of gospl_hillslope_frozen for the same sea flags.
+ * sweeps < 1 is an error for both hillslope entries.
+ */
+int gospl_flowacc_adjoint(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const double* wgt,
+                          const double* lam, double* y, gospl_stream_t stream);
+int gospl_spl_adjoint(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                      const double* disti, const double* fal, const double* kfac, double K, double m,
+                      double dt, double sealevel, const double* lfill, const int8_t* lsink,
+                      const double* h, const double* x, const double* lam, double* grad_h,
+                      double* grad_logk, double* grad_fal, gospl_stream_t stream);
+int gospl_hillslope_frozen(gospl_mesh_t* mesh, double cda, double cdm, double dt, const int8_t* sea,
+                           int32_t sweeps, const double* h_in, double* h_out, gospl_stream_t stream);
+int gospl_hillslope_adjoint(gospl_mesh_t* mesh, double cda, double cdm, double dt, const int8_t* sea,
+                            int32_t sweeps, const double* lam, double* out, gospl_stream_t stream);
+
+/*
  * Hillslope operator coefficients: replaces `sethillslopecoeff`
  * (functions.pyf:10-14, functions.F90:901-938).  kd (N) = diffusivity * dt
  * per vertex; coeff (13, N): row 0 the diagonal 1 + sum v, row 1 + p the

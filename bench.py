@@ -23,6 +23,11 @@ FP64 runs from the surface nudged by +-1 ulp (three seeds), and each refined
 trajectory's relative elevation error against the FP64 one is reported next
 to the FP64 path's own divergence under that nudge.
 
+A gradient leg (``--adjoint-level``, 8 by default: 655,362 vertices) times
+the inversion workload's forward run and its gradient in per-vertex log10 K
+and uplift (50 steps, checkpoints every 10; ``gospl_amd.adjoint``), with a
+finite-difference spot check of the gradient.
+
 Prints one JSON line (the driver's contract) with the roofline of the
 dominant kernel of the reported leg and the CPU oracle timed on a bounded
 sample beside it.
@@ -278,6 +283,68 @@ def accuracy(m, h0, p, dev, nsteps, legs):
     return out
 
 
+def adjoint_leg(level, seed, dev, nfd):
+    """The inversion workload (sphere of the given level, 50 steps of 5000 yr,
+    checkpoints every 10): a target from per-vertex log10 K = -5.3 + smooth
+    perturbation and uplift 0.5 mm/yr, then at uniform log10 K = -5.3 the
+    loss sum (h - target)^2 and its gradient in log10 K and uplift.  Times
+    the forward run and the gradient (replay + reverse sweeps); checks nfd
+    random vertices per input against central differences of the frozen map
+    (h = 1e-4 in log10 K and in mm/yr)."""
+    import torch
+
+    from gospl_amd import _lib, workloads
+    from gospl_amd.adjoint import Evolution
+
+    t0 = time.time()
+    m, h0, p, lk_true, lk0, up = workloads.adjoint_case(_lib.load(), level=level, seed=seed, device=dev)
+    nsteps = p.pop("nsteps")
+    log("[bench] adjoint workload level %d: N=%d built in %.1fs" % (level, m.N, time.time() - t0))
+    evo = Evolution(m, h0, nsteps=nsteps, checkpoint_every=10, **p)
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float64)).to(dev)
+    lk_true, lk0, up = t(lk_true), t(lk0), t(up)
+    target, _ = evo.run(lk_true, up)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    h, st = evo.run(lk0, up)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    loss = float(((h - target) ** 2).sum())
+    t0 = time.perf_counter()
+    g_logk, g_up = evo.gradient(st, 2.0 * (h - target))
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    log("[bench] adjoint: forward %.3f s, gradient %.3f s (%d steps)" % (t_fwd, t_grad, nsteps))
+    out = {"workload": "icosahedral sphere level %d, %d vertices, %d steps of %g yr, checkpoints every 10, "
+                       "loss sum (h - target)^2" % (level, m.N, nsteps, p["dt"]),
+           "forward_s": t_fwd, "gradient_s": t_grad, "forward_steps_per_s": nsteps / t_fwd,
+           "gradient_over_forward": t_grad / t_fwd, "loss": loss,
+           "grad_logk_maxabs": float(g_logk.abs().max()), "grad_uplift_maxabs": float(g_up.abs().max())}
+    if nfd > 0:
+        _, str_ = evo.run(lk0, up, record=True)
+        rng = np.random.default_rng(seed + 5)
+        worst = {}
+        for what, base, g, step in (("logk", lk0, g_logk, 1.0e-4), ("uplift", up, g_up, 1.0e-7)):
+            ga = g.cpu().numpy()
+            cand = np.nonzero(np.abs(ga) > 1e-3 * np.abs(ga).max())[0]
+            errs = []
+            for i in rng.choice(cand, min(nfd, len(cand)), replace=False):
+                xp, xm = base.clone(), base.clone()
+                xp[i] += step
+                xm[i] -= step
+                a = evo.frozen_run(str_, xp, up) if what == "logk" else evo.frozen_run(str_, lk0, xp)
+                b = evo.frozen_run(str_, xm, up) if what == "logk" else evo.frozen_run(str_, lk0, xm)
+                fd = float(torch.sum((a - b) * (a + b - 2.0 * target))) / (2.0 * step)
+                errs.append(abs(fd - ga[i]) / abs(fd))
+            worst[what] = max(errs)
+        del str_
+        out["fd_check"] = {"vertices_per_input": nfd, "h": "1e-4 (log10 K), 1e-4 mm/yr (uplift)",
+                           "worst_rel_err_logk": worst["logk"], "worst_rel_err_uplift": worst["uplift"]}
+        log("[bench] adjoint finite-difference check: %s" % json.dumps(out["fd_check"]))
+    evo.dm.close()
+    return out
+
+
 def rocpd_summary(db, first, count, mode, N):
     """Markdown summary of a rocprofv3 database (--kernel-trace): kernels by
     total time, and the exact sweep dispatches first .. first + count - 1
@@ -321,6 +388,10 @@ def main():
                     help="steps of the accuracy comparison (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=1, help="timed CPU-baseline steps per thread count")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-level", type=int, default=8,
+                    help="sphere level of the gradient (inversion) leg; 0 skips it")
+    ap.add_argument("--adjoint-fd", type=int, default=4,
+                    help="vertices per input of the leg's finite-difference check")
     ap.add_argument("--surface", default=None, metavar="NPY",
                     help="initial surface (input order) saved by --save-surface instead of the SH field "
                          "(keeps the SH construction's kernels out of a counter run)")
@@ -404,6 +475,10 @@ def main():
             cpu["threaded"] = {"threads": nthr, "value": res[nthr]["steps_per_s"],
                                "solver_iterations": res[nthr]["solver_iterations"]}
 
+    adj = None
+    if args.adjoint_level > 0:
+        adj = adjoint_leg(args.adjoint_level, args.seed, dev, args.adjoint_fd)
+
     leg_cfg = {}
     for name, lg in legs.items():
         leg_cfg[name] = dict(steps_per_s=lg["sps"], ms_per_step=1e3 / lg["sps"],
@@ -424,6 +499,8 @@ def main():
                 cfg[n + "_over_fp64"] = legs[n]["sps"] / legs["fp64"]["sps"]
     if acc is not None:
         cfg["accuracy"] = acc
+    if adj is not None:
+        cfg["adjoint"] = adj
     dtype = {"refined": "f64 (DAG sweeps, state, residuals) + f32 (hillslope corrections)",
              "refined_dag": "f64 (state, residuals) + i64 fixed-point (DAG corrections) + f32 (hillslope)",
              "fp64": "f64"}[main_leg]

@@ -54,6 +54,11 @@ SIGNATURES = {
     "gospl_cascade_seed": (I32, [P, P, I64, I64, P, P, P, F64, P, P]),
     "gospl_cascade_add": (I32, [P, P, P, P, F64, P, P, P]),
     "gospl_flow_finalize": (I32, [P, P, P, P, I32, P, P, P]),
+    "gospl_mesh_set_tiebreak": (I32, [P, P]),
+    "gospl_fill_elevation_seg": (I32, [P, P, I32, F64, F64, P, I32, P, P]),
+    "gospl_flow_finalize_seg": (I32, [P, P, P, P, P, I32, P, P, P, P]),
+    "gospl_spl_batch": (I32, [P, I32, P, P, P, P, P, F64, P, F64, F64, P, P, P, P, P, P, ctypes.POINTER(I32), P]),
+    "gospl_hillslope_refine_seg": (I32, [P, F64, F64, F64, F64, P, I32, P, P, F64, I32, P, P, P, P, P]),
     "gospl_flow_finalize_noalloc": (I32, [I64, P, P, P, I32, P, P, P]),
     "gospl_sweep_timing": (I32, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64),
                                  ctypes.POINTER(F64), ctypes.POINTER(I64)]),

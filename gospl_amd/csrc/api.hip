@@ -130,6 +130,69 @@ __global__ void k_finalize_b(int64_t N, const double* __restrict__ hl, const dou
   fillfal[i] = g < 0.0 ? 0.0 : g;
 }
 
+// gospl_flow_finalize per segment (ensembles): each segment's own pit flag
+// and discharge maximum; padding vertices (segment < 0) are only clamped.
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_finalize_seg_a(int64_t N, const double* __restrict__ hl,
+                                                               const double* __restrict__ water,
+                                                               const int32_t* __restrict__ pit,
+                                                               const int32_t* __restrict__ seg,
+                                                               const int8_t* __restrict__ has_pits,
+                                                               double* __restrict__ fal,
+                                                               unsigned long long* __restrict__ keys) {
+  __shared__ double s_max[GOSPL_BLOCK / 64];
+  __shared__ int s_lo, s_hi;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int sg = i < N ? seg[i] : -1;
+  double v = -INFINITY;
+  if (sg >= 0) {
+    double f = fal[i];
+    if (has_pits[sg] && hl[i] <= water[i] && pit[i] > -1) f = 0.0;
+    fal[i] = f;
+    v = f;
+  }
+  if (threadIdx.x == 0) {
+    s_lo = 0x7fffffff;
+    s_hi = -1;
+  }
+  __syncthreads();
+  if (sg >= 0) {
+    atomicMin(&s_lo, sg);
+    atomicMax(&s_hi, sg);
+  }
+  __syncthreads();
+  const int lo = s_lo, hi = s_hi;
+  if (hi < 0) return;
+  if (lo != hi) {
+    if (sg >= 0 && v > -INFINITY) atomicMax(keys + sg, dkey(v));
+    return;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_down(v, o, 64);
+    v = v > w ? v : w;
+  }
+  if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_max[0];
+    for (int k = 1; k < GOSPL_BLOCK / 64; ++k) b = b > s_max[k] ? b : s_max[k];
+    if (b > -INFINITY) atomicMax(keys + lo, dkey(b));
+  }
+}
+
+__global__ void k_finalize_seg_b(int64_t N, const double* __restrict__ hl, const double* __restrict__ water,
+                                 const int32_t* __restrict__ pit, const int32_t* __restrict__ seg,
+                                 const int8_t* __restrict__ has_pits, const unsigned long long* __restrict__ keys,
+                                 double* __restrict__ fal, double* __restrict__ fillfal) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int sg = seg[i];
+  const double f = fal[i];
+  double g = f;
+  if (sg >= 0 && has_pits[sg] && hl[i] <= water[i] && pit[i] > -1) g = dkey_inv(keys[sg]) * 0.1;
+  fal[i] = f < 0.0 ? 0.0 : f;
+  fillfal[i] = g < 0.0 ? 0.0 : g;
+}
+
 // gospl_flow_finalize without a handle or any allocation: the discharge
 // maximum goes through `scratch`, one partial maximum (order-preserving key)
 // per workgroup of k_fin_partial, each workgroup striding over the mesh; every
@@ -311,6 +374,7 @@ int gospl_mesh_create(int64_t N, int32_t W, int32_t flat, const int32_t* nbr, co
   m->vdist = vdist;
   m->area = area;
   m->gid = gid;
+  m->tgid = gid;
   m->outlet = outlet;
   if (mesh_alloc(m)) {
     gospl_mesh_destroy(m);  // releases what was allocated before the failure
@@ -381,6 +445,10 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->hcoef32);
   dfree(m->adj_hcoef);
   dfree(m->adj_hdiag);
+  dfree(m->seg_keys);
+  dfree(m->seg_ctl);
+  dfree(m->seg_aux);
+  free(m->seg_haux);
   dfree(m->hdiag32);
   dfree(m->hsea);
   delete m;
@@ -449,6 +517,32 @@ int gospl_flow_finalize(gospl_mesh_t* m, const double* hl, const double* water,
   k_finalize_a<<<FINALIZE_BLOCKS, GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, fal, key);
   k_finalize_b<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, has_pits, key, fal,
                                                            fillfal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_mesh_set_tiebreak(gospl_mesh_t* m, const int32_t* tgid) {
+  if (!m) {
+    set_error("gospl_mesh_set_tiebreak: null handle");
+    return 1;
+  }
+  m->tgid = tgid ? tgid : m->gid;
+  return 0;
+}
+
+int gospl_flow_finalize_seg(gospl_mesh_t* m, const double* hl, const double* water, const int32_t* pitids,
+                            const int32_t* seg, int32_t nseg, const int8_t* has_pits, double* fal, double* fillfal,
+                            gospl_stream_t stream) {
+  if (nseg < 1 || !seg || !has_pits) {
+    set_error("gospl_flow_finalize_seg: need segment and per-segment pit flag arrays, nseg >= 1");
+    return 1;
+  }
+  if (ensure_seg(m, nseg)) return 1;
+  GOSPL_CHECK(hipMemsetAsync(m->seg_keys, 0, sizeof(unsigned long long) * nseg, stream));
+  k_finalize_seg_a<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, seg, has_pits, fal,
+                                                              m->seg_keys);
+  k_finalize_seg_b<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, hl, water, pitids, seg, has_pits, m->seg_keys,
+                                                              fal, fillfal);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

@@ -20,6 +20,7 @@ struct gospl_mesh {
   const double* vdist;        // (W, N)
   const double* area;         // (N)
   const int32_t* gid;         // (N) input-mesh ids
+  const int32_t* tgid;        // (N) ids of the receivers' slope tie-break (gid unless set)
   const int8_t* outlet;       // (N) draining border flags (flat meshes)
   int device;
   int num_cu;
@@ -68,6 +69,13 @@ struct gospl_mesh {
   float* hdiag32;
   double* adj_hcoef;          // hillslope rows from given sea flags (adjoint, frozen replay)
   double* adj_hdiag;
+  // segmented (ensemble) calls: per-segment keys and control words
+  unsigned long long* seg_keys;
+  double* seg_ctl;
+  int32_t* seg_aux;
+  int seg_cap;
+  int32_t* seg_haux;          // host copy of the last seg_aux upload
+  int seg_haux_n;
   int8_t* hsea;               // the sea flags the rows were built from
   double hpar[3];             // cda, cdm, dt they were built with
   int hvalid;
@@ -103,6 +111,7 @@ int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double
 // positions; donor rows / overflow entries of capacity `cap`
 int ensure_records(gospl_mesh* m, hipStream_t s);
 int ensure_donors(gospl_mesh* m, int64_t cap);
+int ensure_seg(gospl_mesh* m, int nseg);  // per-segment keys / control words (k_fill.hip)
 
 // Refined solve on the receiver DAG (k_refine.hip): x (in: initial guess,
 // out: solution) of the flow system (mode 0: (I - W^T) x = rhs, coef = W) or

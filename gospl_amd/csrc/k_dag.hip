@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(CYCLE_THREADS) void k_flow_cycles(DagArgs a, const 
 __global__ void k_spl_coef(int64_t N, int NR, const int32_t* __restrict__ rcv,
                            const double* __restrict__ wgt, const double* __restrict__ dist,
                            const double* __restrict__ fal, const double* __restrict__ kfac,
-                           double K, double m, double dt, double sl,
+                           double K, double m, const double* __restrict__ mvec, double dt, double sl,
                            const double* __restrict__ lfill, const double* __restrict__ h,
                            double* __restrict__ off, double* __restrict__ diag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1241,7 +1241,7 @@ __global__ void k_spl_coef(int64_t N, int NR, const int32_t* __restrict__ rcv,
   const bool sea = lfill[i] <= sl;
   double kbr = K * 1.0;
   if (kfac) kbr = kbr * kfac[i];
-  kbr = kbr * (dt * pow(fal[i], m));
+  kbr = kbr * (dt * pow(fal[i], mvec ? mvec[i] : m));
   if (sea) kbr = 0.0;
   const double hi = h[i];
   double d = 1.0;
@@ -1808,7 +1808,7 @@ int gospl_spl_refine(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const d
   if (passes) *passes = 0;
   if (relres) *relres = 0.0;
   k_spl_coef<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, nrcv, rcvi, wgti, disti, fal, kfac, K,
-                                                        mexp, dt, sealevel, lfill, h, offc, diag);
+                                                        mexp, nullptr, dt, sealevel, lfill, h, offc, diag);
   GOSPL_CHECK(hipGetLastError());
   // the iterate starts at the old surface
   GOSPL_CHECK(hipMemcpyAsync(x, h, sizeof(double) * m->N, hipMemcpyDeviceToDevice, stream));
@@ -1836,11 +1836,11 @@ int gospl_spl_refine(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const d
   return 0;
 }
 
-int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
-              const double* disti, const double* fal, const double* kfac, double K, double mexp,
-              double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
-              double* cumed, double* eb, double* eblocal, int32_t* nlevels,
-              gospl_stream_t stream) {
+static int spl_step(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                    const double* disti, const double* fal, const double* kfac, double K, double mexp,
+                    const double* mvec, double dt, double sealevel, const double* lfill, const int8_t* lsink,
+                    double* h, double* cumed, double* eb, double* eblocal, int32_t* nlevels,
+                    gospl_stream_t stream) {
   if (nrcv < 1 || nrcv > GOSPL_MAXW) {
     set_error("gospl_spl: nrcv must be in 1..12");
     return 1;
@@ -1850,7 +1850,7 @@ int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* 
   double* diag = m->wd[3];
   double* x = m->wd[2];
   k_spl_coef<<<nblocks(m->N), GOSPL_BLOCK, 0, stream>>>(m->N, nrcv, rcvi, wgti, disti, fal, kfac, K,
-                                                        mexp, dt, sealevel, lfill, h, offc, diag);
+                                                        mexp, mvec, dt, sealevel, lfill, h, offc, diag);
   GOSPL_CHECK(hipGetLastError());
   int rc = dag_solve(m, 1, nrcv, rcvi, offc, diag, h, x, nlevels, nullptr, stream);
   if (rc) return rc;
@@ -1858,6 +1858,27 @@ int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* 
                                                           cumed, eb, eblocal);
   GOSPL_CHECK(hipGetLastError());
   return 0;
+}
+
+int gospl_spl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+              const double* disti, const double* fal, const double* kfac, double K, double mexp,
+              double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+              double* cumed, double* eb, double* eblocal, int32_t* nlevels,
+              gospl_stream_t stream) {
+  return spl_step(m, nrcv, rcvi, wgti, disti, fal, kfac, K, mexp, nullptr, dt, sealevel, lfill, lsink, h, cumed,
+                  eb, eblocal, nlevels, stream);
+}
+
+int gospl_spl_batch(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                    const double* disti, const double* fal, const double* kfac, double K, const double* mvec,
+                    double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+                    double* cumed, double* eb, double* eblocal, int32_t* nlevels, gospl_stream_t stream) {
+  if (!mvec) {
+    set_error("gospl_spl_batch: mvec (the per-vertex exponent) is required");
+    return 1;
+  }
+  return spl_step(m, nrcv, rcvi, wgti, disti, fal, kfac, K, 0.0, mvec, dt, sealevel, lfill, lsink, h, cumed, eb,
+                  eblocal, nlevels, stream);
 }
 
 int gospl_flowacc_adjoint(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const double* wgt,
@@ -1898,8 +1919,8 @@ int gospl_spl_adjoint(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const 
   double* diag = m->wd[3];
   double* rhs = m->wd[2];
   const int g = nblocks(N);
-  k_spl_coef<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, wgti, disti, fal, kfac, K, mexp, dt, sealevel, lfill, h,
-                                           off, diag);
+  k_spl_coef<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, wgti, disti, fal, kfac, K, mexp, nullptr, dt, sealevel,
+                                           lfill, h, off, diag);
   GOSPL_CHECK(hipGetLastError());
   int64_t bad = 0;
   if (!x) {

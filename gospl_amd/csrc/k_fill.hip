@@ -44,13 +44,19 @@ constexpr unsigned long long KEY_NEG_INF = 0x000FFFFFFFFFFFFFull;  // dkey(-inf)
 // minh = min(h) (+ 1e-3 when lv->add_eps), level = max(minh, lv->floor) with
 // Python's max (the first argument unless the second is greater).
 struct FillLevel {
-  const unsigned long long* minkey;  // dkey of min(h)
+  const unsigned long long* minkey;  // dkey of min(h), one per segment
   int add_eps;
   double floor;
+  const int32_t* seg;  // segment of each vertex (NULL: one segment); < 0: padding at `floor`
 };
-__device__ __forceinline__ double fill_level(double level, const FillLevel& lv) {
+__device__ __forceinline__ double fill_level(double level, const FillLevel& lv, int64_t i) {
   if (!lv.minkey) return level;
-  double mh = dkey_inv(*lv.minkey);
+  int sg = 0;
+  if (lv.seg) {
+    sg = lv.seg[i];
+    if (sg < 0) return lv.floor;
+  }
+  double mh = dkey_inv(lv.minkey[sg]);
   if (lv.add_eps) mh += 1.0e-3;
   return lv.floor > mh ? lv.floor : mh;
 }
@@ -78,6 +84,45 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_min_key(int64_t N, const double
   }
 }
 
+// dkey of min(h) per segment: one atomic per workgroup whose vertices share
+// a segment (the ensemble layout aligns segments to workgroups), one per
+// vertex otherwise; padding vertices (segment < 0) take no part.
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_min_key_seg(int64_t N, const double* __restrict__ h,
+                                                            const int32_t* __restrict__ seg,
+                                                            unsigned long long* __restrict__ keys) {
+  __shared__ unsigned long long sh[GOSPL_BLOCK / 64];
+  __shared__ int s_lo, s_hi;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int sg = i < N ? seg[i] : -1;
+  unsigned long long k = (sg >= 0) ? dkey(h[i]) : ~0ull;
+  if (threadIdx.x == 0) {
+    s_lo = 0x7fffffff;
+    s_hi = -1;
+  }
+  __syncthreads();
+  if (sg >= 0) {
+    atomicMin(&s_lo, sg);
+    atomicMax(&s_hi, sg);
+  }
+  __syncthreads();
+  const int lo = s_lo, hi = s_hi;
+  if (hi < 0) return;
+  if (lo != hi) {  // mixed workgroup
+    if (sg >= 0) atomicMin(keys + sg, k);
+    return;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long c = __shfl_xor(k, o);
+    k = c < k ? c : k;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) k = sh[w] < k ? sh[w] : k;
+    atomicMin(keys + lo, k);
+  }
+}
+
 // state: 1 seed, 0 included, -1 excluded (edge_tile, functions.F90:2036-2080)
 __global__ void k_fill_init(int64_t N, int W, const int32_t* __restrict__ nbr,
                             const int32_t* __restrict__ deg, const int8_t* __restrict__ outlet,
@@ -85,7 +130,7 @@ __global__ void k_fill_init(int64_t N, int W, const int32_t* __restrict__ nbr,
                             double* __restrict__ dem, int32_t* __restrict__ state) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double level = fill_level(level_in, lv);
+  const double level = fill_level(level_in, lv, i);
   const double z = h[i] - level;
   dem[i] = z;
   const bool border = flat && outlet[i];
@@ -239,7 +284,7 @@ __global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
                              double sl, double* __restrict__ lfill) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double level = fill_level(level_in, lv);
+  const double level = fill_level(level_in, lv, i);
   const double z = dem[i];
   double f = z;
   if (state[i] >= 0) {
@@ -584,6 +629,23 @@ static int until_stable(gospl_mesh* m, hipStream_t s, int max_iter, int check_ev
   return 0;
 }
 
+int ensure_seg(gospl_mesh* m, int nseg) {
+  if (nseg <= m->seg_cap) return 0;
+  if (m->seg_keys) GOSPL_CHECK(hipFree(m->seg_keys));
+  if (m->seg_ctl) GOSPL_CHECK(hipFree(m->seg_ctl));
+  if (m->seg_aux) GOSPL_CHECK(hipFree(m->seg_aux));
+  m->seg_keys = nullptr;
+  m->seg_ctl = nullptr;
+  m->seg_aux = nullptr;
+  m->seg_cap = 0;
+  const int cap = nseg < 256 ? 256 : nseg;
+  GOSPL_CHECK(hipMalloc(&m->seg_keys, sizeof(unsigned long long) * cap));
+  GOSPL_CHECK(hipMalloc(&m->seg_ctl, sizeof(double) * 4 * cap));
+  GOSPL_CHECK(hipMalloc(&m->seg_aux, sizeof(int32_t) * 4 * (cap + 1)));
+  m->seg_cap = cap;
+  return 0;
+}
+
 }  // namespace gospl
 
 using namespace gospl;
@@ -632,7 +694,7 @@ static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel l
 
 int gospl_fill_depressions(gospl_mesh_t* m, const double* h, double level, double sealevel,
                            double* lfill, gospl_stream_t s) {
-  return fill_body(m, h, level, FillLevel{nullptr, 0, 0.0}, sealevel, lfill, s);
+  return fill_body(m, h, level, FillLevel{nullptr, 0, 0.0, nullptr}, sealevel, lfill, s);
 }
 
 int gospl_fill_elevation(gospl_mesh_t* m, const double* h, int32_t add_eps, double floor_level,
@@ -641,7 +703,20 @@ int gospl_fill_elevation(gospl_mesh_t* m, const double* h, int32_t add_eps, doub
   GOSPL_CHECK(hipMemsetAsync(key, 0xff, sizeof(unsigned long long), s));
   k_min_key<<<FILL_MIN_BLOCKS, GOSPL_BLOCK, 0, s>>>(m->N, h, key);
   GOSPL_CHECK(hipGetLastError());
-  return fill_body(m, h, 0.0, FillLevel{key, add_eps ? 1 : 0, floor_level}, sealevel, lfill, s);
+  return fill_body(m, h, 0.0, FillLevel{key, add_eps ? 1 : 0, floor_level, nullptr}, sealevel, lfill, s);
+}
+
+int gospl_fill_elevation_seg(gospl_mesh_t* m, const double* h, int32_t add_eps, double floor_level,
+                             double sealevel, const int32_t* seg, int32_t nseg, double* lfill, gospl_stream_t s) {
+  if (nseg < 1 || !seg) {
+    set_error("gospl_fill_elevation_seg: need a segment array and nseg >= 1");
+    return 1;
+  }
+  if (ensure_seg(m, nseg)) return 1;
+  GOSPL_CHECK(hipMemsetAsync(m->seg_keys, 0xff, sizeof(unsigned long long) * nseg, s));
+  k_min_key_seg<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, h, seg, m->seg_keys);
+  GOSPL_CHECK(hipGetLastError());
+  return fill_body(m, h, 0.0, FillLevel{m->seg_keys, add_eps ? 1 : 0, floor_level, seg}, sealevel, lfill, s);
 }
 
 int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double sealevel,

@@ -224,7 +224,7 @@ static void launch_receivers(gospl_mesh* m, double fexp, const double* h, double
                              const int32_t* flatdirs, int32_t* rcv, double* dist, double* wgt,
                              int8_t* lsink, hipStream_t s) {
   k_receivers<NR><<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(
-      m->N, m->W, m->nbr, m->deg, m->elgt, m->gid, m->outlet, m->flat, fexp, h, sl, lfill, post,
+      m->N, m->W, m->nbr, m->deg, m->elgt, m->tgid, m->outlet, m->flat, fexp, h, sl, lfill, post,
       down, pitids, flatdirs, rcv, dist, wgt, lsink);
 }
 

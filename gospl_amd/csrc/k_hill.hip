@@ -11,6 +11,9 @@
 // built from, and a call rebuilds only the rows where one of those flags
 // changed (goSPL keeps its operator until the coastline moves,
 // hillslope.py:184-200); new diffusivities or dt rebuild every row.
+#include <cstring>
+#include <vector>
+
 #include "common.h"
 
 namespace gospl {
@@ -335,6 +338,147 @@ __global__ void k_hill_sweep32(int64_t N, int W, const int32_t* __restrict__ nbr
   dout[i] = s / d;
 }
 
+// Segmented refined solve (ensembles: one union mesh of independent
+// members, each segment a run of whole workgroups).  Every segment keeps
+// its own control words, residual norm (summed over its own workgroups in
+// the order of the single-mesh solve), sweep count and stop test, so each
+// member's result equals its own solve's bit for bit.  aux (int32):
+// [0, nseg) first workgroup, [nseg, 2 nseg) workgroup count, [2 nseg,
+// 3 nseg) sweeps, [3 nseg, 4 nseg) first vertex.  Padding vertices
+// (segment < 0) keep x = h.
+__global__ void k_hill_resid_seg(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                                 const double* __restrict__ coef, const double* __restrict__ diag,
+                                 const double* __restrict__ h, const double* __restrict__ x, float* __restrict__ r32,
+                                 double* __restrict__ part, const int32_t* __restrict__ seg,
+                                 const HillCtl* __restrict__ ctl) {
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  const int bs = b0 < N ? seg[b0] : -1;  // a workgroup lies in one segment and starts on a real vertex
+  if (bs < 0 || ctl[bs].done != 0.0) return;
+  const int64_t i = b0 + threadIdx.x;
+  double rr = 0.0, hh = 0.0;
+  if (i < N && seg[i] >= 0) {
+    const double d = diag[i];
+    const double hi = h[i];
+    double r = hi - x[i];
+    if (d != 0.0) {
+      double s = d * x[i];
+      const int dg = deg[i];
+      for (int p = 0; p < W && p < dg; ++p) {
+        const double c = coef[(int64_t)p * N + i];
+        if (c != 0.0) s = s + c * x[nbr[(int64_t)p * N + i]];
+      }
+      r = hi - s;
+    }
+    r32[i] = (float)r;
+    rr = r * r;
+    hh = hi * hi;
+  } else if (i < N) {
+    r32[i] = 0.0f;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    hh += __shfl_xor(hh, o, 64);
+  }
+  __shared__ double sh[2][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = rr;
+    sh[1][threadIdx.x >> 6] = hh;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = sh[0][0], b = sh[1][0];
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      a += sh[0][w];
+      b += sh[1][w];
+    }
+    part[2 * (int64_t)blockIdx.x] = a;
+    part[2 * (int64_t)blockIdx.x + 1] = b;
+  }
+}
+
+// One workgroup per segment; the partials of the segment's workgroups are
+// summed in k_hill_norm's order.
+__global__ void k_hill_norm_seg(int nseg, const double* __restrict__ part, const int32_t* __restrict__ aux,
+                                double rtol, int pass, int maxpass, HillCtl* __restrict__ ctl) {
+  const int sg = blockIdx.x;
+  if (sg >= nseg || ctl[sg].done != 0.0) return;
+  const int64_t blk0 = aux[sg];
+  const int nblk = aux[nseg + sg];
+  double rr = 0.0, hh = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    rr += part[2 * (blk0 + b)];
+    hh += part[2 * (blk0 + b) + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    hh += __shfl_xor(hh, o, 64);
+  }
+  __shared__ double sh[2][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = rr;
+    sh[1][threadIdx.x >> 6] = hh;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+    sh[0][0] += sh[0][w];
+    sh[1][0] += sh[1][w];
+  }
+  rr = sh[0][0];
+  hh = sh[1][0];
+  const double rel = hh > 0.0 ? sqrt(rr / hh) : 0.0;
+  ctl[sg].relres = rel;
+  if (pass == 0) ctl[sg].relres0 = rel;
+  if (!(rel > rtol) || pass >= maxpass) ctl[sg].done = 1.0;
+}
+
+// Sweep k of the segmented correction: segments whose count is reached copy
+// their values through.
+__global__ void k_hill_sweep32_seg(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                                   const float* __restrict__ coef32, const float* __restrict__ diag32,
+                                   const float* __restrict__ r32, const float* __restrict__ din, int k,
+                                   float* __restrict__ dout, const int32_t* __restrict__ seg, int nseg,
+                                   const int32_t* __restrict__ aux, const HillCtl* __restrict__ ctl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int sg = seg[i];
+  if (sg < 0) {
+    dout[i] = 0.0f;
+    return;
+  }
+  if (ctl[sg].done != 0.0) return;
+  if (k >= aux[2 * nseg + sg]) {
+    dout[i] = din[i];
+    return;
+  }
+  const float d = diag32[i];
+  const float r = r32[i];
+  if (d == 0.0f) {
+    dout[i] = r;
+    return;
+  }
+  float s = r;
+  if (k != 0) {
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      const float c = coef32[(int64_t)p * N + i];
+      if (c != 0.0f) s = s - c * din[nbr[(int64_t)p * N + i]];
+    }
+  }
+  dout[i] = s / d;
+}
+
+__global__ void k_hill_addcorr_seg(int64_t N, const float* __restrict__ d, double* __restrict__ x,
+                                   const int32_t* __restrict__ seg, int nseg, const int32_t* __restrict__ aux,
+                                   HillCtl* __restrict__ ctl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int sg = seg[i];
+  if (sg < 0 || ctl[sg].done != 0.0) return;
+  if (i == aux[3 * nseg + sg]) ctl[sg].pass = ctl[sg].pass + 1.0;
+  x[i] = x[i] + (double)d[i];
+}
+
 // x += d (FP64 iterate) and count the pass.
 __global__ void k_hill_addcorr(int64_t N, const float* __restrict__ d, double* __restrict__ x,
                                HillCtl* __restrict__ ctl) {
@@ -607,5 +751,93 @@ extern "C" int gospl_hillslope_adjoint(gospl_mesh_t* m, double cda, double cdm, 
   }
   GOSPL_CHECK(hipMemcpyAsync(out, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int gospl_hillslope_refine_seg(gospl_mesh_t* m, double cda, double cdm, double dt, double sealevel,
+                                          const int32_t* seg, int32_t nseg, const int64_t* offsets,
+                                          const int32_t* sweeps32, double rtol, int32_t maxpass, double* h,
+                                          double* cumed, double* eblocal, int32_t* passes, gospl_stream_t s) {
+  const int64_t N = m->N;
+  if (nseg < 1 || !seg || !offsets || !sweeps32) {
+    set_error("gospl_hillslope_refine_seg: need seg, offsets and sweeps32, nseg >= 1");
+    return 1;
+  }
+  if (offsets[0] != 0 || offsets[nseg] != N) {
+    set_error("gospl_hillslope_refine_seg: offsets must run from 0 to N (%lld)", (long long)N);
+    return 1;
+  }
+  int maxsw = 0;
+  for (int k = 0; k < nseg; ++k) {
+    if (offsets[k + 1] <= offsets[k] || offsets[k] % GOSPL_BLOCK != 0 || offsets[k + 1] - offsets[k] > INT32_MAX) {
+      set_error("gospl_hillslope_refine_seg: segment %d must be a non-empty run of whole %d-vertex workgroups", k,
+                GOSPL_BLOCK);
+      return 1;
+    }
+    if (sweeps32[k] < 1) {
+      set_error("gospl_hillslope_refine_seg: segment %d has %d FP32 sweeps (>= 1 needed)", k, (int)sweeps32[k]);
+      return 1;
+    }
+    maxsw = sweeps32[k] > maxsw ? sweeps32[k] : maxsw;
+  }
+  if (maxpass < 1) maxpass = 1;
+  if (ensure_seg(m, nseg)) return 1;
+  if (!m->seg_haux || m->seg_haux_n < 4 * nseg) {
+    free(m->seg_haux);
+    m->seg_haux = static_cast<int32_t*>(calloc(4 * (size_t)nseg, sizeof(int32_t)));
+    if (!m->seg_haux) {
+      set_error("gospl_hillslope_refine_seg: out of host memory");
+      return 1;
+    }
+    m->seg_haux_n = -1;  // forces the upload below
+  }
+  // segment layout and sweep counts: uploaded when they change
+  std::vector<int32_t> aux(4 * (size_t)nseg);
+  for (int k = 0; k < nseg; ++k) {
+    aux[k] = (int32_t)(offsets[k] / GOSPL_BLOCK);
+    aux[nseg + k] = (int32_t)((offsets[k + 1] - offsets[k]) / GOSPL_BLOCK);
+    aux[2 * nseg + k] = sweeps32[k];
+    aux[3 * nseg + k] = (int32_t)offsets[k];
+  }
+  if (m->seg_haux_n != 4 * nseg || memcmp(m->seg_haux, aux.data(), sizeof(int32_t) * 4 * nseg) != 0) {
+    GOSPL_CHECK(hipMemcpyAsync(m->seg_aux, aux.data(), sizeof(int32_t) * 4 * nseg, hipMemcpyHostToDevice, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));  // the host vector goes out of scope
+    memcpy(m->seg_haux, aux.data(), sizeof(int32_t) * 4 * nseg);
+    m->seg_haux_n = 4 * nseg;
+  }
+  const int g = nblocks(N);
+  if (hill_operator(m, cda, cdm, dt, sealevel, h, s)) return 1;
+  HillCtl* ctl = reinterpret_cast<HillCtl*>(m->seg_ctl);
+  double* x = m->wd[1];
+  double* part = m->wd[0];  // 2 per workgroup <= N + 64
+  float* r32 = reinterpret_cast<float*>(m->wi[5]);
+  float* da = reinterpret_cast<float*>(m->wi[6]);
+  float* db = reinterpret_cast<float*>(m->wi[7]);
+  GOSPL_CHECK(hipMemsetAsync(ctl, 0, sizeof(HillCtl) * nseg, s));
+  GOSPL_CHECK(hipMemcpyAsync(x, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  for (int pass = 0; pass <= maxpass; ++pass) {
+    k_hill_resid_seg<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->hcoef, m->hdiag, h, x, r32, part, seg,
+                                               ctl);
+    k_hill_norm_seg<<<nseg, GOSPL_BLOCK, 0, s>>>(nseg, part, m->seg_aux, rtol, pass, maxpass, ctl);
+    if (pass == maxpass) break;
+    const float* din = da;
+    float* dout = da;
+    for (int k = 0; k < maxsw; ++k) {
+      k_hill_sweep32_seg<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->hcoef32, m->hdiag32, r32, din, k,
+                                                   dout, seg, nseg, m->seg_aux, ctl);
+      din = dout;
+      dout = (dout == da) ? db : da;
+    }
+    k_hill_addcorr_seg<<<g, GOSPL_BLOCK, 0, s>>>(N, din, x, seg, nseg, m->seg_aux, ctl);
+    GOSPL_CHECK(hipGetLastError());
+  }
+  k_hill_update<<<g, GOSPL_BLOCK, 0, s>>>(N, x, dt, h, cumed, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  if (passes) {
+    std::vector<HillCtl> hc(nseg);
+    GOSPL_CHECK(hipMemcpyAsync(hc.data(), ctl, sizeof(HillCtl) * nseg, hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < nseg; ++k) passes[k] = (int32_t)hc[k].pass;
+  }
   return 0;
 }

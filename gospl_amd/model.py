@@ -39,6 +39,26 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
+def contraction_gmax(mesh):
+    """max_i sum_p vdist/elgt / area_i: the hillslope operator's contraction
+    bound is s/(1+s) with s = Kd*dt*gmax."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ratio = np.where(mesh.elgt > 0, mesh.vdist / np.where(mesh.elgt > 0, mesh.elgt, 1.0), 0.0)
+        g = np.where(mesh.area > 0, ratio.sum(axis=0) / np.where(mesh.area > 0, mesh.area, 1.0), 0.0)
+    return float(np.max(g)) if len(g) else 0.0
+
+
+def sweeps_for(kd_dt, gmax, bits):
+    """Jacobi sweeps that bring the contraction bound below 2^-bits (at least
+    2); 0 past 64 sweeps."""
+    s = kd_dt * gmax
+    if s <= 0.0:
+        return 1
+    rho = s / (1.0 + s)
+    n = int(math.ceil(math.log(2.0 ** -bits) / math.log(rho))) + 1
+    return max(2, n) if n <= 64 else 0
+
+
 class DeviceMesh:
     """A ``Mesh`` uploaded to the GPU plus its library handle."""
 
@@ -66,11 +86,7 @@ class DeviceMesh:
             _ptr(self.vdist), _ptr(self.area), _ptr(self.gid), _ptr(self.outlet), ctypes.byref(h)),
             "gospl_mesh_create")
         self.handle = h
-        # contraction bound of the hillslope operator: g_i = sum_p vdist/elgt / area_i
-        with np.errstate(divide="ignore", invalid="ignore"):
-            ratio = np.where(mesh.elgt > 0, mesh.vdist / np.where(mesh.elgt > 0, mesh.elgt, 1.0), 0.0)
-            g = np.where(mesh.area > 0, ratio.sum(axis=0) / np.where(mesh.area > 0, mesh.area, 1.0), 0.0)
-        self.gmax = float(np.max(g)) if len(g) else 0.0
+        self.gmax = contraction_gmax(mesh)
         self.outletIDs = torch.nonzero(self.outlet > 0).flatten() if self.flat else None
 
     def close(self):
@@ -345,25 +361,13 @@ class Model:
         contraction bound rho = s/(1+s), s = Kd*dt*gmax; 0 (the library then
         iterates to a device-checked convergence) when that takes more than
         64 sweeps."""
-        kd = max(self.Cda, self.Cdm) * self.dt
-        s = kd * self.dm.gmax
-        if s <= 0.0:
-            return 1
-        rho = s / (1.0 + s)
-        n = int(math.ceil(math.log(2.0 ** -60) / math.log(rho))) + 1
-        return max(2, n) if n <= 64 else 0
+        return sweeps_for(max(self.Cda, self.Cdm) * self.dt, self.dm.gmax, 60)
 
     def hillslope_sweeps32(self):
         """FP32 Jacobi sweeps per correction of the refined hillslope solve:
         the contraction bound brought below 2^-27 (FP32 rounding of the
         correction); 0 (the converged FP64 solve) past 64 sweeps."""
-        kd = max(self.Cda, self.Cdm) * self.dt
-        s = kd * self.dm.gmax
-        if s <= 0.0:
-            return 1
-        rho = s / (1.0 + s)
-        n = int(math.ceil(math.log(2.0 ** -27) / math.log(rho))) + 1
-        return max(2, n) if n <= 64 else 0
+        return sweeps_for(max(self.Cda, self.Cdm) * self.dt, self.dm.gmax, 27)
 
     # -------------------------------------------------------------- pits
     def fillElevation(self, sed=False):

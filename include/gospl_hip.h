@@ -348,6 +348,48 @@ int gospl_flow_finalize_noalloc(int64_t N, const double* hl, const double* water
                                 double* fillfal, gospl_stream_t stream);
 
 /*
+ * Ensembles in one mesh (no counterpart in goSPL, which runs one model per
+ * process): a union mesh of independent members, `seg` (device, N int32)
+ * giving each vertex's member 0..nseg-1, or a negative value for padding
+ * vertices.  The union's gid must stay a permutation of 0..N-1 (each member's
+ * ids plus its offset), which orders pits and spill candidates as each
+ * member's own ids do.  Each entry reproduces, per member, what its
+ * single-mesh counterpart computes on that member alone:
+ *  - gospl_mesh_set_tiebreak: the ids mfdreceivers' slope perturbation
+ *    val*(1 - 1e-15 gid) uses (functions.F90:1618-1629), device N int32 owned
+ *    by the caller (the members' own ids); NULL restores gid.
+ *  - gospl_fill_elevation_seg: gospl_fill_elevation with each member's own
+ *    level max(min h (+1e-3), floor_level) (pitfilling.py:849-870); padding
+ *    vertices get floor_level.
+ *  - gospl_flow_finalize_seg: gospl_flow_finalize with each member's own pit
+ *    flag (has_pits, device int8[nseg]) and discharge maximum.
+ *  - gospl_spl_batch: gospl_spl with the stream-power exponent m per vertex
+ *    (mvec, device N); per-member K goes through kfac.
+ *  - gospl_hillslope_refine_seg: gospl_hillslope_refine with per-member FP32
+ *    sweep counts (sweeps32, HOST int32[nseg], each >= 1), residual norms and
+ *    stop tests.  offsets (HOST int64[nseg+1]) are the members' vertex ranges,
+ *    each a run of whole 256-vertex workgroups (padding at its end), from 0 to
+ *    N.  passes (HOST int32[nseg], may be NULL: then nothing is read back)
+ *    receives each member's correction count.
+ */
+int gospl_mesh_set_tiebreak(gospl_mesh_t* mesh, const int32_t* tgid);
+int gospl_fill_elevation_seg(gospl_mesh_t* mesh, const double* h, int32_t add_eps, double floor_level,
+                             double sealevel, const int32_t* seg, int32_t nseg, double* lfill,
+                             gospl_stream_t stream);
+int gospl_flow_finalize_seg(gospl_mesh_t* mesh, const double* hl, const double* water,
+                            const int32_t* pitids, const int32_t* seg, int32_t nseg,
+                            const int8_t* has_pits, double* fal, double* fillfal, gospl_stream_t stream);
+int gospl_spl_batch(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                    const double* disti, const double* fal, const double* kfac, double K,
+                    const double* mvec, double dt, double sealevel, const double* lfill,
+                    const int8_t* lsink, double* h, double* cumed, double* eb, double* eblocal,
+                    int32_t* nlevels, gospl_stream_t stream);
+int gospl_hillslope_refine_seg(gospl_mesh_t* mesh, double cda, double cdm, double dt, double sealevel,
+                               const int32_t* seg, int32_t nseg, const int64_t* offsets,
+                               const int32_t* sweeps32, double rtol, int32_t maxpass, double* h,
+                               double* cumed, double* eblocal, int32_t* passes, gospl_stream_t stream);
+
+/*
  * Instrumentation (no counterpart in goSPL): when enabled, HIP events on the
  * launch stream bracket every DAG sweep kernel (flow accumulation and SPL).
  * Returns the accumulated milliseconds and call counts since the previous

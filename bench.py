@@ -26,7 +26,10 @@ to the FP64 path's own divergence under that nudge.
 A gradient leg (``--adjoint-level``, 8 by default: 655,362 vertices) times
 the inversion workload's forward run and its gradient in per-vertex log10 K
 and uplift (50 steps, checkpoints every 10; ``gospl_amd.adjoint``), with a
-finite-difference spot check of the gradient.
+finite-difference spot check of the gradient.  An ensemble leg
+(``--ensemble-steps``) steps 144 members of uneven size (64 x level 6,
+64 x level 7, 16 x level 8) together on one union mesh and reports
+member-steps/s beside running them one at a time.
 
 Prints one JSON line (the driver's contract) with the roofline of the
 dominant kernel of the reported leg and the CPU oracle timed on a bounded
@@ -345,6 +348,77 @@ def adjoint_leg(level, seed, dev, nfd):
     return out
 
 
+def ensemble_leg(dev, steps, warmup, seed):
+    """The ensemble workload (64 members at level 6, 64 at level 7, 16 at
+    level 8; per-member K, m and rain scale; refined mode) stepped together
+    on one union mesh: member-steps/s over ``steps`` timed steps, beside the
+    rate of running the members one after another (one member per level
+    timed alone, weighted by the member counts)."""
+    import torch
+
+    from gospl_amd import _lib, workloads
+    from gospl_amd.ensemble import EnsembleModel
+    from gospl_amd.model import Model
+
+    t0 = time.time()
+    members = workloads.ensemble_case(_lib.load(), seed=seed, device=dev)
+    log("[bench] ensemble: %d members built in %.1fs" % (len(members), time.time() - t0))
+    t0 = time.time()
+    ens = EnsembleModel(members, tend=1e12, device=dev)
+    ens.initialise()
+    torch.cuda.synchronize()
+    log("[bench] ensemble: union of %d vertices (%d with padding), initialised in %.1fs" %
+        (int(ens.sizes.sum()), ens.dm.N, time.time() - t0))
+    for i in range(warmup):
+        ens.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ens.step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nm = ens.nmem
+    log("[bench] ensemble: %d steps in %.3fs -> %.1f member-steps/s" % (steps, el, nm * steps / el))
+    ens.enable_profiling(True)
+    for i in range(2):
+        ens.step()
+    ph = {k: v[0] / 2 for k, v in ens.phase_times().items()}
+    ens.enable_profiling(False)
+    # the same members one at a time: one member per level, timed alone
+    seq = {}
+    levels = {}
+    for (m, h0, p) in members:
+        levels.setdefault(m.N, (m, h0, p, 0))
+        mm, hh, pp, c = levels[m.N]
+        levels[m.N] = (mm, hh, pp, c + 1)
+    total = 0.0
+    for n, (m, h0, p, count) in sorted(levels.items()):
+        g = Model(m, h0, tend=1e12, device=dev, refine=True, **p)
+        g.initialise()
+        for _ in range(2):
+            g.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(10):
+            g.step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) / 10 * 1e3
+        seq["%d vertices" % n] = {"members": count, "ms_per_step_alone": ms}
+        total += count * ms
+        g.close()
+    seq_rate = nm / (total / 1e3)
+    out = {"workload": "%d members (64 x 40,962, 64 x 163,842, 16 x 655,362 vertices), per-member K, m, rain "
+                       "scale, dt 5000 yr, refined mode, one union mesh" % nm,
+           "members": nm, "vertices": int(ens.sizes.sum()), "steps": steps, "warmup": warmup,
+           "elapsed_s": el, "member_steps_per_s": nm * steps / el, "ms_per_union_step": el / steps * 1e3,
+           "vertex_steps_per_s": float(ens.sizes.sum()) * steps / el,
+           "phases_ms_per_step": ph,
+           "one_member_at_a_time": {"member_steps_per_s": seq_rate, "per_level": seq},
+           "speedup_over_one_at_a_time": (nm * steps / el) / seq_rate}
+    ens.close()
+    return out
+
+
 def rocpd_summary(db, first, count, mode, N):
     """Markdown summary of a rocprofv3 database (--kernel-trace): kernels by
     total time, and the exact sweep dispatches first .. first + count - 1
@@ -392,6 +466,8 @@ def main():
                     help="sphere level of the gradient (inversion) leg; 0 skips it")
     ap.add_argument("--adjoint-fd", type=int, default=4,
                     help="vertices per input of the leg's finite-difference check")
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="timed steps of the 144-member ensemble leg; 0 skips it")
     ap.add_argument("--surface", default=None, metavar="NPY",
                     help="initial surface (input order) saved by --save-surface instead of the SH field "
                          "(keeps the SH construction's kernels out of a counter run)")
@@ -478,6 +554,9 @@ def main():
     adj = None
     if args.adjoint_level > 0:
         adj = adjoint_leg(args.adjoint_level, args.seed, dev, args.adjoint_fd)
+    ens = None
+    if args.ensemble_steps > 0:
+        ens = ensemble_leg(dev, args.ensemble_steps, 2, args.seed)
 
     leg_cfg = {}
     for name, lg in legs.items():
@@ -501,6 +580,8 @@ def main():
         cfg["accuracy"] = acc
     if adj is not None:
         cfg["adjoint"] = adj
+    if ens is not None:
+        cfg["ensemble"] = ens
     dtype = {"refined": "f64 (DAG sweeps, state, residuals) + f32 (hillslope corrections)",
              "refined_dag": "f64 (state, residuals) + i64 fixed-point (DAG corrections) + f32 (hillslope)",
              "fp64": "f64"}[main_leg]

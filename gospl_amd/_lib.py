@@ -55,6 +55,8 @@ SIGNATURES = {
     "gospl_cascade_add": (I32, [P, P, P, P, F64, P, P, P]),
     "gospl_flow_finalize": (I32, [P, P, P, P, I32, P, P, P]),
     "gospl_mesh_set_tiebreak": (I32, [P, P]),
+    "gospl_dag_set_shape": (I32, [P, I32, I32, I32]),
+    "gospl_dag_get_shape": (I32, [P, P]),
     "gospl_fill_elevation_seg": (I32, [P, P, I32, F64, F64, P, I32, P, P]),
     "gospl_flow_finalize_seg": (I32, [P, P, P, P, P, I32, P, P, P, P]),
     "gospl_spl_batch": (I32, [P, I32, P, P, P, P, P, F64, P, F64, F64, P, P, P, P, P, P, ctypes.POINTER(I32), P]),

@@ -32,6 +32,7 @@ struct gospl_mesh {
   int dag_csr;                // use the CSR rows even when packed records apply
   int dag_chain;              // flow accumulation as a dataflow sweep (records only)
   int chain_blocks[2];        // its grid, by sweep (flow, stream power)
+  int chain_resident[2];      // workgroups of it that fit on the device at once
   int chain_pool;             // its pool capacity per wavefront
   int chain_share;            // pool size from which batches go to the global queue
   int chain_pollers;          // idle wavefronts that stay to poll it

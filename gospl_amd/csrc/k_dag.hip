@@ -1697,8 +1697,11 @@ extern "C" int gospl_dag_init_occupancy(gospl_mesh* m) {
       set_error("the dataflow kernel does not fit on a compute unit");
       return 1;
     }
+    m->chain_resident[i] = b * m->num_cu;
     m->chain_blocks[i] = m->num_cu;
     if (te_cb && atoi(te_cb) > 0) m->chain_blocks[i] = atoi(te_cb);
+    // every workgroup resident at once
+    if (m->chain_blocks[i] > m->chain_resident[i]) m->chain_blocks[i] = m->chain_resident[i];
   }
   // pool capacity per wavefront (GOSPL_CHAIN_POOL, 128..512) and the pool
   // size kept when batches go to the global queue (GOSPL_CHAIN_SHARE,
@@ -1879,6 +1882,40 @@ int gospl_spl_batch(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const do
   }
   return spl_step(m, nrcv, rcvi, wgti, disti, fal, kfac, K, 0.0, mvec, dt, sealevel, lfill, lsink, h, cumed, eb,
                   eblocal, nlevels, stream);
+}
+
+int gospl_dag_set_shape(gospl_mesh_t* m, int32_t chain_blocks, int32_t chain_share, int32_t chain_pollers) {
+  if (!m) {
+    set_error("gospl_dag_set_shape: null handle");
+    return 1;
+  }
+  const int lim = m->chain_resident[0] < m->chain_resident[1] ? m->chain_resident[0] : m->chain_resident[1];
+  if (chain_blocks > lim) {
+    set_error("gospl_dag_set_shape: %d workgroups exceed the %d that fit on the device at once", (int)chain_blocks,
+              lim);
+    return 1;
+  }
+  if (chain_share > m->chain_pool - 64 || chain_pollers > 4096) {
+    set_error("gospl_dag_set_shape: share must be <= %d and pollers <= 4096", m->chain_pool - 64);
+    return 1;
+  }
+  if (chain_blocks > 0) m->chain_blocks[0] = m->chain_blocks[1] = chain_blocks;
+  if (chain_share >= 0) m->chain_share = chain_share;
+  if (chain_pollers >= 0) m->chain_pollers = chain_pollers;
+  return 0;
+}
+
+int gospl_dag_get_shape(gospl_mesh_t* m, int32_t* out) {
+  if (!m || !out) {
+    set_error("gospl_dag_get_shape: null argument");
+    return 1;
+  }
+  out[0] = m->chain_blocks[0];
+  out[1] = m->chain_share;
+  out[2] = m->chain_pollers;
+  out[3] = m->chain_resident[0] < m->chain_resident[1] ? m->chain_resident[0] : m->chain_resident[1];
+  out[4] = m->num_cu;
+  return 0;
 }
 
 int gospl_flowacc_adjoint(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcv, const double* wgt,

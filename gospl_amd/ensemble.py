@@ -90,11 +90,14 @@ class EnsembleModel(Model):
     order), params) with params as ``Model``'s keywords; ``K``, ``m``,
     ``rain`` and ``upsub`` may differ between members, the other keywords
     must agree.  ``member(k, t)`` is member k's slice of a per-vertex
-    tensor (e.g. ``hLocal``)."""
+    tensor (e.g. ``hLocal``).  ``dag_shape="tuned"`` launches the dataflow
+    sweeps with the shape recorded by ``gospl_amd.autotune`` for ensembles
+    (None: the library's default); results do not depend on it."""
 
     COMMON = ("dt", "flowdir", "cda", "cdm", "sealevel", "ofill", "tstart")
 
-    def __init__(self, members, *, device="cuda", tend=None, refine_rtol=1.0e-12, refine_maxpass=4):
+    def __init__(self, members, *, device="cuda", tend=None, refine_rtol=1.0e-12, refine_maxpass=4,
+                 dag_shape="tuned"):
         if not members:
             raise ValueError("EnsembleModel: no members")
         meshes = [m for m, _, _ in members]
@@ -160,6 +163,10 @@ class EnsembleModel(Model):
         self._lsink_eff = torch.zeros(N, dtype=torch.int8, device=dev)
         self._has_pits = torch.zeros(self.nmem, dtype=torch.int8, device=dev)
         self.member_cascade_passes = np.zeros(self.nmem, np.int64)
+        # launch shape of the dataflow sweeps: the autotuner's choice for
+        # ensembles (a union's sweeps are throughput-bound), when recorded
+        from .autotune import apply_tuned
+        self.dag_shape = apply_tuned(self, "ensemble") if dag_shape == "tuned" else None
 
     def member(self, k, t):
         o = int(self.offsets[k])

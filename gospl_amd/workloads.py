@@ -70,6 +70,32 @@ def adjoint_case(lib, level: int = 8, seed: int = 0, lmax: int = 64, device=None
     return m, h0, p, logk_true, logk0, uplift
 
 
+def ensemble_case(lib, counts=((6, 64), (7, 64), (8, 16)), seed: int = 0, lmax: int = 64, device=None,
+                  order="morton"):
+    """The ensemble workload: members on icosahedral spheres of uneven size
+    (default 64 at level 6, 64 at level 7, 16 at level 8), each initialised
+    as ``sphere_case`` with its own seed, with per-member K log-uniform in
+    [1e-6, 1e-5], m uniform in [0.3, 0.6] and a rain scale uniform in
+    [0.5, 2]; dt 5000 yr, 50 steps.  Returns [(mesh, h0, params)] (the mesh
+    of a level is built once and shared)."""
+    rng = np.random.default_rng(seed + 1000)
+    members = []
+    k = 0
+    for level, n in counts:
+        coords, cells = _mesh.icosphere(level, 6.371e6)
+        m = _mesh.build_mesh(coords, cells, lib, flat=False, order=order)
+        for _ in range(n):
+            s = seed + 1 + k
+            z = fields.affine_range(fields.sh_field(coords, lmax, 2.0, seed=s, device=device), -6000.0, 5000.0)
+            rain = fields.lognormal_rain(m.N, 1.0, 0.6, 0.2, 3.0, seed=s + 1)
+            p = dict(K=float(10.0 ** rng.uniform(-6.0, -5.0)), m=float(rng.uniform(0.3, 0.6)), d=0.0, flowdir=3,
+                     cda=0.5, cdm=0.5, dt=5000.0, rain=m.to_device_order(rain) * float(rng.uniform(0.5, 2.0)),
+                     upsub=None, sealevel=0.0)
+            members.append((m, m.to_device_order(z), p))
+            k += 1
+    return members
+
+
 def oracle_geometry(m: "_mesh.Mesh"):
     """Input-order geometry dict for ``oracle.OracleModel``."""
     ex = m.extras

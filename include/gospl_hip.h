@@ -373,6 +373,19 @@ int gospl_flow_finalize_noalloc(int64_t N, const double* hl, const double* water
  *    receives each member's correction count.
  */
 int gospl_mesh_set_tiebreak(gospl_mesh_t* mesh, const int32_t* tgid);
+
+/*
+ * Launch shape of the receiver-DAG dataflow sweeps (no counterpart in
+ * goSPL): chain_blocks workgroups (<= the number that fit on the device at
+ * once; <= 0 keeps), chain_share = pool size from which a wavefront hands
+ * batches to the global queue (< 0 keeps), chain_pollers = idle wavefronts
+ * that stay to poll it (< 0 keeps).  Results do not depend on the shape.
+ * gospl_dag_get_shape: out (HOST int32[5]) = chain_blocks, chain_share,
+ * chain_pollers, the resident limit, the compute-unit count.
+ */
+int gospl_dag_set_shape(gospl_mesh_t* mesh, int32_t chain_blocks, int32_t chain_share,
+                        int32_t chain_pollers);
+int gospl_dag_get_shape(gospl_mesh_t* mesh, int32_t* out);
 int gospl_fill_elevation_seg(gospl_mesh_t* mesh, const double* h, int32_t add_eps, double floor_level,
                              double sealevel, const int32_t* seg, int32_t nseg, double* lfill,
                              gospl_stream_t stream);

@@ -69,3 +69,28 @@ def test_members_match_single_runs_bit_for_bit():
         print("  step %d: identical (cascade passes %s)" % (step, list(ens.member_cascade_passes)), flush=True)
     assert casc > 0, "no member ran a pit-overflow cascade: the test would not cover it"
     assert ens.member_cascade_passes[-1] == 0 and ens.member_cascade_passes[:-1].min() > 0
+
+
+def test_dag_launch_shape_does_not_change_results():
+    """The dataflow sweeps' launch shape (the autotuner's knob) changes their
+    time only: a level-6 sphere stepped with the default shape and with
+    four workgroups per compute unit, pool hand-off at 64 and 128 pollers
+    gives the same bits."""
+    import torch
+
+    from gospl_amd import _lib, autotune, workloads
+    from gospl_amd.model import Model
+
+    m, h0, p = workloads.sphere_case(_lib.load(), level=6, seed=2, lmax=32)
+    runs = []
+    for shape in (None, "wide"):
+        g = Model(m, h0, tend=1e12, refine=True, **p)
+        if shape == "wide":
+            sh = autotune.get_shape(g)
+            autotune.set_shape(g, min(4 * sh["num_cu"], sh["resident"]), 64, 128)
+            assert autotune.get_shape(g)["chain_blocks"] > sh["num_cu"]
+        g.initialise()
+        for _ in range(4):
+            g.step()
+        runs.append((g.hLocal.clone(), g.FAL.clone()))
+    assert torch.equal(runs[0][0], runs[1][0]) and torch.equal(runs[0][1], runs[1][1])

@@ -58,6 +58,7 @@ ALG_BYTES = {
     "gospl_pit_info": 64,
     "gospl_flow_direction": 156,
     "gospl_flowacc": 68,
+    "gospl_flow_accumulation": 68,   # its flow sweeps (the native flowAccumulation call)
     "gospl_spl": 116,
     "gospl_hillslope": 116,
     "gospl_flowacc_refine": 68,
@@ -68,6 +69,7 @@ METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel 
 STEP_BYTES = 564  # compulsory bytes per vertex and step (sum over the phases)
 # the sweep kernels between the library's timing events, per entry point
 SWEEP_KERNELS = {"gospl_flowacc": ("k_dag_level0<0> + k_chain<0>", 0),
+                 "gospl_flow_accumulation": ("k_dag_level0<0> + k_chain<0>", 0),
                  "gospl_spl": ("k_dag_level0<1> + k_chain<1>", 1),
                  "gospl_flowacc_refine": ("k_rx_level0<0> + k_rx_chain<0>", 0),
                  "gospl_spl_refine": ("k_rx_level0<1> + k_rx_chain<1>", 1)}
@@ -82,6 +84,7 @@ PMC_KIB = {
     "gospl_flowacc": {"FETCH_SIZE": 213481.8 + 50784.1, "WRITE_SIZE": 15297.1 + 139938.3},
     "gospl_spl": {"FETCH_SIZE": 196818.8 + 46179.1, "WRITE_SIZE": 17258.1 + 132079.6},
 }
+PMC_KIB["gospl_flow_accumulation"] = PMC_KIB["gospl_flowacc"]
 
 
 def pmc_traffic(entry, level):
@@ -515,7 +518,8 @@ def main():
     L = legs[main_leg]
     sps = L["sps"]
     roof = dominant_kernel(L, N, args.level)
-    sys_ = "flow" if roof["entry_point"] in ("gospl_flowacc", "gospl_flowacc_refine") else "spl"
+    sys_ = "flow" if roof["entry_point"] in ("gospl_flowacc", "gospl_flowacc_refine",
+                                             "gospl_flow_accumulation") else "spl"
     a0, a1 = L["window"][sys_]
     roof["window"] = ("steps %d-%d of the %s leg (after %d warm-up and %d timed steps); exact %s sweeps "
                       "%d-%d of the process (0-based dispatch order of the sweep kernels)" % (

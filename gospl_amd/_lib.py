@@ -20,6 +20,26 @@ I64 = ctypes.c_int64
 I32 = ctypes.c_int32
 F64 = ctypes.c_double
 
+class FlowParams(ctypes.Structure):
+    """gospl_flow_params."""
+    _fields_ = [("nrcv", I32), ("flowexp", F64), ("sealevel", F64), ("floor_level", F64), ("add_eps", I32),
+                ("dt", F64), ("maxarea", F64), ("max_passes", I32), ("patience", I32), ("rel_improve", F64)]
+
+
+class FlowBuffers(ctypes.Structure):
+    """gospl_flow_buffers (device pointers)."""
+    _fields_ = [(n, P) for n in ("h", "rain", "lfill", "pitids", "flatdirs", "ptdir", "lspill", "rcv0", "dist0",
+                                 "wgt0", "lsink0", "rcv1", "dist1", "wgt1", "lsink1", "b", "fal", "fillfal",
+                                 "water", "hl", "q", "nfa", "rhs", "fa")] + \
+        [("cap", I64)] + [(n, P) for n in ("spill", "pitparams", "flvl", "fvol", "inv")]
+
+
+class FlowResult(ctypes.Structure):
+    """gospl_flow_result."""
+    _fields_ = [("npits", I64), ("cascade_passes", I32), ("flow_solves", I32), ("undrained", I64),
+                ("last_set", I32)]
+
+
 # name -> (restype, argtypes).  Mirrors include/gospl_hip.h.
 SIGNATURES = {
     "gospl_last_error": (ctypes.c_char_p, []),
@@ -62,6 +82,10 @@ SIGNATURES = {
     "gospl_spl_batch": (I32, [P, I32, P, P, P, P, P, F64, P, F64, F64, P, P, P, P, P, P, ctypes.POINTER(I32), P]),
     "gospl_hillslope_refine_seg": (I32, [P, F64, F64, F64, F64, P, I32, P, P, F64, I32, P, P, P, P, P]),
     "gospl_flow_finalize_noalloc": (I32, [I64, P, P, P, I32, P, P, P]),
+    "gospl_flow_accumulation": (I32, [P, ctypes.POINTER(FlowParams), ctypes.POINTER(FlowBuffers),
+                                      ctypes.POINTER(FlowResult), P]),
+    "gospl_flow_tables": (I32, [P, ctypes.POINTER(P), ctypes.POINTER(I64)]),
+    "gospl_np_sum": (F64, [P, I64]),
     "gospl_sweep_timing": (I32, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64),
                                  ctypes.POINTER(F64), ctypes.POINTER(I64)]),
 }

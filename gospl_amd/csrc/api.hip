@@ -334,6 +334,7 @@ static int mesh_alloc(gospl_mesh* m) {
   GOSPL_CHECK(hipGetDeviceProperties(&prop, m->device));
   m->num_cu = prop.multiProcessorCount;
   for (int k = 0; k < 8; ++k) GOSPL_CHECK(hipMalloc(&m->wi[k], sizeof(int32_t) * (N + 64)));
+  for (int k = 0; k < 5; ++k) GOSPL_CHECK(hipMalloc(&m->wdag[k], sizeof(int32_t) * (N + 64)));
   for (int k = 0; k < 2; ++k) GOSPL_CHECK(hipMalloc(&m->wu[k], sizeof(unsigned long long) * (N + 64)));
   for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
@@ -346,6 +347,7 @@ static int mesh_alloc(gospl_mesh* m) {
   if (b2 > bytes) bytes = b2;
   m->cub_tmp_bytes = bytes + 256;
   GOSPL_CHECK(hipMalloc(&m->cub_tmp, m->cub_tmp_bytes));
+  GOSPL_CHECK(hipMalloc(&m->dag_tmp, m->cub_tmp_bytes));
   if (gospl_dag_init_occupancy(m)) return 1;
   if (gospl_ensure_coef(m, 3)) return 1;
   return 0;
@@ -416,6 +418,12 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
     if (p) (void)hipFree(p);
   };
   for (int k = 0; k < 8; ++k) dfree(m->wi[k]);
+  for (int k = 0; k < 5; ++k) dfree(m->wdag[k]);
+  dfree(m->dag_tmp);
+  if (m->side) {
+    (void)hipStreamDestroy(m->side);
+    for (int k = 0; k < 3; ++k) (void)hipEventDestroy(m->side_ev[k]);
+  }
   for (int k = 0; k < 2; ++k) dfree(m->wu[k]);
   for (int k = 0; k < 4; ++k) dfree(m->wd[k]);
   dfree(m->counts);
@@ -449,6 +457,9 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->seg_ctl);
   dfree(m->seg_aux);
   free(m->seg_haux);
+  if (m->flow_htab) (void)hipHostFree(m->flow_htab);
+  if (m->flow_hstage) (void)hipHostFree(m->flow_hstage);
+  dfree(m->flow_dstage);
   dfree(m->hdiag32);
   dfree(m->hsea);
   delete m;

@@ -42,6 +42,13 @@ struct gospl_mesh {
   long long* trace_v;         // its per-vertex solve times (GOSPL_DAG_TRACE)
   // workspace
   int32_t* wi[8];             // 8 x N int32
+  int32_t* wdag[5];           // 5 x N int32 of the DAG sweeps alone (they may overlap pit information)
+  void* dag_tmp;              // their rocPRIM workspace
+  hipStream_t side;           // second stream of the native flow driver (default priority)
+  int dag_pending_chain;      // what the issued sweep was (read by its finish)
+  int dag_pending_trace;
+  int dag_pending_ltrace;
+  hipEvent_t side_ev[3];
   unsigned long long* wu[2];  // 2 x N uint64
   double* wd[4];              // 4 x N double
   int32_t* counts;            // N + 2 level counters
@@ -77,6 +84,12 @@ struct gospl_mesh {
   int seg_cap;
   int32_t* seg_haux;          // host copy of the last seg_aux upload
   int seg_haux_n;
+  // native flow driver: pinned per-pit tables and staging, device staging
+  double* flow_htab;
+  double* flow_hstage;
+  double* flow_dstage;
+  int64_t flow_cap;           // doubles in each
+  int64_t flow_n1;            // rows of the last call's tables
   int8_t* hsea;               // the sea flags the rows were built from
   double hpar[3];             // cda, cdm, dt they were built with
   int hvalid;
@@ -113,6 +126,11 @@ int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double
 int ensure_records(gospl_mesh* m, hipStream_t s);
 int ensure_donors(gospl_mesh* m, int64_t cap);
 int ensure_seg(gospl_mesh* m, int nseg);  // per-segment keys / control words (k_fill.hip)
+// gospl_flowacc in two halves (k_dag.hip): enqueue, then wait and read the status
+int flowacc_issue(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* b, double* q,
+                  hipStream_t s);
+int flowacc_finish(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, int32_t* nlevels,
+                   int64_t* unsolved, hipStream_t s);
 
 // Refined solve on the receiver DAG (k_refine.hip): x (in: initial guess,
 // out: solution) of the flow system (mode 0: (I - W^T) x = rhs, coef = W) or

@@ -1404,16 +1404,23 @@ int ensure_donors(gospl_mesh* m, int64_t cap) {
   return 0;
 }
 
-static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
-                     const double* diag, const double* rhs, double* x, int32_t* nlevels,
-                     int64_t* unsolved, hipStream_t s) {
+// A sweep in two halves: dag_issue enqueues everything, the status words
+// included (their copies to the host flags too), without waiting;
+// dag_finish waits for the stream and reads them.  The sweeps use their
+// own integer workspace (wdag), so pit information may run beside them on
+// another stream.
+static int dag_finish(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
+                      int32_t* nlevels, int64_t* unsolved, hipStream_t s);
+
+static int dag_issue(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
+                     const double* diag, const double* rhs, double* x, hipStream_t s) {
   const int64_t N = m->N;
   const int g = nblocks(N);
-  int32_t* pend = m->wi[0];
-  int32_t* fa = m->wi[1];
-  int32_t* fb = m->wi[2];
-  int32_t* dcnt = m->wi[4];
-  int32_t* doff = m->wi[5];
+  int32_t* pend = m->wdag[0];
+  int32_t* fa = m->wdag[1];
+  int32_t* fb = m->wdag[2];
+  int32_t* dcnt = m->wdag[3];
+  int32_t* doff = m->wdag[4];
   int32_t* counts = m->counts;
   int32_t* dev = m->dflag;
   // packed records: up to 3 receivers, int32 word indices, row masks of W * nrcv <= 64 bits
@@ -1445,7 +1452,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (ensure_donors(m, (int64_t)nrcv * m->W * N)) return 1;
     unsigned long long* rowmask = m->wu[0];
     GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
-    int8_t* rpos = reinterpret_cast<int8_t*>(m->wi[5]);  // 3 N bytes
+    int8_t* rpos = reinterpret_cast<int8_t*>(m->wdag[4]);  // 3 N bytes
     k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos,
                                          zl);
     if (mode == 0)
@@ -1461,7 +1468,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     if (slot) GOSPL_CHECK(hipMemsetAsync(slot, 0xff, sizeof(int32_t) * nrcv * N, s));
     k_dep_count<<<nblocks(N + 1), GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, dcnt);
     size_t bytes = m->cub_tmp_bytes;
-    GOSPL_CHECK(rocprim::exclusive_scan(m->cub_tmp, bytes, dcnt, doff, 0, (size_t)(N + 1),
+    GOSPL_CHECK(rocprim::exclusive_scan(m->dag_tmp, bytes, dcnt, doff, 0, (size_t)(N + 1),
                                         rocprim::plus<int32_t>(), s));
     k_dep_fill<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, m->wdon, slot);
     k_pending<<<g, GOSPL_BLOCK, 0, s>>>(mode, N, m->W, nrcv, m->nbr, m->deg, rcv, coef, doff, pend, dev + 3);
@@ -1532,10 +1539,31 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   int32_t* left = fa;  // the frontier lists are free again
   k_dag_unsolved<<<g, GOSPL_BLOCK, 0, s>>>(N, pend, x, dev + 1, left);
   GOSPL_CHECK(hipGetLastError());
+  if (mode == 0) {
+    // a receiver cycle: iterate on what the sweep left over (nothing to do,
+    // and converged at once, when the sweep solved every vertex)
+    if (rec) k_flow_cycles<true><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
+    else k_flow_cycles<false><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
+    GOSPL_CHECK(hipGetLastError());
+  }
   GOSPL_CHECK(hipMemcpyAsync(m->hflag, dev, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, s));
   // abort word
   GOSPL_CHECK(hipMemcpyAsync(m->hflag + 34, chain ? m->chain_ctl + CQ_ABORT : (int32_t*)(bar + 2),
                              sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  m->dag_pending_chain = chain ? 1 : 0;
+  m->dag_pending_trace = vtrace != nullptr;
+  m->dag_pending_ltrace = trace != nullptr;
+  return 0;
+}
+
+static int dag_finish(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
+                      int32_t* nlevels, int64_t* unsolved, hipStream_t s) {
+  const int64_t N = m->N;
+  const bool chain = m->dag_pending_chain != 0;
+  int32_t* gq = m->chain_gq;
+  const int64_t gq_cap = N / 64 + 1;
+  long long* vtrace = m->dag_pending_trace ? m->trace_v : nullptr;
+  long long* trace = m->dag_pending_ltrace ? m->trace_buf : nullptr;
   GOSPL_CHECK(hipStreamSynchronize(s));
   if (m->hflag[34]) {
     // an aborted dataflow sweep can leave queue entries behind
@@ -1549,15 +1577,7 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
               "neighbour list)", m->hflag[3]);
     return 1;
   }
-  if (mode == 0 && m->hflag[1] > 0) {
-    // a receiver cycle: iterate on what the sweep left over
-    if (rec) k_flow_cycles<true><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
-    else k_flow_cycles<false><<<1, CYCLE_THREADS, 0, s>>>(a, left, dev + 1, m->wd[1], dev + 2);
-    GOSPL_CHECK(hipGetLastError());
-    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 2, dev + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GOSPL_CHECK(hipStreamSynchronize(s));
-    if (m->hflag[2]) m->hflag[1] = 0;
-  }
+  if (mode == 0 && m->hflag[1] > 0 && m->hflag[2]) m->hflag[1] = 0;  // the cycle iteration converged
   if (m->timing) {
     float ms = 0.0f;
     GOSPL_CHECK(hipEventElapsedTime(&ms, m->tev[0], m->tev[1]));
@@ -1642,6 +1662,27 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   if (nlevels) *nlevels = m->hflag[0];
   if (unsolved) *unsolved = m->hflag[1];
   return 0;
+}
+
+static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef,
+                     const double* diag, const double* rhs, double* x, int32_t* nlevels,
+                     int64_t* unsolved, hipStream_t s) {
+  if (dag_issue(m, mode, nrcv, rcv, coef, diag, rhs, x, s)) return 1;
+  return dag_finish(m, mode, nrcv, rcv, coef, nlevels, unsolved, s);
+}
+
+int flowacc_issue(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* b, double* q,
+                  hipStream_t s) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_flowacc: nrcv must be in 1..12");
+    return 1;
+  }
+  return dag_issue(m, 0, nrcv, rcv, wgt, nullptr, b, q, s);
+}
+
+int flowacc_finish(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, int32_t* nlevels,
+                   int64_t* unsolved, hipStream_t s) {
+  return dag_finish(m, 0, nrcv, rcv, wgt, nlevels, unsolved, s);
 }
 
 }  // namespace gospl

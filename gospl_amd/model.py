@@ -236,6 +236,8 @@ class Model:
         self.solve_log = collections.deque(maxlen=4096)
         self.refine_stats = False  # also log the hillslope passes (one read-back per call)
         self._sched_state = {}
+        self.native_flow = True
+        self._fbuf = None
 
     @staticmethod
     def _from_file(filename, device):
@@ -517,6 +519,75 @@ class Model:
         return excess, pitVol, nFA
 
     def flowAccumulation(self):
+        """flowplex.py:752-945.  The exact-sweep modes run it as one native
+        call (``gospl_flow_accumulation``: the same entry points and the
+        cascade's decisions in C++); ``native_flow = False``, and the
+        refined DAG mode (warm-started flow solves), run the Python driver
+        below over the same entry points."""
+        if self.native_flow and not self.refine_dag:
+            return self._flowAccumulation_native()
+        return self._flowAccumulation_host()
+
+    def _flow_buffers(self):
+        if self._fbuf is None:
+            N = self.lpoints
+            f64 = dict(dtype=torch.float64, device=self.device)
+            self._water = torch.zeros(N, **f64)
+            self._hl = torch.zeros(N, **f64)
+            self._fa = torch.zeros(N, **f64)
+            r0, r1 = self._rcv_sets
+            fb = _lib.FlowBuffers()
+            for name, t in (("h", self.hLocal), ("rain", self.rainVal), ("lfill", self.lFill),
+                            ("pitids", self.pitIDs), ("flatdirs", self.flatDirs), ("ptdir", self.ptdir),
+                            ("lspill", self.lspill), ("rcv0", r0[0]), ("dist0", r0[1]), ("wgt0", r0[2]),
+                            ("lsink0", r0[3]), ("rcv1", r1[0]), ("dist1", r1[1]), ("wgt1", r1[2]),
+                            ("lsink1", r1[3]), ("b", self._b), ("fal", self.FAL), ("fillfal", self.fillFAL),
+                            ("water", self._water), ("hl", self._hl), ("q", self._q), ("nfa", self._nfa),
+                            ("rhs", self._rhs), ("fa", self._fa), ("spill", self._spill),
+                            ("pitparams", self._pitparams), ("flvl", self._flvl), ("fvol", self._fvol),
+                            ("inv", self._inv)):
+                setattr(fb, name, t.data_ptr())
+            fb.cap = self._cap
+            self._fbuf = fb
+            self._fbuf_keep = (self.hLocal, self.rainVal)
+        # the state tensors may have been replaced since (hLocal, rainVal)
+        self._fbuf.h = self.hLocal.data_ptr()
+        self._fbuf.rain = self.rainVal.data_ptr()
+        self._fbuf_keep = (self.hLocal, self.rainVal)
+        return self._fbuf
+
+    def _flowAccumulation_native(self):
+        t0 = time.perf_counter()
+        fb = self._flow_buffers()
+        fp = _lib.FlowParams(nrcv=self.flowDir, flowexp=self.flowExp, sealevel=self.sealevel,
+                             floor_level=self.sealevel + self.oFill, add_eps=0 if self.flatModel else 1,
+                             dt=self.dt, maxarea=float(self.maxarea[0]), max_passes=self._cascade_max_steps,
+                             patience=self._cascade_patience, rel_improve=self._cascade_rel_improve)
+        res = _lib.FlowResult()
+        self._call("gospl_flow_accumulation", self.H, ctypes.byref(fp), ctypes.byref(fb), ctypes.byref(res),
+                   self.stream)
+        Model.sweeps["flow"] += int(res.flow_solves)
+        n1 = int(res.npits) + 1
+        self.npits = n1 - 1
+        ptr = ctypes.c_void_p()
+        rows = ctypes.c_int64(0)
+        self._call("gospl_flow_tables", self.H, ctypes.byref(ptr), ctypes.byref(rows))
+        packed = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_double)), shape=(14 * n1,))
+        spill = packed[:n1].astype(np.int64)
+        self.pitParams = packed[n1:4 * n1].reshape(3, n1).T.copy()
+        self.filled_lvl = packed[4 * n1:9 * n1].reshape(5, n1).T.copy()
+        self.filled_vol = packed[9 * n1:14 * n1].reshape(5, n1).T.copy()
+        self.pitInfo = np.zeros((n1, 2), dtype=np.int64)
+        self.pitInfo[:, 0] = spill
+        self.pitInfo[:, 1] = np.where(spill >= 0, 0, -1)
+        self.rcvID, self.distRcv, self.wghtVal, self.lsink = self._rcv_sets[int(res.last_set)]
+        self.cascade_passes = int(res.cascade_passes)
+        self.undrained = int(res.undrained)
+        self.fa_levels = 0
+        self.waterFilled = self._water
+        self.timers["flow"] = time.perf_counter() - t0
+
+    def _flowAccumulation_host(self):
         t0 = time.perf_counter()
         self.fillElevation(sed=False)
         pitVol = self.pitParams[:, 0].copy()

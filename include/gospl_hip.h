@@ -334,6 +334,85 @@ int gospl_flow_finalize(gospl_mesh_t* mesh, const double* hl, const double* wate
                         double* fillfal, gospl_stream_t stream);
 
 /*
+ * FAMesh.flowAccumulation (gospl/flow/flowplex.py:752-945) in one call: the
+ * fill and pit information of fillElevation (pitfilling.py:849-903), the
+ * step's receivers (the main set, flowplex.py:786-791), the runoff source, the
+ * flow solve, the pit-overflow cascade with goSPL's decisions taken on the
+ * host in native code (flowplex.py:576-686, 849-904: the same decisions as a
+ * Python driver over the entry points above, bit for bit, including numpy's
+ * pairwise summation of the spilled volumes), and the end of the call
+ * (flowplex.py:906-933).  No counterpart in goSPL, whose driver is Python.
+ * Buffers: every pointer of gospl_flow_buffers is a DEVICE array of N
+ * (per-vertex) or `cap` (per-pit) rows; the handle keeps the host tables and
+ * staging it needs (grown on demand), and gospl_flow_tables returns the
+ * per-pit tables of the last call (HOST, gospl_pit_tables' layout, n1 =
+ * npits + 1 rows; valid until the next call).  Results: water = waterFilled - h (goSPL's waterFilled after the
+ * call), the main set in rcv0/dist0/wgt0/lsink0, the cascade's last rebuild
+ * in set 1; res->last_set says which set goSPL's live arrays (rcvID, lsink)
+ * name.  Returns 2 when the main flow solve meets an undrained receiver cycle
+ * (goSPL raises there); a cascade solve that does not converge is zeroed, as
+ * goSPL does.  Synchronises `stream` (the pit count, the tables, each pass's
+ * inflows).
+ */
+typedef struct {
+  int32_t nrcv;
+  double flowexp;          /* 1.1 (inputparser.py:194) */
+  double sealevel;
+  double floor_level;      /* sealevel + oFill */
+  int32_t add_eps;         /* 1 on a sphere (fillElevation's +1e-3) */
+  double dt;
+  double maxarea;          /* the largest cell area */
+  int32_t max_passes;      /* 100 */
+  int32_t patience;        /* 3 */
+  double rel_improve;      /* 1e-3 */
+} gospl_flow_params;
+typedef struct {
+  const double* h;
+  const double* rain;
+  double* lfill;
+  int32_t* pitids;
+  int32_t* flatdirs;
+  int32_t* ptdir;
+  int8_t* lspill;
+  int32_t* rcv0;
+  double* dist0;
+  double* wgt0;
+  int8_t* lsink0;
+  int32_t* rcv1;
+  double* dist1;
+  double* wgt1;
+  int8_t* lsink1;
+  double* b;
+  double* fal;
+  double* fillfal;
+  double* water;
+  double* hl;
+  double* q;
+  double* nfa;
+  double* rhs;
+  double* fa;
+  int64_t cap;
+  int32_t* spill;
+  double* pitparams;
+  double* flvl;
+  double* fvol;
+  double* inv;
+} gospl_flow_buffers;
+typedef struct {
+  int64_t npits;
+  int32_t cascade_passes;
+  int32_t flow_solves;
+  int64_t undrained;
+  int32_t last_set;
+} gospl_flow_result;
+int gospl_flow_accumulation(gospl_mesh_t* mesh, const gospl_flow_params* params,
+                            const gospl_flow_buffers* buffers, gospl_flow_result* result,
+                            gospl_stream_t stream);
+int gospl_flow_tables(gospl_mesh_t* mesh, const double** tables, int64_t* n1);
+/* numpy's pairwise summation of n doubles (HOST), the order np.sum uses. */
+double gospl_np_sum(const double* a, int64_t n);
+
+/*
  * gospl_flow_finalize for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops): no mesh handle, and no hipMalloc, hipFree or
  * allocating library call.  No counterpart in goSPL.  Same inputs, outputs and

@@ -23,6 +23,7 @@
 // on the schedule (no floating-point atomics).
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include <rocprim/device/device_scan.hpp>
@@ -1278,6 +1279,101 @@ __global__ void k_spl_update(int64_t N, const double* __restrict__ x, const int8
 }
 
 // ---------------------------------------------------------------------------
+// Non-linear stream power (n != 1, detachment-limited; nlSPL._solveNL,
+// gospl/eroder/nlSPL.py:321-396).  F(x)_i = x_i - hOld_i + Kbr_i S_i(x)^n
+// with S_i = sum over receivers with w > 0 and x_i > x_r of w (x_i - x_r)/d
+// (local_spl, functions.F90:1469-1502) is lower-triangular in the receiver
+// order, and so is its Jacobian (local_spl_coeff, :1504-1542): each Newton
+// step J d = -F is one exact erosion-direction sweep.
+// ---------------------------------------------------------------------------
+
+// Kbr = K kfac dt FAL^m elimiter, 0 at sea, with elimiter = dh / (dh + 0.01)
+// of the largest drop to a receiver dh (0 when dh = 0), nlSPL.py:330-346.
+__global__ void k_nl_kbr(int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ h,
+                         const double* __restrict__ fal, const double* __restrict__ kfac, double K, double m,
+                         double dt, double sl, const double* __restrict__ lfill, double* __restrict__ kbr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double hi = h[i];
+  double dh = -INFINITY;
+  for (int k = 0; k < NR; ++k) {
+    const double d = hi - h[rcv[(int64_t)k * N + i]];
+    dh = d > dh ? d : dh;
+  }
+  const double el = dh != 0.0 ? dh / (dh + 1.0e-2) : 0.0;
+  double kb = K * 1.0;
+  if (kfac) kb = kb * kfac[i];
+  kb = kb * ((dt * pow(fal[i], m)) * el);
+  kbr[i] = lfill[i] <= sl ? 0.0 : kb;
+}
+
+// One Newton linearisation at x: off/diag of J and rhs = -F.
+__global__ void k_nl_jac(int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ wgt,
+                         const double* __restrict__ dist, const double* __restrict__ x,
+                         const double* __restrict__ hold, const double* __restrict__ kbr, double n,
+                         double* __restrict__ off, double* __restrict__ diag, double* __restrict__ rhs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double xi = x[i];
+  double S = 0.0, c0 = 0.0;
+  double ck[GOSPL_MAXW];
+  for (int k = 0; k < NR; ++k) {
+    ck[k] = 0.0;
+    const double w = wgt[(int64_t)k * N + i];
+    if (w > 0.0) {
+      const int r = rcv[(int64_t)k * N + i];
+      const double xr = x[r];
+      if (xi > xr) {
+        const double d = dist[(int64_t)k * N + i];
+        S = S + w * (xi - xr) / d;
+        ck[k] = -w / d;
+        c0 = c0 + w / d;
+      }
+    }
+  }
+  const double kb = kbr[i];
+  const double F = (xi - hold[i]) + kb * (S > 0.0 ? pow(S, n) : 0.0);
+  const double dd = S > 0.0 ? n * kb * pow(S, n - 1.0) : 0.0;
+  for (int k = 0; k < NR; ++k) off[(int64_t)k * N + i] = dd * ck[k];
+  diag[i] = 1.0 + dd * c0;
+  rhs[i] = -F;
+}
+
+// x += d; keys of max |d| and max |x| (one atomic per workgroup).
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_nl_step(int64_t N, const double* __restrict__ d,
+                                                        double* __restrict__ x,
+                                                        unsigned long long* __restrict__ keys) {
+  __shared__ unsigned long long sh[2][GOSPL_BLOCK / 64];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long a = 0ull, b = 0ull;
+  if (i < N) {
+    const double dv = d[i];
+    const double xv = x[i] + dv;
+    x[i] = xv;
+    a = dkey(fabs(dv));
+    b = dkey(fabs(xv));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long ca = __shfl_xor(a, o), cb = __shfl_xor(b, o);
+    a = ca > a ? ca : a;
+    b = cb > b ? cb : b;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      a = sh[0][w] > a ? sh[0][w] : a;
+      b = sh[1][w] > b ? sh[1][w] : b;
+    }
+    atomicMax(keys, a);
+    atomicMax(keys + 1, b);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoints of the receiver-DAG solves (gradients with receivers, weights,
 // discharge and sea/sink flags held at their forward values).
 // ---------------------------------------------------------------------------
@@ -1956,6 +2052,63 @@ int gospl_dag_get_shape(gospl_mesh_t* m, int32_t* out) {
   out[2] = m->chain_pollers;
   out[3] = m->chain_resident[0] < m->chain_resident[1] ? m->chain_resident[0] : m->chain_resident[1];
   out[4] = m->num_cu;
+  return 0;
+}
+
+int gospl_spl_nl(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti, const double* disti,
+                 const double* fal, const double* kfac, double K, double mexp, double nexp, double dt,
+                 double sealevel, const double* lfill, const int8_t* lsink, double* h, double* cumed, double* eb,
+                 double* eblocal, double tol, int32_t maxit, int32_t* iters, double* step_rel,
+                 gospl_stream_t stream) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_spl_nl: nrcv must be in 1..12");
+    return 1;
+  }
+  if (!(nexp > 0.0)) {
+    set_error("gospl_spl_nl: n must be positive");
+    return 1;
+  }
+  // coefficient rows: the Jacobian's nrcv off-diagonal rows, then Kbr
+  if (gospl_ensure_coef(m, nrcv + 1)) return 1;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  double* off = m->wcoef;
+  double* kbr = m->wcoef + (int64_t)nrcv * (N + 64);
+  double* diag = m->wd[3];
+  double* rhs = m->wd[2];
+  double* d = m->wd[0];
+  double* x = m->wd[1];  // the iterate (the sweeps' cycle scratch is not used in erosion mode)
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 56);
+  k_nl_kbr<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, h, fal, kfac, K, mexp, dt, sealevel, lfill, kbr);
+  GOSPL_CHECK(hipMemcpyAsync(x, h, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+  GOSPL_CHECK(hipGetLastError());
+  int it = 0;
+  double rel = INFINITY;
+  while (it < maxit) {
+    k_nl_jac<<<g, GOSPL_BLOCK, 0, stream>>>(N, nrcv, rcvi, wgti, disti, x, h, kbr, nexp, off, diag, rhs);
+    GOSPL_CHECK(hipGetLastError());
+    if (dag_solve(m, 1, nrcv, rcvi, off, diag, rhs, d, nullptr, nullptr, stream)) return 1;
+    GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, stream));
+    k_nl_step<<<g, GOSPL_BLOCK, 0, stream>>>(N, d, x, keys);
+    GOSPL_CHECK(hipGetLastError());
+    unsigned long long hk[2];
+    GOSPL_CHECK(hipMemcpyAsync(m->hflag + 56, keys, sizeof(hk), hipMemcpyDeviceToHost, stream));
+    GOSPL_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(hk, m->hflag + 56, sizeof(hk));
+    ++it;
+    const double dmax = dkey_inv_host(hk[0]), xmax = dkey_inv_host(hk[1]);
+    rel = dmax / (xmax > 1.0 ? xmax : 1.0);
+    if (!(rel == rel)) break;  // NaN
+    if (rel <= tol) break;
+  }
+  if (iters) *iters = it;
+  if (step_rel) *step_rel = rel;
+  if (!(rel <= tol)) {
+    set_error("gospl_spl_nl: Newton did not converge in %d iterations (last step %.3g of max|h|)", it, rel);
+    return 1;
+  }
+  k_spl_update<<<g, GOSPL_BLOCK, 0, stream>>>(N, x, m->outlet, m->flat, lsink, dt, h, cumed, eb, eblocal);
+  GOSPL_CHECK(hipGetLastError());
   return 0;
 }
 

@@ -24,9 +24,9 @@ per-step surface-process path uses, with goSPL's defaults
 
 Everything else in goSPL's input describes subsystems outside this path and
 is ignored.  Settings that would change what this path computes and that it
-does not implement raise ``NotImplementedError`` (non-linear stream power
-n != 1, transport-limited G > 0, a sea-level curve, open or cyclic planar
-edges); sediment routing (``nodep: False``, goSPL's default) is not on this
+does not implement raise ``NotImplementedError`` (transport-limited G > 0, a
+sea-level curve, open or cyclic planar edges); n != 1 selects the non-linear
+stream-power solver, as goSPL's runProcesses does; sediment routing (``nodep: False``, goSPL's default) is not on this
 path and is reported with a warning.  Relative file names are resolved
 against the YAML file's folder, then the working directory, as goSPL opens
 them from the working directory.
@@ -111,8 +111,8 @@ def read_input(filename):
             raise KeyError("Key K is required in the spl declaration")
         K, m, n, d, G = (float(spl["K"]), float(spl.get("m", 0.5)), float(spl.get("n", 1.0)),
                          float(spl.get("d", 0.0)), float(spl.get("G", 0.0)))
-    if n != 1.0:
-        raise NotImplementedError("spl.n = %g: non-linear stream power is not on this path (n = 1)" % n)
+    if not n > 0.0:
+        raise ValueError("spl.n = %g: the slope exponent must be positive" % n)
     if G != 0.0:
         raise NotImplementedError("spl.G = %g: transport-limited erosion is not on this path (G = 0)" % G)
     hill = inp.get("diffusion") or {}
@@ -149,7 +149,7 @@ def read_input(filename):
     return dict(name=inp.get("name"), coords=coords, cells=cells, elev=elev, bc=bc,
                 flowdir=int(dom.get("flowdir", 8)), tstart=float(tdict["start"]), tend=float(tdict["end"]),
                 dt=float(tdict["dt"]), tout=float(tdict.get("tout", float(tdict["end"]) - float(tdict["start"]))),
-                K=K, m=m, d=d, cda=cda, cdm=cdm, ofill=ofill, sealevel=sealevel, climate=climate,
+                K=K, m=m, d=d, n=n, cda=cda, cdm=cdm, ofill=ofill, sealevel=sealevel, climate=climate,
                 tectonics=tecto, sedfactor=sedfac)
 
 

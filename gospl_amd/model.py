@@ -128,7 +128,7 @@ class Model:
 
     sweeps = {"flow": 0, "spl": 0}
 
-    def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
+    def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, n=1.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
                  tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
                  refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None):
@@ -157,6 +157,9 @@ class Model:
         self.lpoints = N
         self.flatModel = self.dm.flat
         self.K, self.spl_m, self.coeffd = float(K), float(m), float(d)
+        self.spl_n = float(n)
+        self.nl_tol, self.nl_maxit = 1.0e-13, 100
+        self.nl_iters = 0
         self.flowDir, self.flowExp = int(flowdir), 1.1
         self.Cda, self.Cdm = float(cda), float(cdm)
         self.dt = float(dt)
@@ -256,7 +259,8 @@ class Model:
                      tectonics=[(t0, t1, dev(u) if u is not None else None) for t0, t1, u in inp["tectonics"]],
                      sedfactor=[(t, dev(v)) for t, v in inp["sedfactor"]])
         first = lambda evs, t, dflt: next((v for s, v in reversed(evs) if s <= t), dflt)
-        params = dict(K=inp["K"], m=inp["m"], d=inp["d"], flowdir=inp["flowdir"], cda=inp["cda"], cdm=inp["cdm"],
+        params = dict(K=inp["K"], m=inp["m"], d=inp["d"], n=inp["n"], flowdir=inp["flowdir"], cda=inp["cda"],
+                      cdm=inp["cdm"],
                       dt=inp["dt"], rain=first(sched["climate"], inp["tstart"], 0.0),
                       sedfac=first(sched["sedfactor"], inp["tstart"], None), sealevel=inp["sealevel"],
                       ofill=inp["ofill"], tstart=inp["tstart"], tend=inp["tend"], schedule=sched)
@@ -625,7 +629,24 @@ class Model:
         self.timers["flow"] = time.perf_counter() - t0
 
     # -------------------------------------------------------------- erosion
+    def erodepSPLnl(self):
+        """nlSPL.erodepSPLnl (gospl/eroder/nlSPL.py:321-456), G = 0: Newton
+        steps of exact erosion-direction sweeps to max|step| <= 1e-13
+        max|h| (goSPL's SNES stops at 1e-6)."""
+        it = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        self._call("gospl_spl_nl", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
+                   _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.spl_n, self.dt,
+                   self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),
+                   _ptr(self.Eb), _ptr(self.EbLocal), self.nl_tol, self.nl_maxit, ctypes.byref(it),
+                   ctypes.byref(rel), self.stream)
+        Model.sweeps["spl"] += int(it.value)
+        self.nl_iters = int(it.value)
+
     def erodepSPL(self):
+        if self.spl_n != 1.0:
+            # goSPL's runProcesses takes the non-linear solver for n != 1 (model.py:396-401)
+            return self.erodepSPLnl()
         lev = ctypes.c_int32(0)
         if self.refine_dag:
             npass = ctypes.c_int32(0)

@@ -178,6 +178,26 @@ int gospl_spl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const doubl
               gospl_stream_t stream);
 
 /*
+ * Non-linear stream power, detachment-limited (n != 1, G = 0): replaces
+ * nlSPL._solveNL and the update of erodepSPLnl (gospl/eroder/nlSPL.py:321-396,
+ * 398-456) with local_spl / local_spl_coeff (functions.F90:1469-1542).
+ * F(x)_i = x_i - h_i + Kbr_i S_i(x)^n = 0 with S_i the weighted positive slope to
+ * the receivers (w > 0) and Kbr = K kfac dt FAL^m elimiter (0 where lfill <=
+ * sealevel; elimiter = dh/(dh + 0.01) of the largest drop dh to a receiver).
+ * The system is lower-triangular in the receiver order, so each Newton step
+ * is one exact sweep; Newton runs until max|step| <= tol max(max|x|, 1)
+ * (goSPL's SNES stops at rtol/atol 1e-6), at most maxit steps, failing
+ * otherwise.  Then the state update of gospl_spl.  *iters / *step_rel (HOST,
+ * may be NULL) receive the Newton steps and the last relative step.
+ * Synchronises `stream` once per step.
+ */
+int gospl_spl_nl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                 const double* disti, const double* fal, const double* kfac, double K, double m, double n,
+                 double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+                 double* cumed, double* eb, double* eblocal, double tol, int32_t maxit, int32_t* iters,
+                 double* step_rel, gospl_stream_t stream);
+
+/*
  * Refined solves (the reduced-cost mode; no counterpart in goSPL, which
  * solves these systems with FGMRES + ILU(0) to rtol 1e-8, flowplex.py:341-408).
  * Iterative refinement: each pass takes the FP64 residual r = rhs - A x of the

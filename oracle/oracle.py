@@ -13,6 +13,7 @@ mesh order:
 * ``flowAccumulation``         gospl/flow/flowplex.py:752-945
 * ``_distributeDownstream``    gospl/flow/flowplex.py:576-686
 * ``erodepSPL``                gospl/eroder/SPL.py:45-114, 260-366, 402-454
+* ``erodepSPLnl`` (n != 1)     gospl/eroder/nlSPL.py:321-456
 * ``getHillslope``             gospl/sed/hillslope.py:90-293, 614-649
 * ``applyForces``              gospl/mesher/unstructuredmesh.py:743-794, 903-972
 * ``runProcesses``             gospl/model.py:331-466
@@ -200,7 +201,7 @@ class OracleModel:
 
     def __init__(self, geom, h, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0,
-                 tstart=0.0, tend=None):
+                 tstart=0.0, tend=None, n=1.0):
         self.ngb = _c(geom["ngb"], np.int32)
         self.nnb = _c(geom["nnb"], np.int32)
         self.elgt = _c(geom["elgt"], np.float64)
@@ -221,6 +222,7 @@ class OracleModel:
         self.inIDs = np.ones(N, dtype=int)
         self.maxarea = np.array([self.larea.max()])
         self.K, self.spl_m, self.coeffd = float(K), float(m), float(d)
+        self.spl_n = float(n)
         self.flowDir, self.flowExp = int(flowdir), 1.1
         self.Cda, self.Cdm = float(cda), float(cdm)
         self.dt = float(dt)
@@ -550,6 +552,74 @@ class OracleModel:
         self.h = self.h + self.tmp
         self.EbLocal = self.tmp / self.dt
 
+    # ---------------- non-linear erosion (nlSPL.py) ----------------
+    def erodepSPLnl(self):
+        """nlSPL._solveNL + erodepSPLnl (gospl/eroder/nlSPL.py:321-456) for
+        G = 0.  goSPL solves F(x) = x - hOld + Kbr S(x)^n = 0 with SNES
+        (rtol/atol 1e-6); S_i = sum over receivers with w > 0 and x_i > x_r of
+        w (x_i - x_r) / d (local_spl, functions.F90:1469-1502).  F_i depends on
+        x_i and its receivers only, so here each vertex is solved exactly in
+        order of increasing hOld (a receiver with w > 0 is lower): F_i is
+        continuous and increasing in x_i, with a root in [min(hOld_i, min x_r),
+        hOld_i], found by safeguarded Newton to the last bits."""
+        hOld = self.h.copy()
+        PA = self.FAL
+        dh = (hOld[:, None] - hOld[self.rcvIDi]).max(axis=1)
+        elim = np.divide(dh, dh + 1.0e-2, out=np.zeros_like(dh), where=dh != 0)
+        if self.sedfacVal is not None:
+            Kbr = self.K * 1.0 * self.sedfacVal * (self.rainVal ** self.coeffd)
+        else:
+            Kbr = self.K * 1.0 * (self.rainVal ** self.coeffd)
+        Kbr = Kbr * (self.dt * (PA ** self.spl_m) * elim)
+        Kbr[self.seaID] = 0.0
+        n = self.spl_n
+        x = hOld.copy()
+        rcv, wgt, dist = self.rcvIDi, self.wghtVali, self.distRcvi
+        for i in np.argsort(hOld, kind="stable"):
+            kb = Kbr[i]
+            if kb == 0.0:
+                continue
+            ks = [k for k in range(rcv.shape[1]) if wgt[i, k] > 0.0]
+            if not ks:
+                continue
+            xr = np.array([x[rcv[i, k]] for k in ks])
+            wd = np.array([wgt[i, k] / dist[i, k] for k in ks])
+
+            def f(y):
+                on = y > xr
+                S = float(np.sum(wd[on] * (y - xr[on])))
+                return y - hOld[i] + kb * (S ** n if S > 0.0 else 0.0), S, float(np.sum(wd[on]))
+
+            lo, hi = min(hOld[i], xr.min()), hOld[i]
+            y = hi
+            for _ in range(400):
+                fy, S, c = f(y)
+                if fy == 0.0:
+                    break
+                if fy > 0.0:
+                    hi = y
+                else:
+                    lo = y
+                dfy = 1.0 + (n * kb * S ** (n - 1.0) * c if S > 0.0 else 0.0)
+                yn = y - fy / dfy
+                if not (lo < yn < hi):
+                    yn = 0.5 * (lo + hi)
+                if abs(yn - y) <= 1.0e-15 * max(abs(y), 1.0) or hi - lo <= 1.0e-15 * max(abs(y), 1.0):
+                    y = yn
+                    break
+                y = yn
+            x[i] = y
+        self.nl_x = x
+        E = (x - hOld) / self.dt
+        if self.flatModel:
+            E[self.outletIDs] = 0.0
+        E[self.lsink] = 0.0
+        self.Eb = E
+        self.tmp = E * self.dt
+        self.cumED = self.cumED + self.tmp
+        self.h = self.h + self.tmp
+        self.EbLocal = self.tmp / self.dt
+
     # ---------------- hillslope (hillslope.py) ----------------
     def getHillslope(self):
         h = self.h.copy()
@@ -577,7 +647,10 @@ class OracleModel:
     def step(self):
         """One iteration of the runProcesses loop body (model.py:379-450)."""
         self.flowAccumulation()
-        self.erodepSPL()
+        if self.spl_n == 1.0:
+            self.erodepSPL()
+        else:
+            self.erodepSPLnl()
         self.getHillslope()
         if self.tNow < self.tEnd:
             self.applyForces()

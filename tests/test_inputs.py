@@ -80,7 +80,7 @@ def test_nodep_default_warns(tmp_path, extra, err):
 
 
 @pytest.mark.parametrize("old,new,exc", [
-    ("  d: 0.5\n", "  d: 0.5\n  n: 2.\n", NotImplementedError),
+    ("  d: 0.5\n", "  d: 0.5\n  n: -1.\n", ValueError),
     ("  d: 0.5\n", "  d: 0.5\n  G: 1.\n", NotImplementedError),
     ("  position: -10.\n", "  position: -10.\n  curve: 'sea.csv'\n", NotImplementedError),
     ("  K: 1.e-5\n", "  m: 0.4\n", KeyError),
@@ -94,6 +94,12 @@ def test_unsupported_or_missing_settings_raise(tmp_path, old, new, exc):
     fn.write_text(text.replace(old, new, 1))
     with pytest.raises(exc):
         inputs.read_input(str(fn))
+
+
+def test_slope_exponent_is_read(tmp_path):
+    fn, *_ = _write_case(tmp_path)
+    fn.write_text(fn.read_text().replace("  d: 0.5\n", "  d: 0.5\n  n: 1.5\n", 1))
+    assert inputs.read_input(str(fn))["n"] == 1.5
 
 
 def test_border_outlets_follow_bc(tmp_path):

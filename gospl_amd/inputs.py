@@ -9,7 +9,10 @@ per-step surface-process path uses, with goSPL's defaults
   the triangles and the initial elevation; ``flowdir`` (default 8); ``bc``
   (default ``oooo``; [North, East, South, West] edges of a planar mesh);
   ``nodep`` (default False);
-* ``time`` (`_readTime`, :343-452): ``start``, ``end``, ``dt``, ``tout``;
+* ``time`` (`_readTime`, :343-452): ``start``, ``end``, ``dt``, ``tout``,
+  ``rstep`` (restart from that output);
+* ``output`` (`_readOut`, :2095-2118): ``dir`` ("output"), ``makedir``
+  (False), ``compression`` ("gzip");
 * ``spl`` (`_readSPL`, :454-496): ``K``, ``m`` (0.5), ``n`` (1), ``d`` (0),
   ``G`` (0);
 * ``diffusion`` (`_readHillslope`, :498-599): ``hillslopeKa``,
@@ -146,7 +149,10 @@ def read_input(filename):
             sedfac.append((float(ev["start"]), _npz_field(ev["map"], base, "sedfactor map")))
         elif ev.get("uniform") is not None:
             sedfac.append((float(ev["start"]), float(ev["uniform"])))
+    out = inp.get("output") or {}
     return dict(name=inp.get("name"), coords=coords, cells=cells, elev=elev, bc=bc,
+                outputDir=str(out.get("dir", "output")), makedir=bool(out.get("makedir", False)),
+                compression=out.get("compression", "gzip"), rstep=int(tdict.get("rstep", 0)),
                 flowdir=int(dom.get("flowdir", 8)), tstart=float(tdict["start"]), tend=float(tdict["end"]),
                 dt=float(tdict["dt"]), tout=float(tdict.get("tout", float(tdict["end"]) - float(tdict["start"]))),
                 K=K, m=m, d=d, n=n, cda=cda, cdm=cdm, ofill=ofill, sealevel=sealevel, climate=climate,

@@ -318,5 +318,5 @@ def build_mesh(coords, cells, lib, flat=None, order="morton", outlet_mask=None) 
         outlet=outlet[perm].copy(),
         maxarea=float(area.max()),
         extras=dict(ngb_input=ngb, nnb_input=nnb, elgt_input=elgt, vdist_input=vdist,
-                    area_input=area),
+                    area_input=area, cells=cells),
     )

@@ -31,6 +31,7 @@ import torch
 
 from . import _lib
 from .mesh import Mesh
+from .outmesh import WriteMesh
 
 MISSING_DATA_SENTINEL = -1.0e8   # gospl/tools/constants.py:34
 
@@ -101,7 +102,7 @@ class DeviceMesh:
             pass
 
 
-class Model:
+class Model(WriteMesh):
     """Per-step hot path of goSPL on one GPU (FP64 state).
 
     ``Model.sweeps`` counts, process-wide, the exact sweeps launched by every
@@ -131,7 +132,8 @@ class Model:
     def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, n=1.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
                  tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
-                 refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None):
+                 refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None,
+                 outputDir=None, makedir=False, tout=None, rstep=0, compression="gzip", finput=None):
         if isinstance(mesh, (str, os.PathLike)):
             # goSPL's constructor, Model(filename, verbose, showlog, profile)
             # (gospl/model.py:212-304): the YAML file and its npz mesh
@@ -241,6 +243,9 @@ class Model:
         self._sched_state = {}
         self.native_flow = True
         self._fbuf = None
+        self.vSedLocal = torch.zeros(N, **f64)
+        self._init_output(outputDir=outputDir, makedir=makedir, tout=tout, rstep=rstep, compression=compression,
+                          finput=finput)
 
     @staticmethod
     def _from_file(filename, device):
@@ -263,7 +268,9 @@ class Model:
                       cdm=inp["cdm"],
                       dt=inp["dt"], rain=first(sched["climate"], inp["tstart"], 0.0),
                       sedfac=first(sched["sedfactor"], inp["tstart"], None), sealevel=inp["sealevel"],
-                      ofill=inp["ofill"], tstart=inp["tstart"], tend=inp["tend"], schedule=sched)
+                      ofill=inp["ofill"], tstart=inp["tstart"], tend=inp["tend"], schedule=sched,
+                      outputDir=inp["outputDir"], makedir=inp["makedir"], tout=inp["tout"], rstep=inp["rstep"],
+                      compression=inp["compression"], finput=os.path.abspath(filename))
         return (m, m.to_device_order(inp["elev"])), params
 
     def _apply_schedule(self):
@@ -668,6 +675,9 @@ class Model:
 
     # -------------------------------------------------------------- hillslope
     def getHillslope(self):
+        if self.outputDir is not None:
+            # goSPL's seaID as the outputs read it (hillslope.py:620)
+            self._seaOut = self.hLocal <= self.sealevel
         if self.Cda == 0.0 and self.Cdm == 0.0:
             # goSPL then adds the erosion step's thickness again (hillslope.py:638-643)
             self.EbLocal += self.Eb * self.dt / self.dt
@@ -697,8 +707,11 @@ class Model:
             self.hLocal += self.upsub * self.dt
 
     def initialise(self):
-        """model.py:284-295: forcing, then the init-time flow accumulation."""
+        """model.py:284-295: forcing, a restart's state read back (readData),
+        then the init-time flow accumulation."""
         self.applyForces()
+        if self.rStep > 0:
+            self.readData()
         self.flowAccumulation()
 
     def step(self):
@@ -711,7 +724,9 @@ class Model:
         self.tNow += self.dt
 
     def runProcesses(self):
+        """model.py:347-453: an output when one is due (visModel), the step."""
         while self.tNow <= self.tEnd:
+            self.visModel()
             if self.tNow == self.tEnd:
                 break
             self.step()

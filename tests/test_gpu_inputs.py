@@ -12,8 +12,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_yaml_model_matches_kwargs_model(tmp_path):
+def test_yaml_model_matches_kwargs_model(tmp_path, monkeypatch):
     import torch
+
+    monkeypatch.chdir(tmp_path)   # goSPL writes its outputs under the working directory
 
     from gospl_amd import _lib, workloads
     from gospl_amd.model import Model
@@ -71,8 +73,10 @@ def test_yaml_model_matches_kwargs_model(tmp_path):
     gk.close()
 
 
-def test_forcing_events_switch(tmp_path):
+def test_forcing_events_switch(tmp_path, monkeypatch):
     import torch
+
+    monkeypatch.chdir(tmp_path)
 
     from gospl_amd import _lib, mesh as gmesh
     from gospl_amd.model import Model

@@ -88,6 +88,20 @@ SIGNATURES = {
                                       ctypes.POINTER(FlowResult), P]),
     "gospl_flow_tables": (I32, [P, ctypes.POINTER(P), ctypes.POINTER(I64)]),
     "gospl_np_sum": (F64, [P, I64]),
+    "gospl_pit_info_sed": (I32, [P, P, P, F64, F64, P, P, P, P, I64, P, P, P, P, ctypes.POINTER(I64), P]),
+    "gospl_sed_source": (I32, [P, P, P, P]),
+    "gospl_sed_stray": (I32, [P, P, P, I32, P, P, P]),
+    "gospl_sed_add": (I32, [P, P, P, P, F64, P, P, P]),
+    "gospl_sed_closed": (I32, [P, P, P, P, P]),
+    "gospl_sed_bottomup": (I32, [P, P, P, P, P, P, P]),
+    "gospl_sed_tilt": (I32, [P, P, P, I64, P, P, P, P, P]),
+    "gospl_sed_inlets": (I32, [P, I32, P, P, I64, P, P, P, P]),
+    "gospl_sed_lake_depth": (I32, [P, P, I64, P, P, P, P, P, P, P]),
+    "gospl_sed_lake_init": (I32, [P, P, P, P, P, P, P, P, P, P, P]),
+    "gospl_sed_pit_volume": (I32, [P, P, P, P, I64, P, P]),
+    "gospl_sed_scale": (I32, [P, P, P, P, P, P]),
+    "gospl_sed_apply": (I32, [P, P, P, P, P]),
+    "gospl_nl_diffusion": (I32, [P, P, P, F64, F64, F64, F64, I32, I32, P, P, ctypes.POINTER(I64), P]),
     "gospl_sweep_timing": (I32, [P, I32, ctypes.POINTER(F64), ctypes.POINTER(I64),
                                  ctypes.POINTER(F64), ctypes.POINTER(I64)]),
 }

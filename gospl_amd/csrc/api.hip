@@ -453,6 +453,9 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->hcoef32);
   dfree(m->adj_hcoef);
   dfree(m->adj_hdiag);
+  dfree(m->nld_buf);
+  dfree(m->sed_mask);
+  dfree(m->sed_buf);
   dfree(m->seg_keys);
   dfree(m->seg_ctl);
   dfree(m->seg_aux);

@@ -104,6 +104,9 @@ struct gospl_mesh {
   double* rx_hctl;            // its pinned host mirror
   int rx_part_cap;            // partial-sum slots available in wd[0]
   size_t cub_tmp_bytes;
+  double* nld_buf;            // non-linear diffusion: rows (W, N), diagonal and four N vectors
+  int8_t* sed_mask;           // sediment routing: (N) int8 cell flags
+  double* sed_buf;            // sediment routing: two N vectors
 };
 
 namespace gospl {

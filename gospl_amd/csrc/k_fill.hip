@@ -719,10 +719,19 @@ int gospl_fill_elevation_seg(gospl_mesh_t* m, const double* h, int32_t add_eps, 
   return fill_body(m, h, 0.0, FillLevel{m->seg_keys, add_eps ? 1 : 0, floor_level, seg}, sealevel, lfill, s);
 }
 
-int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double sealevel,
-                   int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill, int64_t cap,
-                   int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
-                   int64_t* npits_out, gospl_stream_t s) {
+}  // extern "C"
+
+namespace gospl {
+
+// Depression information with its three levels: pits are the equal-fill
+// components above `label_lvl` (label_pits' lvl), cells filled to `sealevel`
+// or below are restored, and the per-pit sums clip elevations below
+// `clip_lvl` to it (goSPL: sea level, sea level, sea level for water; the fill
+// level, sea level and no clip for sediment, pitfilling.py:718-721, 828-832).
+static int pit_info_body(gospl_mesh_t* m, const double* hl, double* lfill, double label_lvl, double sealevel,
+                         double clip_lvl, int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill,
+                         int64_t cap, int32_t* spill, double* pitparams, double* filled_lvl,
+                         double* filled_vol, int64_t* npits_out, gospl_stream_t s) {
   const int64_t N = m->N;
   const int g = nblocks(N);
   int32_t* parent = m->wi[3];
@@ -734,7 +743,7 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   int8_t* qual = reinterpret_cast<int8_t*>(m->wi[1]);
   int32_t* best = m->wi[2];
 
-  k_pit_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, sealevel, parent, noflow, cmin,
+  k_pit_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, label_lvl, parent, noflow, cmin,
                                         cnon);
   k_cc_hook<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, parent);
   k_cc_compress<<<g, GOSPL_BLOCK, 0, s>>>(N, parent, noflow, m->gid, m->outlet, m->flat, cmin, cnon);
@@ -796,7 +805,27 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
   // marine restore, then the pit parameters and depth-volume curves
   k_pit_marine<<<g, GOSPL_BLOCK, 0, s>>>(N, hl, lfill, flatdirs, sealevel);
   GOSPL_CHECK(hipGetLastError());
-  return pit_group_params(m, hl, lfill, sealevel, cap, pitparams, filled_lvl, filled_vol, s);
+  return pit_group_params(m, hl, lfill, clip_lvl, cap, pitparams, filled_lvl, filled_vol, s);
+}
+
+}  // namespace gospl
+
+extern "C" {
+
+int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double sealevel,
+                   int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill, int64_t cap,
+                   int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
+                   int64_t* npits_out, gospl_stream_t s) {
+  return pit_info_body(m, hl, lfill, sealevel, sealevel, sealevel, pitids, flatdirs, ptdir, lspill, cap, spill,
+                       pitparams, filled_lvl, filled_vol, npits_out, s);
+}
+
+int gospl_pit_info_sed(gospl_mesh_t* m, const double* hl, double* lfill, double level, double sealevel,
+                       int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill, int64_t cap,
+                       int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
+                       int64_t* npits_out, gospl_stream_t s) {
+  return pit_info_body(m, hl, lfill, level, sealevel, -INFINITY, pitids, flatdirs, ptdir, lspill, cap, spill,
+                       pitparams, filled_lvl, filled_vol, npits_out, s);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 // built from, and a call rebuilds only the rows where one of those flags
 // changed (goSPL keeps its operator until the coastline moves,
 // hillslope.py:184-200); new diffusivities or dt rebuild every row.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -839,5 +840,163 @@ extern "C" int gospl_hillslope_refine_seg(gospl_mesh_t* m, double cda, double cd
     GOSPL_CHECK(hipStreamSynchronize(s));
     for (int k = 0; k < nseg; ++k) passes[k] = (int32_t)hc[k].pass;
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Non-linear thickness diffusion dh/dt = div(Cd(h) grad h) over one step by
+// goSPL's lagged-diffusivity scheme (hillSLP._diffuseImplicitPicard,
+// gospl/sed/hillslope.py:925-1076): `nsub` backward-Euler sub-steps of dt /
+// nsub, each with `npic` Picard updates that freeze Cd at the current iterate
+// and solve (I + dt_sub L(Cd)) h' = h_start, L the finite-volume operator of
+// jacobiancoeff with a zero derivative term (functions.F90:1421-1467).
+// Cd = minDiff + Cd0 (1 - exp(-dexp dh)) on the masked cells (0 elsewhere),
+// dh = h - bed with thicknesses below 0.1 m taken as 0 (hillslope.py:
+// 1030-1034); a face carries flux only when both of its cells diffuse.  Each
+// linear system is solved to convergence by Jacobi sweeps from the current
+// iterate (the contraction bound rho = max (d - 1) / d of the rows: stop once
+// the change times rho / (1 - rho) is below 2^-52 of the largest value);
+// goSPL stops its Richardson solve at rtol 1e-8.
+// ---------------------------------------------------------------------------
+
+namespace gospl {
+
+__device__ __forceinline__ double nld_cd(int64_t i, const double* __restrict__ h, const double* __restrict__ zb,
+                                         const int8_t* __restrict__ mask, double cd0, double mind,
+                                         double dexp) {
+  if (!mask[i]) return 0.0;
+  double dh = h[i] - zb[i];
+  if (dh < 0.1) dh = 0.0;
+  return mind + cd0 * (1.0 - exp(-dexp * dh));
+}
+
+__global__ void k_nld_rows(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                           const double* __restrict__ area, const double* __restrict__ elgt,
+                           const double* __restrict__ vdist, const double* __restrict__ h,
+                           const double* __restrict__ zb, const int8_t* __restrict__ mask, double cd0,
+                           double mind, double dexp, double dts, double* __restrict__ coef,
+                           double* __restrict__ diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double s = 0.0;
+  const double a = area[i];
+  const double ck = nld_cd(i, h, zb, mask, cd0, mind, dexp);
+  const int dg = deg[i];
+  for (int p = 0; p < W; ++p) {
+    double off = 0.0;
+    if (a > 0.0 && p < dg) {
+      const int64_t e = (int64_t)p * N + i;
+      const double vd = vdist[e];
+      if (vd > 0.0) {
+        const int n = nbr[e];
+        const double cn = nld_cd(n, h, zb, mask, cd0, mind, dexp);
+        if (ck > 0.0 && cn > 0.0) {
+          const double c = 0.5 * (ck + cn) / a;
+          const double v = c * vd / elgt[e];
+          s = s + v;
+          off = dts * (-v);
+        }
+      }
+    }
+    coef[(int64_t)p * N + i] = off;
+  }
+  diag[i] = dts * s + 1.0;
+}
+
+__global__ void k_nld_init(int64_t N, const double* __restrict__ dh, const double* __restrict__ zb,
+                           double* __restrict__ h) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  h[i] = dh[i] + zb[i];
+}
+
+__global__ void k_nld_out(int64_t N, const double* __restrict__ h, const double* __restrict__ zb,
+                          double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double d = h[i] - zb[i];
+  out[i] = d < 0.0 ? 0.0 : d;
+}
+
+}  // namespace gospl
+
+extern "C" int gospl_nl_diffusion(gospl_mesh_t* m, const double* dh, const int8_t* mask, double cd0,
+                                  double min_diff, double dexp, double dt, int32_t nsub, int32_t npic,
+                                  const double* zb, double* ndepo, int64_t* sweeps, gospl_stream_t s) {
+  if (nsub < 1 || npic < 1 || !(dt > 0.0)) {
+    set_error("gospl_nl_diffusion: bad arguments (nsub=%d, npic=%d, dt=%g)", (int)nsub, (int)npic, dt);
+    return 1;
+  }
+  const int64_t N = m->N;
+  const int64_t ld = N + 64;
+  const int g = nblocks(N);
+  if (!m->nld_buf) GOSPL_CHECK(hipMalloc(&m->nld_buf, sizeof(double) * ld * (m->W + 5)));
+  double* coef = m->nld_buf;
+  double* diag = coef + ld * m->W;
+  double* h = diag + ld;       // current iterate
+  double* hs = h + ld;         // the sub-step's start (right-hand side)
+  double* xa = hs + ld;
+  double* xb = xa + ld;
+  // the coefficient kernel reads rows column-major with stride N
+  const double dts = dt / nsub;
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
+  unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
+  int64_t total = 0;
+  k_nld_init<<<g, GOSPL_BLOCK, 0, s>>>(N, dh, zb, h);
+  GOSPL_CHECK(hipGetLastError());
+  for (int sub = 0; sub < nsub; ++sub) {
+    GOSPL_CHECK(hipMemcpyAsync(hs, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    for (int it = 0; it < npic; ++it) {
+      k_nld_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, h, zb, mask, cd0,
+                                           min_diff, dexp, dts, coef, diag);
+      GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 3, s));
+      k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, keys + 2);
+      GOSPL_CHECK(hipGetLastError());
+      GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GOSPL_CHECK(hipStreamSynchronize(s));
+      const double rho = hk[2] ? dkey_inv_host(hk[2]) : 0.0;
+      const double factor = rho < 1.0 ? rho / (1.0 - rho) : 1.0e300;
+      // sweeps that bring the contraction bound below 2^-60 before the first
+      // measured sweep, then chunks of 32
+      int first = 2;
+      if (rho > 0.0) {
+        const double n = std::ceil(-60.0 * std::log(2.0) / std::log(rho));
+        first = n > 62.0 ? 62 : (n < 2.0 ? 2 : (int)n);
+      }
+      const double* xin = h;
+      double* xout = xa;
+      int64_t done = 0;
+      bool conv = false;
+      int chunk = first;
+      while (!conv && done < GOSPL_HILL_MAX_SWEEPS) {
+        for (int k = 0; k < chunk - 1; ++k) {
+          k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, xin, xout);
+          xin = xout;
+          xout = (xout == xa) ? xb : xa;
+        }
+        GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
+        k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, xin, xout, keys);
+        GOSPL_CHECK(hipGetLastError());
+        xin = xout;
+        xout = (xout == xa) ? xb : xa;
+        done += chunk;
+        GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+        GOSPL_CHECK(hipStreamSynchronize(s));
+        const double chg = dkey_inv_host(hk[0]), mag = dkey_inv_host(hk[1]);
+        if (!(chg == chg) || !(mag == mag)) break;
+        conv = chg * factor <= 0x1p-52 * mag;
+        chunk = 32;
+      }
+      if (!conv) {
+        set_error("gospl_nl_diffusion: Jacobi sweeps did not converge (contraction %.6g)", rho);
+        return 1;
+      }
+      total += done;
+      GOSPL_CHECK(hipMemcpyAsync(h, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  k_nld_out<<<g, GOSPL_BLOCK, 0, s>>>(N, h, zb, ndepo);
+  GOSPL_CHECK(hipGetLastError());
+  if (sweeps) *sweeps = total;
   return 0;
 }

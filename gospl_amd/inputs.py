@@ -8,7 +8,8 @@ per-step surface-process path uses, with goSPL's defaults
   elev]`` -- an npz (loaded without pickles) holding the vertex coordinates,
   the triangles and the initial elevation; ``flowdir`` (default 8); ``bc``
   (default ``oooo``; [North, East, South, West] edges of a planar mesh);
-  ``nodep`` (default False);
+  ``nodep`` (default False: continental sediment routing runs),
+  ``seadepo`` (default True);
 * ``time`` (`_readTime`, :343-452): ``start``, ``end``, ``dt``, ``tout``,
   ``rstep`` (restart from that output);
 * ``output`` (`_readOut`, :2095-2118): ``dir`` ("output"), ``makedir``
@@ -16,7 +17,10 @@ per-step surface-process path uses, with goSPL's defaults
 * ``spl`` (`_readSPL`, :454-496): ``K``, ``m`` (0.5), ``n`` (1), ``d`` (0),
   ``G`` (0);
 * ``diffusion`` (`_readHillslope`, :498-599): ``hillslopeKa``,
-  ``hillslopeKm``, ``oFill`` (-6000);
+  ``hillslopeKm``, ``oFill`` (-6000), and the lake-deposit keys
+  ``nonlinKm`` (10), ``nlPitVolume`` (1e9), ``nlPitDepth`` (100), ``nlPitK``,
+  ``pitInletBias`` (0.1), ``marineSolver``, ``picardSub`` (10),
+  ``picardIts`` (2);
 * ``sea`` (`_readSealevel`, :710-787): ``position`` (0);
 * ``climate`` (`_readRain`, :1352-1520): events ``{start, uniform | map:
   [file, key]}``;
@@ -28,9 +32,9 @@ per-step surface-process path uses, with goSPL's defaults
 Everything else in goSPL's input describes subsystems outside this path and
 is ignored.  Settings that would change what this path computes and that it
 does not implement raise ``NotImplementedError`` (transport-limited G > 0, a
-sea-level curve, open or cyclic planar edges); n != 1 selects the non-linear
-stream-power solver, as goSPL's runProcesses does; sediment routing (``nodep: False``, goSPL's default) is not on this
-path and is reported with a warning.  Relative file names are resolved
+sea-level curve, open or cyclic planar edges, marine deposition of river
+sediment); n != 1 selects the non-linear stream-power solver, as goSPL's
+runProcesses does; ``nodep: False`` runs continental sediment routing.  Relative file names are resolved
 against the YAML file's folder, then the working directory, as goSPL opens
 them from the working directory.
 """
@@ -38,7 +42,6 @@ them from the working directory.
 from __future__ import annotations
 
 import os
-import warnings
 
 import numpy as np
 import yaml
@@ -97,9 +100,12 @@ def read_input(filename):
     bc = str(dom.get("bc", "oooo")).replace("0", "o").replace("1", "f")
     if len(bc) != 4 or any(c not in "ofwc" for c in bc):
         raise ValueError("domain.bc must be 4 characters from {o, f, w, c}; got %r" % dom.get("bc"))
-    if not dom.get("nodep", False):
-        warnings.warn("domain.nodep is False: goSPL would also route and deposit sediment, which is not on "
-                      "this path; the model runs as with nodep: True", stacklevel=2)
+    nodep = bool(dom.get("nodep", False))
+    seadepo = bool(dom.get("seadepo", True))
+    if not nodep and seadepo:
+        raise NotImplementedError("domain.nodep is False with domain.seadepo True (goSPL's defaults): marine "
+                                  "deposition of river sediment (seaChange) is not on this path; set seadepo: "
+                                  "False to route and deposit sediment on the continents only, or nodep: True")
     tdict = inp.get("time")
     if tdict is None:
         raise KeyError("Key time is required in the input file!")
@@ -122,6 +128,13 @@ def read_input(filename):
     cda = float(hill.get("hillslopeKa", 0.0))
     cdm = float(hill.get("hillslopeKm", 0.0))
     ofill = float(hill.get("oFill", -6000.0))
+    # _extraHillslope (inputparser.py:546-597): the lake-deposit diffusion
+    sedkw = dict(nonlinKm=float(hill.get("nonlinKm", 10.0)), nlPitVolume=float(hill.get("nlPitVolume", 1.0e9)),
+                 nlPitDepth=float(hill.get("nlPitDepth", 100.0)),
+                 nlPitK=float(hill["nlPitK"]) if hill.get("nlPitK") is not None else None,
+                 pitInletBias=float(hill.get("pitInletBias", 0.10 if "diffusion" in inp else 0.50)),
+                 marineSolver=str(hill.get("marineSolver", "ts")), picardSub=int(hill.get("picardSub", 10)),
+                 picardIts=int(hill.get("picardIts", 2)))
     sea = inp.get("sea") or {}
     if sea.get("curve") is not None:
         raise NotImplementedError("sea.curve: a time-varying sea level is not on this path (sea.position only)")
@@ -150,7 +163,8 @@ def read_input(filename):
         elif ev.get("uniform") is not None:
             sedfac.append((float(ev["start"]), float(ev["uniform"])))
     out = inp.get("output") or {}
-    return dict(name=inp.get("name"), coords=coords, cells=cells, elev=elev, bc=bc,
+    return dict(name=inp.get("name"), coords=coords, cells=cells, elev=elev, bc=bc, nodep=nodep, seadepo=seadepo,
+                sed=sedkw,
                 outputDir=str(out.get("dir", "output")), makedir=bool(out.get("makedir", False)),
                 compression=out.get("compression", "gzip"), rstep=int(tdict.get("rstep", 0)),
                 flowdir=int(dom.get("flowdir", 8)), tstart=float(tdict["start"]), tend=float(tdict["end"]),

@@ -32,6 +32,7 @@ import torch
 from . import _lib
 from .mesh import Mesh
 from .outmesh import WriteMesh
+from .sed import SEDMesh
 
 MISSING_DATA_SENTINEL = -1.0e8   # gospl/tools/constants.py:34
 
@@ -102,7 +103,7 @@ class DeviceMesh:
             pass
 
 
-class Model(WriteMesh):
+class Model(SEDMesh, WriteMesh):
     """Per-step hot path of goSPL on one GPU (FP64 state).
 
     ``Model.sweeps`` counts, process-wide, the exact sweeps launched by every
@@ -133,7 +134,9 @@ class Model(WriteMesh):
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
                  tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
                  refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None,
-                 outputDir=None, makedir=False, tout=None, rstep=0, compression="gzip", finput=None):
+                 outputDir=None, makedir=False, tout=None, rstep=0, compression="gzip", finput=None,
+                 nodep=True, seadepo=False, nlPitVolume=1.0e9, nlPitDepth=100.0, nlPitK=None, nonlinKm=10.0,
+                 pitInletBias=0.10, picardSub=10, picardIts=2, marineSolver="ts"):
         if isinstance(mesh, (str, os.PathLike)):
             # goSPL's constructor, Model(filename, verbose, showlog, profile)
             # (gospl/model.py:212-304): the YAML file and its npz mesh
@@ -244,6 +247,13 @@ class Model(WriteMesh):
         self.native_flow = True
         self._fbuf = None
         self.vSedLocal = torch.zeros(N, **f64)
+        if not nodep and seadepo:
+            raise NotImplementedError("marine deposition of river sediment (seaChange, domain.seadepo) is not on "
+                                      "this path: set seadepo: False")
+        self.seaDepo = bool(seadepo)
+        self._init_sed(nodep=nodep, nlPitVolume=nlPitVolume, nlPitDepth=nlPitDepth, nlPitK=nlPitK,
+                       nonlinKm=nonlinKm, pitInletBias=pitInletBias, picardSub=picardSub, picardIts=picardIts,
+                       marineSolver=marineSolver)
         self._init_output(outputDir=outputDir, makedir=makedir, tout=tout, rstep=rstep, compression=compression,
                           finput=finput)
 
@@ -270,7 +280,8 @@ class Model(WriteMesh):
                       sedfac=first(sched["sedfactor"], inp["tstart"], None), sealevel=inp["sealevel"],
                       ofill=inp["ofill"], tstart=inp["tstart"], tend=inp["tend"], schedule=sched,
                       outputDir=inp["outputDir"], makedir=inp["makedir"], tout=inp["tout"], rstep=inp["rstep"],
-                      compression=inp["compression"], finput=os.path.abspath(filename))
+                      compression=inp["compression"], finput=os.path.abspath(filename), nodep=inp["nodep"],
+                      seadepo=inp["seadepo"], **inp["sed"])
         return (m, m.to_device_order(inp["elev"])), params
 
     def _apply_schedule(self):
@@ -384,17 +395,32 @@ class Model(WriteMesh):
 
     # -------------------------------------------------------------- pits
     def fillElevation(self, sed=False):
-        """pitfilling.py:849-903 (water filling, sed=False)."""
-        # level = max(min h (+1e-3 on the sphere), sealevel + oFill), chosen
-        # on the device (no read-back)
+        """pitfilling.py:849-903: water filling (sed=False) or sediment
+        filling (sed=True: submerged cells keep their filled values, pits are
+        labelled above the fill level, volumes are not clipped at sea
+        level)."""
         S = self.stream
-        self._call("gospl_fill_elevation", self.H, _ptr(self.hLocal), 0 if self.flatModel else 1,
-                   self.sealevel + self.oFill, self.sealevel, _ptr(self.lFill), S)
         npits = ctypes.c_int64(0)
-        self._call("gospl_pit_info", self.H, _ptr(self.hLocal), _ptr(self.lFill), self.sealevel,
-                   _ptr(self.pitIDs), _ptr(self.flatDirs), _ptr(self.ptdir), _ptr(self.lspill),
-                   self._cap, _ptr(self._spill), _ptr(self._pitparams), _ptr(self._flvl),
-                   _ptr(self._fvol), ctypes.byref(npits), S)
+        if sed:
+            # the fill level read back: the pits are labelled above it
+            minh = float(torch.min(self.hLocal))
+            if not self.flatModel:
+                minh += 1.0e-3
+            level = max(minh, self.sealevel + self.oFill)
+            self._call("gospl_fill_depressions", self.H, _ptr(self.hLocal), level, -math.inf, _ptr(self.lFill), S)
+            self._call("gospl_pit_info_sed", self.H, _ptr(self.hLocal), _ptr(self.lFill), level, self.sealevel,
+                       _ptr(self.pitIDs), _ptr(self.flatDirs), _ptr(self.ptdir), _ptr(self.lspill),
+                       self._cap, _ptr(self._spill), _ptr(self._pitparams), _ptr(self._flvl),
+                       _ptr(self._fvol), ctypes.byref(npits), S)
+        else:
+            # level = max(min h (+1e-3 on the sphere), sealevel + oFill), chosen
+            # on the device (no read-back)
+            self._call("gospl_fill_elevation", self.H, _ptr(self.hLocal), 0 if self.flatModel else 1,
+                       self.sealevel + self.oFill, self.sealevel, _ptr(self.lFill), S)
+            self._call("gospl_pit_info", self.H, _ptr(self.hLocal), _ptr(self.lFill), self.sealevel,
+                       _ptr(self.pitIDs), _ptr(self.flatDirs), _ptr(self.ptdir), _ptr(self.lspill),
+                       self._cap, _ptr(self._spill), _ptr(self._pitparams), _ptr(self._flvl),
+                       _ptr(self._fvol), ctypes.byref(npits), S)
         n1 = int(npits.value) + 1
         self.npits = n1 - 1
         # the per-pit tables in one transfer (spill ids are exact in f64)
@@ -715,9 +741,14 @@ class Model(WriteMesh):
         self.flowAccumulation()
 
     def step(self):
-        """Body of the runProcesses loop (model.py:379-453) for this path."""
+        """Body of the runProcesses loop (model.py:379-453) for this path:
+        flow, erosion, (with nodep False) flow again and continental
+        sediment routing, hillslope."""
         self.flowAccumulation()
         self.erodepSPL()
+        if not self.nodep:
+            self.flowAccumulation()
+            self.sedChange()
         self.getHillslope()
         if self.tNow < self.tEnd:
             self.applyForces()

@@ -101,4 +101,5 @@ def oracle_geometry(m: "_mesh.Mesh"):
     ex = m.extras
     return dict(ngb=ex["ngb_input"], nnb=ex["nnb_input"], elgt=ex["elgt_input"],
                 vdist=ex["vdist_input"], area=m.to_input_order(m.area),
-                outlet=m.to_input_order(m.outlet).astype(bool), flat=m.flat)
+                outlet=m.to_input_order(m.outlet).astype(bool), flat=m.flat,
+                coords=m.to_input_order(m.coords))

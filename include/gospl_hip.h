@@ -502,6 +502,97 @@ int gospl_hillslope_refine_seg(gospl_mesh_t* mesh, double cda, double cdm, doubl
                                double* cumed, double* eblocal, int32_t* passes, gospl_stream_t stream);
 
 /*
+ * Sediment-mode depression information: replaces PITFill._pitInformation with
+ * sed=True (gospl/flow/pitfilling.py:704-847, 885-887 as fillElevation(sed=
+ * True) calls it from SEDMesh.sedChange, gospl/sed/sedplex.py:1075).  As
+ * gospl_pit_info, except that pits are labelled above the fill `level`
+ * (label_pits(level, lFill), :718-719) instead of sea level and the per-pit
+ * sums do not clip elevations at sea level (:828-832, 885-887).  `lfill`
+ * comes from gospl_fill_depressions(h, level, sealevel = -INFINITY), which
+ * keeps the filled values of submerged cells (:690-693).
+ */
+int gospl_pit_info_sed(gospl_mesh_t* mesh, const double* hl, double* lfill, double level, double sealevel,
+                       int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill, int64_t cap,
+                       int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
+                       int64_t* npits, gospl_stream_t stream);
+
+/*
+ * Continental sediment routing (SEDMesh, gospl/sed/sedplex.py).  Per-pit
+ * arrays are indexed by pit id (rows 0..npit-1, row 0 unused); per-pit sums
+ * use gospl_pit_inflow's fixed-order grouping of the last gospl_pit_info(_sed)
+ * call for the same `pitids`.
+ *  - gospl_sed_source: b = (-eb) * area, the eroded volume rate that
+ *    _getSedFlux routes (:109-131).
+ *  - gospl_sed_stray: flow-terminal cells outside every pit (lsink &
+ *    pitID < 0) drop their volume, kept in closed_depo when `closed` (no
+ *    draining border, :236-251).
+ *  - gospl_sed_add: one routed pass of _distributeSediment (:424-431): vnext
+ *    = q (0 at draining borders), vsedl += (hl < sedfilled ? 0 : q) / dt.
+ *  - gospl_sed_closed: h and cumed += closed_depo / area where positive
+ *    (:491-500).
+ *  - gospl_sed_bottomup: _bottomUpDelta (:525-560): delta = max(0,
+ *    min(target[pit], lfill) - hl) on cells of pits with a finite target,
+ *    else 0.
+ *  - gospl_sed_tilt: _addPitMicroTilt (:584-655): on cells of selected pits
+ *    (psel[pit]) with delta > 0, delta = max(0, delta + 1e-6 (d - mean_pit d))
+ *    with d the distance to the pit's spill point (spill_xyz, 3 per pit) and
+ *    the mean over those cells (also returned in `mean`, npit rows).
+ *  - gospl_sed_inlets: _identifyPitInlets (:657-697) on the selected pits:
+ *    inlet[c] = 1 where c receives (rcv, nrcv x N) from a donor outside its
+ *    pit; count[p] = inlets of pit p.
+ *  - gospl_sed_lake_depth: sel = cells of selected pits, wd = max(0,
+ *    target[pit] - hl) there, denom[p] = sum of wd * area (:833-871).
+ *  - gospl_sed_lake_init: the initial deposit of _diffuseLargePit (:873-898):
+ *    ca[pit] wd / max(denom, 1e-30) where denom > 0, plus cb[pit] /
+ *    (count[pit] area) at inlets with count > 0.
+ *  - gospl_sed_pit_volume: vol[p] = sum of delta * area over sel cells of p.
+ *  - gospl_sed_scale: delta *= scale[pit] on sel cells, 0 elsewhere.
+ *  - gospl_sed_apply: cumed += delta; h += delta (:1041-1046).
+ */
+int gospl_sed_source(gospl_mesh_t* mesh, const double* eb, double* b, gospl_stream_t stream);
+int gospl_sed_stray(gospl_mesh_t* mesh, const int8_t* lsink, const int32_t* pitids, int32_t closed,
+                    double* vsed, double* closed_depo, gospl_stream_t stream);
+int gospl_sed_add(gospl_mesh_t* mesh, const double* q, const double* hl, const double* sedfilled, double dt,
+                  double* vnext, double* vsedl, gospl_stream_t stream);
+int gospl_sed_closed(gospl_mesh_t* mesh, const double* closed_depo, double* h, double* cumed,
+                     gospl_stream_t stream);
+int gospl_sed_bottomup(gospl_mesh_t* mesh, const int32_t* pitids, const double* target, const double* lfill,
+                       const double* hl, double* delta, gospl_stream_t stream);
+int gospl_sed_tilt(gospl_mesh_t* mesh, const double* xyz, const int32_t* pitids, int64_t npit,
+                   const int8_t* psel, const double* spill_xyz, double* delta, double* mean,
+                   gospl_stream_t stream);
+int gospl_sed_inlets(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcv, const int32_t* pitids, int64_t npit,
+                     const int8_t* psel, int8_t* inlet, double* count, gospl_stream_t stream);
+int gospl_sed_lake_depth(gospl_mesh_t* mesh, const int32_t* pitids, int64_t npit, const int8_t* psel,
+                         const double* target, const double* hl, int8_t* sel, double* wd, double* denom,
+                         gospl_stream_t stream);
+int gospl_sed_lake_init(gospl_mesh_t* mesh, const int32_t* pitids, const int8_t* sel, const int8_t* inlet,
+                        const double* ca, const double* cb, const double* denom, const double* count,
+                        const double* wd, double* delta, gospl_stream_t stream);
+int gospl_sed_pit_volume(gospl_mesh_t* mesh, const double* delta, const int8_t* sel, const int32_t* pitids,
+                         int64_t npit, double* vol, gospl_stream_t stream);
+int gospl_sed_scale(gospl_mesh_t* mesh, const int32_t* pitids, const int8_t* sel, const double* scale,
+                    double* delta, gospl_stream_t stream);
+int gospl_sed_apply(gospl_mesh_t* mesh, const double* delta, double* h, double* cumed, gospl_stream_t stream);
+
+/*
+ * Non-linear thickness diffusion of a deposit over one step: replaces
+ * hillSLP._diffuseImplicit in its lagged-diffusivity form
+ * (_diffuseImplicitPicard, gospl/sed/hillslope.py:925-1076; PETSc KSP
+ * solves of the 13-column jacobiancoeff operator, functions.F90:1421-1467).
+ * h = bed + dh; nsub backward-Euler sub-steps of dt / nsub, each with npic
+ * Picard updates solving (I + dt_sub L(Cd(h))) h' = h_start with Cd =
+ * min_diff + cd0 (1 - exp(-dexp max(h - bed, 0 below 0.1))) on mask cells,
+ * 0 elsewhere; ndepo = max(h - bed, 0).  Each linear solve runs to
+ * convergence (goSPL's Richardson stops at rtol 1e-8); *sweeps (HOST, may
+ * be NULL) receives the Jacobi sweeps used.  Synchronises `stream` once per
+ * linear solve.
+ */
+int gospl_nl_diffusion(gospl_mesh_t* mesh, const double* dh, const int8_t* mask, double cd0, double min_diff,
+                       double dexp, double dt, int32_t nsub, int32_t npic, const double* bed, double* ndepo,
+                       int64_t* sweeps, gospl_stream_t stream);
+
+/*
  * Instrumentation (no counterpart in goSPL): when enabled, HIP events on the
  * launch stream bracket every DAG sweep kernel (flow accumulation and SPL).
  * Returns the accumulated milliseconds and call counts since the previous

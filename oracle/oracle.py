@@ -15,6 +15,7 @@ mesh order:
 * ``erodepSPL``                gospl/eroder/SPL.py:45-114, 260-366, 402-454
 * ``erodepSPLnl`` (n != 1)     gospl/eroder/nlSPL.py:321-456
 * ``getHillslope``             gospl/sed/hillslope.py:90-293, 614-649
+* ``sedChange`` (nodep False)  gospl/sed/sedplex.py:103-1086 (oracle/sed.py)
 * ``applyForces``              gospl/mesher/unstructuredmesh.py:743-794, 903-972
 * ``runProcesses``             gospl/model.py:331-466
 
@@ -44,6 +45,8 @@ import os
 import subprocess
 
 import numpy as np
+
+from .sed import SED
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -191,7 +194,7 @@ def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
 # Model-level restatement
 # --------------------------------------------------------------------------
 
-class OracleModel:
+class OracleModel(SED):
     """goSPL's hot-path state machine in FP64 on the CPU (one rank).
 
     ``geom`` holds input-order arrays: ``ngb`` (N,12) int32 (-1 pad), ``nnb``,
@@ -201,7 +204,7 @@ class OracleModel:
 
     def __init__(self, geom, h, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0,
-                 tstart=0.0, tend=None, n=1.0):
+                 tstart=0.0, tend=None, n=1.0, nodep=True, **sedkw):
         self.ngb = _c(geom["ngb"], np.int32)
         self.nnb = _c(geom["nnb"], np.int32)
         self.elgt = _c(geom["elgt"], np.float64)
@@ -249,9 +252,11 @@ class OracleModel:
         self.cascade_passes = 0
         self.tmp = np.zeros(N)
         self.bL = self.rainVal * self.larea
+        self.coords = None if geom.get("coords") is None else np.asarray(geom["coords"], np.float64)
+        self._init_sed(nodep=nodep, **sedkw)
 
     # ---------------- pit filling (pitfilling.py) ----------------
-    def fillElevation(self):
+    def fillElevation(self, sed=False):
         hl = self.h.copy()
         minh = self.h.min()
         if not self.flatModel:
@@ -260,10 +265,11 @@ class OracleModel:
         self.level = level
         dem = hl - level
         lFill, reached, ledge = fill(dem, self.outlet, self.ngb, self.nnb)
-        idx = lFill < self.sealevel - level
-        lFill[idx] = dem[idx]
+        if not sed:
+            idx = lFill < self.sealevel - level
+            lFill[idx] = dem[idx]
         self.lFill = lFill + level
-        self._pitInformation(hl, level)
+        self._pitInformation(hl, level, sed)
 
         ids = self.pitParams[:, 0] > 0.0
         dh = np.zeros((len(self.pitInfo), 6), dtype=np.float64)
@@ -272,13 +278,14 @@ class OracleModel:
         self.filled_lvl = np.cumsum(dh, axis=1)[:, 1:]
         self.filled_vol = np.zeros((len(self.pitInfo), 5), dtype=np.float64)
         hl_clip = hl.copy()
-        hl_clip[hl_clip < self.sealevel] = self.sealevel
+        if not sed:
+            hl_clip[hl_clip < self.sealevel] = self.sealevel
         self.filled_vol[:, :-1] = getpitvol(self.filled_lvl[:, :-1], hl_clip, self.pitIDs,
                                             self.inIDs, self.larea)
         self.filled_vol[:, -1] = self.pitParams[:, 0]
 
-    def _pitInformation(self, hl, level):
-        pitIDs = label_pits(self.sealevel, self.lFill, self.ngb, self.nnb).astype(np.int64)
+    def _pitInformation(self, hl, level, sed=False):
+        pitIDs = label_pits(level if sed else self.sealevel, self.lFill, self.ngb, self.nnb).astype(np.int64)
         pitIDs[self.outletIDs] = -1
         # _transferIDs at one rank: no cross-rank pairs; consecutive ids (pits_cons)
         pitNb = np.unique(pitIDs[pitIDs >= 0])
@@ -328,7 +335,8 @@ class OracleModel:
         h = hl.copy()
         self.lFill[idx] = hl[idx]
         self.flatDirs[idx] = -1
-        h[h < self.sealevel] = self.sealevel
+        if not sed:
+            h[h < self.sealevel] = self.sealevel
         self._getPitParams(h, pitnbs)
 
     def _dirFlats(self):
@@ -651,6 +659,9 @@ class OracleModel:
             self.erodepSPL()
         else:
             self.erodepSPLnl()
+        if not self.nodep:
+            self.flowAccumulation()
+            self.sedChange()
         self.getHillslope()
         if self.tNow < self.tEnd:
             self.applyForces()

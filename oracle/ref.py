@@ -218,3 +218,24 @@ def getpitvol(hlvl, elev, pit, ids):
     out = np.zeros((n, 4), order="F")
     L.getpitvol_(_p(hv), _p(e), _p(p), _p(i_), _p(out), _i(m), _i(n))
     return np.ascontiguousarray(out)
+
+
+def jacobiancoeff(h, kd, kp):
+    """functions.F90:1421 — dcoeff (N, 13): diagonal then one column per
+    neighbour slot."""
+    L = lib()
+    nb = len(h)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (h, kd, kp)]
+    out = np.zeros((nb, 13), order="F")
+    L.jacobiancoeff_(_p(a[0]), _p(a[1]), _p(a[2]), _p(out), _i(nb))
+    return np.ascontiguousarray(out)
+
+
+def fctcoeff(h, kd):
+    """functions.F90:1382 — the non-linear diffusion residual term (N,)."""
+    L = lib()
+    nb = len(h)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (h, kd)]
+    out = np.zeros(nb)
+    L.fctcoeff_(_p(a[0]), _p(a[1]), _p(out), _i(nb))
+    return out

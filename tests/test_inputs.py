@@ -68,15 +68,24 @@ def test_read_input_keys_and_defaults(tmp_path):
     assert np.all(inp["sedfactor"][0][1] == 0.05)
 
 
-@pytest.mark.parametrize("extra,err", [
-    ("", None),
-])
-def test_nodep_default_warns(tmp_path, extra, err):
+def test_sediment_routing_keys(tmp_path):
+    """nodep False routes sediment; goSPL's default seadepo True (marine
+    deposition) is refused; the lake-deposit diffusion keys are read with
+    goSPL's defaults."""
     fn, *_ = _write_case(tmp_path)
     text = fn.read_text().replace("  nodep: True\n", "")
     fn.write_text(text)
-    with pytest.warns(UserWarning, match="nodep"):
+    with pytest.raises(NotImplementedError, match="seadepo"):
         inputs.read_input(str(fn))
+    fn.write_text(text.replace("  bc: 'ffff'\n", "  bc: 'ffff'\n  seadepo: False\n"))
+    inp = inputs.read_input(str(fn))
+    assert inp["nodep"] is False and inp["seadepo"] is False
+    assert inp["sed"] == dict(nonlinKm=10.0, nlPitVolume=1.0e9, nlPitDepth=100.0, nlPitK=None, pitInletBias=0.10,
+                              marineSolver="ts", picardSub=10, picardIts=2)
+    fn.write_text(text.replace("  bc: 'ffff'\n", "  bc: 'ffff'\n  seadepo: False\n").replace(
+        "  hillslopeKm: 0.2\n", "  hillslopeKm: 0.2\n  nlPitK: 3.\n  marineSolver: picard\n  picardSub: 4\n"))
+    inp = inputs.read_input(str(fn))
+    assert inp["sed"]["nlPitK"] == 3.0 and inp["sed"]["marineSolver"] == "picard" and inp["sed"]["picardSub"] == 4
 
 
 @pytest.mark.parametrize("old,new,exc", [

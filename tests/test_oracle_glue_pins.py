@@ -118,9 +118,9 @@ IDS = [c[0] for c in CASES]
 class Recorder(orc.OracleModel):
     """The oracle with what each glue step saw recorded (no behaviour change)."""
 
-    def _pitInformation(self, hl, level):
+    def _pitInformation(self, hl, level, sed=False):
         self.rec_lfill_pre = self.lFill.copy()
-        super()._pitInformation(hl, level)
+        super()._pitInformation(hl, level, sed)
 
     def _solve_flow(self, b, fatal=True):
         q = super()._solve_flow(b, fatal)

@@ -11,7 +11,8 @@ overflow cascade), stream-power erosion, hillslope diffusion and forcing.
 Inputs are resident in HBM before the timed region.
 
 Three legs of the same workload, each timed over the same window (W untimed
-warm-up steps, then K timed steps between synchronisations):
+warm-up steps, then K timed steps between synchronisations), after ~2 s of
+untimed steps of a throw-away model that bring the GPU out of idle:
 * refined (``value``): hillslope diffusion by FP32 Jacobi corrections of an
   FP64 residual (rtol 1e-12); the triangular receiver-DAG systems by their
   exact FP64 sweeps (a direct solve: one pass, nothing left to refine);
@@ -160,6 +161,35 @@ def cpu_baseline(m, h0, p, steps):
 
 
 LEGS = {"refined": dict(refine=True), "refined_dag": dict(refine=True, refine_dag=True), "fp64": {}}
+
+
+def device_warmup(m, h0, p, dev, seconds):
+    """Untimed steps of a throw-away model of the same workload before the
+    first leg, for at least `seconds`: a leg run on a GPU that has been idle
+    (mesh construction on the host) measured ~1.5x slower over its first
+    ~25 steps than over the next ones, the clocks still ramping up.  Logs the
+    rate of each 10-step window so the ramp shows in the log."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    if seconds <= 0:
+        return
+    g = Model(m, h0, tend=1e12, device=dev, **p)
+    g.initialise()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        t1 = time.perf_counter()
+        for _ in range(10):
+            g.step()
+        torch.cuda.synchronize()
+        n += 10
+        log("[bench] device warm-up: steps %d-%d at %.1f steps/s" % (n - 10, n - 1, 10 / (time.perf_counter() - t1)))
+        if time.perf_counter() - t0 >= seconds:
+            break
+    g.close()
 
 
 def run_leg(m, h0, p, dev, name, steps, warmup):
@@ -458,6 +488,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps per leg")
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--device-warmup", type=float, default=2.0, metavar="SECONDS",
+                    help="untimed steps of a throw-away model before the first leg (GPU clock ramp)")
     ap.add_argument("--level", type=int, default=9)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--legs", default="refined,refined_dag,fp64", help="comma list of " + ", ".join(LEGS))
@@ -511,6 +543,7 @@ def main():
     if args.save_surface and args.steps == 0:
         return
     N = m.N
+    device_warmup(m, h0, p, dev, args.device_warmup)
     legs = {}
     for name in [x.strip() for x in args.legs.split(",") if x.strip()]:
         legs[name] = run_leg(m, h0, p, dev, name, args.steps, args.warmup)

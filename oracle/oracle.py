@@ -178,6 +178,72 @@ def spl_solve(rcv, off, diag, rhs):
     return x, int(bad)
 
 
+def flow_matrix(rcv, wgt):
+    """goSPL's flow matrix fMat = (I - W)^T of matrixFlow (flowplex.py:429-478):
+    W[i, rcv[i, k]] += wgt[i, k], entries at self receivers dropped."""
+    import scipy.sparse as sp
+
+    N, K = rcv.shape
+    rows = np.repeat(np.arange(N), K)
+    cols = np.asarray(rcv, np.int64).ravel()
+    data = -np.asarray(wgt, np.float64).ravel()
+    data[cols == rows] = 0.0
+    return (sp.identity(N, format="csr") + sp.csr_matrix((data, (rows, cols)), shape=(N, N))).T.tocsr()
+
+
+def ero_matrix(rcv, off, diag):
+    """SPL's eMat (SPL.py:73-108): diag on the diagonal, off[:, k] at column
+    rcv[:, k] (zero at self receivers)."""
+    import scipy.sparse as sp
+
+    N, K = rcv.shape
+    rows = np.repeat(np.arange(N), K)
+    return (sp.diags(np.asarray(diag, np.float64)) +
+            sp.csr_matrix((np.asarray(off, np.float64).ravel(), (rows, np.asarray(rcv, np.int64).ravel())),
+                          shape=(N, N))).tocsr()
+
+
+def coupled_ed_solve(rcv, off, diag, rcv_f, wgt_f, fdep, area, dt, hold):
+    """Transport-limited stream power (SPL._coupledEDSystem, SPL.py:116-258):
+    goSPL's nested system in elevation h and sediment flux q,
+
+        [eMat - diag(fDep)   -diag(fDep dt / area)] [h]   [(1 - fDep) hOld]
+        [diag(area / dt)      fMati               ] [q] = [hOld area / dt ]
+
+    (A00 = eMat - diag(fDep) :155-157, A01 :156, A10 :157, fMati the flow
+    matrix of the step's main receivers, right-hand sides :170-174), assembled
+    the same way and solved by sparse LU where goSPL runs TFQMR with a Schur
+    fieldsplit preconditioner to rtol 1e-8 (:181-206).  Returns (h, q)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    N = len(hold)
+    E = ero_matrix(rcv, off, diag)
+    A00 = E + sp.diags(-fdep)
+    A01 = sp.diags(-fdep * dt / area)
+    A10 = sp.diags(area / dt)
+    F = flow_matrix(rcv_f, wgt_f)
+    M = sp.bmat([[A00, A01], [A10, F]], format="csc")
+    rhs = np.concatenate([(1.0 - fdep) * hold, hold * area / dt])
+    x = spla.spsolve(M, rhs)
+    return x[:N], x[N:]
+
+
+def transport_fdep(G, area, PA, sea, outlets=None):
+    """Dimensionless deposition coefficient of the transport-limited solve
+    (SPL.py:316-333): fDep = G area / PA with the denominator floored where
+    the ratio reaches the 0.99 cap, 0 where PA = 0, at sea and (flat meshes)
+    at outlets, capped at 0.99."""
+    num = G * area
+    floor = num / 0.99
+    fdep = np.divide(num, np.maximum(PA, floor), out=np.zeros_like(PA), where=PA != 0)
+    fdep[sea] = 0.0
+    fdep[fdep > 0.99] = 0.99
+    if outlets is not None:
+        fdep[outlets] = 0.0
+    return fdep
+
+
 def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
     """Jacobi sweeps to the bitwise fixed point; raises when ``maxit`` sweeps
     do not reach it (goSPL's Richardson solve would stop at rtol 1e-8)."""
@@ -204,7 +270,7 @@ class OracleModel(SED):
 
     def __init__(self, geom, h, *, K, m=0.5, d=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0,
-                 tstart=0.0, tend=None, n=1.0, nodep=True, **sedkw):
+                 tstart=0.0, tend=None, n=1.0, G=0.0, nodep=True, **sedkw):
         self.ngb = _c(geom["ngb"], np.int32)
         self.nnb = _c(geom["nnb"], np.int32)
         self.elgt = _c(geom["elgt"], np.float64)
@@ -226,6 +292,7 @@ class OracleModel(SED):
         self.maxarea = np.array([self.larea.max()])
         self.K, self.spl_m, self.coeffd = float(K), float(m), float(d)
         self.spl_n = float(n)
+        self.fDepa = float(G)
         self.flowDir, self.flowExp = int(flowdir), 1.1
         self.Cda, self.Cdm = float(cda), float(cdm)
         self.dt = float(dt)
@@ -547,7 +614,15 @@ class OracleModel(SED):
     def erodepSPL(self):
         hOld = self.h.copy()
         off, diag = self._eroMats(hOld)
-        stepED, bad = spl_solve(self.rcvIDi, off, diag, hOld)
+        if self.fDepa > 0.0:
+            # transport-limited (SPL.py:306-333): the coupled h / q system
+            self.fDep = transport_fdep(self.fDepa, self.larea, self.FAL, self.seaID,
+                                       self.outletIDs if self.flatModel else None)
+            stepED, self.sedFluxED = coupled_ed_solve(self.rcvIDi, off, diag, self.rcvIDi, self.wghtVali,
+                                                      self.fDep, self.larea, self.dt, hOld)
+            bad = 0
+        else:
+            stepED, bad = spl_solve(self.rcvIDi, off, diag, hOld)
         if bad:
             raise RuntimeError("SPL receiver graph has a cycle")
         E = (stepED - hOld) / self.dt

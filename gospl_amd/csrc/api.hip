@@ -456,6 +456,9 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->nld_buf);
   dfree(m->sed_mask);
   dfree(m->sed_buf);
+  dfree(m->kry_buf);
+  dfree(m->kry_d);
+  if (m->kry_h) (void)hipHostFree(m->kry_h);
   dfree(m->seg_keys);
   dfree(m->seg_ctl);
   dfree(m->seg_aux);

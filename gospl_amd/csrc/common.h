@@ -107,6 +107,9 @@ struct gospl_mesh {
   double* nld_buf;            // non-linear diffusion: rows (W, N), diagonal and four N vectors
   int8_t* sed_mask;           // sediment routing: (N) int8 cell flags
   double* sed_buf;            // sediment routing: two N vectors
+  double* kry_buf;            // coupled h / q solves: Krylov basis, its preconditioned copies, work vectors
+  double* kry_h;              // pinned host block of their reductions
+  double* kry_d;              // device block of their reductions
 };
 
 namespace gospl {
@@ -129,6 +132,20 @@ int pit_group_params(gospl_mesh* m, const double* hl, const double* fill, double
 int ensure_records(gospl_mesh* m, hipStream_t s);
 int ensure_donors(gospl_mesh* m, int64_t cap);
 int ensure_seg(gospl_mesh* m, int nseg);  // per-segment keys / control words (k_fill.hip)
+// One exact sweep of the receiver DAG (k_dag.hip), synchronising `s`: mode 0
+// the flow system (I - W^T) x = rhs (coef = W, self receivers dropped, a
+// draining cycle iterated to its fixed point), mode 1 the erosion system
+// diag x + off x_rcv = rhs (coef = off).  *unsolved (HOST, may be NULL): the
+// vertices left unsolved (an undrained cycle).
+int dag_solve_ext(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef, const double* diag,
+                  const double* rhs, double* x, int64_t* unsolved, hipStream_t s);
+// The stream-power rows of SPL._eroMats (off (nrcv, N), diag (N)) and the
+// state update of erodepSPL from the new surface x (k_dag.hip).
+int spl_coef_launch(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* dist,
+                    const double* fal, const double* kfac, double K, double mexp, double dt, double sl,
+                    const double* lfill, const double* h, double* off, double* diag, hipStream_t s);
+int spl_update_launch(gospl_mesh* m, const double* x, const int8_t* lsink, double dt, double* h, double* cumed,
+                      double* eb, double* eblocal, hipStream_t s);
 // gospl_flowacc in two halves (k_dag.hip): enqueue, then wait and read the status
 int flowacc_issue(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* b, double* q,
                   hipStream_t s);

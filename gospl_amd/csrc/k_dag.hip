@@ -1767,6 +1767,31 @@ static int dag_solve(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   return dag_finish(m, mode, nrcv, rcv, coef, nlevels, unsolved, s);
 }
 
+int spl_coef_launch(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* dist,
+                    const double* fal, const double* kfac, double K, double mexp, double dt, double sl,
+                    const double* lfill, const double* h, double* off, double* diag, hipStream_t s) {
+  k_spl_coef<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, nrcv, rcv, wgt, dist, fal, kfac, K, mexp, nullptr, dt, sl,
+                                                   lfill, h, off, diag);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int spl_update_launch(gospl_mesh* m, const double* x, const int8_t* lsink, double dt, double* h, double* cumed,
+                      double* eb, double* eblocal, hipStream_t s) {
+  k_spl_update<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, x, m->outlet, m->flat, lsink, dt, h, cumed, eb, eblocal);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int dag_solve_ext(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef, const double* diag,
+                  const double* rhs, double* x, int64_t* unsolved, hipStream_t s) {
+  if (nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("receiver-DAG solve: nrcv must be in 1..12");
+    return 1;
+  }
+  return dag_solve(m, mode, nrcv, rcv, coef, diag, rhs, x, nullptr, unsolved, s);
+}
+
 int flowacc_issue(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* b, double* q,
                   hipStream_t s) {
   if (nrcv < 1 || nrcv > GOSPL_MAXW) {

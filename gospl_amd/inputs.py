@@ -31,7 +31,7 @@ per-step surface-process path uses, with goSPL's defaults
 
 Everything else in goSPL's input describes subsystems outside this path and
 is ignored.  Settings that would change what this path computes and that it
-does not implement raise ``NotImplementedError`` (transport-limited G > 0, a
+does not implement raise ``NotImplementedError`` (transport-limited G > 0 with n != 1, a
 sea-level curve, open or cyclic planar edges, marine deposition of river
 sediment); n != 1 selects the non-linear stream-power solver, as goSPL's
 runProcesses does; ``nodep: False`` runs continental sediment routing.  Relative file names are resolved
@@ -122,8 +122,11 @@ def read_input(filename):
                          float(spl.get("d", 0.0)), float(spl.get("G", 0.0)))
     if not n > 0.0:
         raise ValueError("spl.n = %g: the slope exponent must be positive" % n)
-    if G != 0.0:
-        raise NotImplementedError("spl.G = %g: transport-limited erosion is not on this path (G = 0)" % G)
+    if G < 0.0:
+        raise ValueError("spl.G = %g: the deposition coefficient must be >= 0" % G)
+    if G > 0.0 and n != 1.0:
+        raise NotImplementedError("spl.G = %g with n = %g: the non-linear transport-limited solve is not on this "
+                                  "path (G > 0 needs n = 1)" % (G, n))
     hill = inp.get("diffusion") or {}
     cda = float(hill.get("hillslopeKa", 0.0))
     cdm = float(hill.get("hillslopeKm", 0.0))
@@ -169,7 +172,7 @@ def read_input(filename):
                 compression=out.get("compression", "gzip"), rstep=int(tdict.get("rstep", 0)),
                 flowdir=int(dom.get("flowdir", 8)), tstart=float(tdict["start"]), tend=float(tdict["end"]),
                 dt=float(tdict["dt"]), tout=float(tdict.get("tout", float(tdict["end"]) - float(tdict["start"]))),
-                K=K, m=m, d=d, n=n, cda=cda, cdm=cdm, ofill=ofill, sealevel=sealevel, climate=climate,
+                K=K, m=m, d=d, n=n, G=G, cda=cda, cdm=cdm, ofill=ofill, sealevel=sealevel, climate=climate,
                 tectonics=tecto, sedfactor=sedfac)
 
 

@@ -25,6 +25,7 @@ import ctypes
 import math
 import os
 import time
+import warnings
 
 import numpy as np
 import torch
@@ -130,7 +131,7 @@ class Model(SEDMesh, WriteMesh):
 
     sweeps = {"flow": 0, "spl": 0}
 
-    def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, n=1.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
+    def __init__(self, mesh, h0=None, *args, K=None, m=0.5, d=0.0, n=1.0, G=0.0, flowdir=3, cda=0.0, cdm=0.0, dt=1000.0,
                  rain=1.0, upsub=None, sedfac=None, sealevel=0.0, ofill=-6000.0, tstart=0.0,
                  tend=None, device="cuda", verbose=False, refine=False, refine_dag=False,
                  refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None,
@@ -163,6 +164,14 @@ class Model(SEDMesh, WriteMesh):
         self.flatModel = self.dm.flat
         self.K, self.spl_m, self.coeffd = float(K), float(m), float(d)
         self.spl_n = float(n)
+        self.fDepa = float(G)
+        if self.fDepa < 0.0:
+            raise ValueError("G (the deposition coefficient) must be >= 0")
+        if self.fDepa > 0.0 and self.spl_n != 1.0:
+            raise NotImplementedError("transport-limited erosion (G > 0) with n != 1 is not on this path")
+        # transport-limited solve: GMRES to this relative residual, at most this many iterations
+        self.ed_rtol, self.ed_maxit = 1.0e-13, 400
+        self.ed_iters, self.ed_relres = 0, 0.0
         self.nl_tol, self.nl_maxit = 1.0e-13, 100
         self.nl_iters = 0
         self.flowDir, self.flowExp = int(flowdir), 1.1
@@ -274,7 +283,7 @@ class Model(SEDMesh, WriteMesh):
                      tectonics=[(t0, t1, dev(u) if u is not None else None) for t0, t1, u in inp["tectonics"]],
                      sedfactor=[(t, dev(v)) for t, v in inp["sedfactor"]])
         first = lambda evs, t, dflt: next((v for s, v in reversed(evs) if s <= t), dflt)
-        params = dict(K=inp["K"], m=inp["m"], d=inp["d"], n=inp["n"], flowdir=inp["flowdir"], cda=inp["cda"],
+        params = dict(K=inp["K"], m=inp["m"], d=inp["d"], n=inp["n"], G=inp["G"], flowdir=inp["flowdir"], cda=inp["cda"],
                       cdm=inp["cdm"],
                       dt=inp["dt"], rain=first(sched["climate"], inp["tstart"], 0.0),
                       sedfac=first(sched["sedfactor"], inp["tstart"], None), sealevel=inp["sealevel"],
@@ -676,10 +685,29 @@ class Model(SEDMesh, WriteMesh):
         Model.sweeps["spl"] += int(it.value)
         self.nl_iters = int(it.value)
 
+    def _erodepSPL_ed(self):
+        """SPL._getEroDepRate with G > 0 (gospl/eroder/SPL.py:306-333,
+        116-258): the coupled elevation / sediment-flux solve; goSPL warns
+        and keeps the iterate when its solve does not converge, as here."""
+        it = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        self._call("gospl_spl_dep", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
+                   _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.fDepa, self.dt,
+                   self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),
+                   _ptr(self.Eb), _ptr(self.EbLocal), self.ed_rtol, self.ed_maxit, ctypes.byref(it),
+                   ctypes.byref(rel), None, self.stream)
+        self.ed_iters, self.ed_relres = int(it.value), float(rel.value)
+        Model.sweeps["spl"] += 2 * self.ed_iters
+        if not self.ed_relres <= self.ed_rtol:
+            warnings.warn("Transport-limited SPL solve did not converge after %d iterations (relative residual "
+                          "%.3e)" % (self.ed_iters, self.ed_relres))
+
     def erodepSPL(self):
         if self.spl_n != 1.0:
             # goSPL's runProcesses takes the non-linear solver for n != 1 (model.py:396-401)
             return self.erodepSPLnl()
+        if self.fDepa > 0.0:
+            return self._erodepSPL_ed()
         lev = ctypes.c_int32(0)
         if self.refine_dag:
             npass = ctypes.c_int32(0)

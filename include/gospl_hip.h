@@ -198,6 +198,32 @@ int gospl_spl_nl(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const do
                  double* step_rel, gospl_stream_t stream);
 
 /*
+ * Transport-limited stream power (G > 0, n = 1): replaces
+ * SPL._coupledEDSystem and the G > 0 branch of SPL._getEroDepRate
+ * (gospl/eroder/SPL.py:116-258, 306-333) and the update of erodepSPL.
+ * goSPL's nested system in elevation h and sediment flux q,
+ *   [eMat - diag(fDep)  -diag(fDep dt/area)] [h]   [(1 - fDep) hOld]
+ *   [diag(area/dt)       fMati             ] [q] = [hOld area / dt ]
+ * with eMat the rows of gospl_spl, fMati = (I - W)^T of the same receivers
+ * (rcvi, wgti) and fDep = G area / max(FAL, G area / 0.99) (0 where FAL = 0,
+ * where lfill <= sealevel and at flat-mesh outlets, at most 0.99), is reduced
+ * to its elevation block and solved by restarted GMRES right-preconditioned
+ * by the stream-power sweep (each iteration one exact sweep in each
+ * direction of the receiver DAG), from hOld, until the relative residual of
+ * the elevation block is <= rtol or maxit iterations (goSPL: TFQMR with a
+ * Schur fieldsplit to rtol 1e-8; on non-convergence goSPL warns and keeps
+ * the iterate, as this entry point does: *relres > rtol then tells).
+ * *iters / *relres (HOST, may be NULL).  qs (N, may be NULL) receives the
+ * sediment flux q of the solution.  Then the state update of gospl_spl.
+ * Synchronises `stream` several times per iteration.
+ */
+int gospl_spl_dep(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                  const double* disti, const double* fal, const double* kfac, double K, double m, double G,
+                  double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+                  double* cumed, double* eb, double* eblocal, double rtol, int32_t maxit, int32_t* iters,
+                  double* relres, double* qs, gospl_stream_t stream);
+
+/*
  * Refined solves (the reduced-cost mode; no counterpart in goSPL, which
  * solves these systems with FGMRES + ILU(0) to rtol 1e-8, flowplex.py:341-408).
  * Iterative refinement: each pass takes the FP64 residual r = rhs - A x of the

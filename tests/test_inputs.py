@@ -90,7 +90,8 @@ def test_sediment_routing_keys(tmp_path):
 
 @pytest.mark.parametrize("old,new,exc", [
     ("  d: 0.5\n", "  d: 0.5\n  n: -1.\n", ValueError),
-    ("  d: 0.5\n", "  d: 0.5\n  G: 1.\n", NotImplementedError),
+    ("  d: 0.5\n", "  d: 0.5\n  G: 1.\n  n: 2.\n", NotImplementedError),
+    ("  d: 0.5\n", "  d: 0.5\n  G: -1.\n", ValueError),
     ("  position: -10.\n", "  position: -10.\n  curve: 'sea.csv'\n", NotImplementedError),
     ("  K: 1.e-5\n", "  m: 0.4\n", KeyError),
     ("  dt: 1000.\n", "", KeyError),
@@ -109,6 +110,13 @@ def test_slope_exponent_is_read(tmp_path):
     fn, *_ = _write_case(tmp_path)
     fn.write_text(fn.read_text().replace("  d: 0.5\n", "  d: 0.5\n  n: 1.5\n", 1))
     assert inputs.read_input(str(fn))["n"] == 1.5
+
+
+def test_deposition_coefficient_is_read(tmp_path):
+    fn, *_ = _write_case(tmp_path)
+    assert inputs.read_input(str(fn))["G"] == 0.0
+    fn.write_text(fn.read_text().replace("  d: 0.5\n", "  d: 0.5\n  G: 1.5\n", 1))
+    assert inputs.read_input(str(fn))["G"] == 1.5
 
 
 def test_border_outlets_follow_bc(tmp_path):

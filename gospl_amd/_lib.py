@@ -68,6 +68,8 @@ SIGNATURES = {
                            ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
     "gospl_spl_dep": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, F64, P, P, P, P, P, P, F64, I32,
                             ctypes.POINTER(I32), ctypes.POINTER(F64), P, P]),
+    "gospl_spl_nl_dep": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, F64, F64, P, P, P, P, P, P, F64, I32,
+                               ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
     "gospl_flowacc_adjoint": (I32, [P, I32, P, P, P, P, P]),
     "gospl_spl_adjoint": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P, P, P, P]),
     "gospl_hillslope_frozen": (I32, [P, F64, F64, F64, P, I32, P, P, P]),

@@ -146,6 +146,9 @@ int spl_coef_launch(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* w
                     const double* lfill, const double* h, double* off, double* diag, hipStream_t s);
 int spl_update_launch(gospl_mesh* m, const double* x, const int8_t* lsink, double dt, double* h, double* cumed,
                       double* eb, double* eblocal, hipStream_t s);
+// Kbr of the non-linear stream power (K kfac dt FAL^m elimiter, 0 at sea; k_dag.hip).
+int nl_kbr_launch(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* h, const double* fal, const double* kfac,
+                  double K, double mexp, double dt, double sl, const double* lfill, double* kbr, hipStream_t s);
 // gospl_flowacc in two halves (k_dag.hip): enqueue, then wait and read the status
 int flowacc_issue(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* wgt, const double* b, double* q,
                   hipStream_t s);

@@ -1783,6 +1783,13 @@ int spl_update_launch(gospl_mesh* m, const double* x, const int8_t* lsink, doubl
   return 0;
 }
 
+int nl_kbr_launch(gospl_mesh* m, int nrcv, const int32_t* rcv, const double* h, const double* fal, const double* kfac,
+                  double K, double mexp, double dt, double sl, const double* lfill, double* kbr, hipStream_t s) {
+  k_nl_kbr<<<nblocks(m->N), GOSPL_BLOCK, 0, s>>>(m->N, nrcv, rcv, h, fal, kfac, K, mexp, dt, sl, lfill, kbr);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
 int dag_solve_ext(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, const double* coef, const double* diag,
                   const double* rhs, double* x, int64_t* unsolved, hipStream_t s) {
   if (nrcv < 1 || nrcv > GOSPL_MAXW) {

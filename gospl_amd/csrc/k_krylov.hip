@@ -35,9 +35,9 @@ constexpr int DOT_BLOCKS = 512;  // partial sums per reduction
 constexpr int DOT_THREADS = 256;
 
 // rows of the workspace (each N + 64 doubles): V (KRY_M + 1), Z (KRY_M), then
-// w, t, y, b, x and five per-vertex vectors of the system
+// w, t, y, b, x and six per-vertex vectors of the system
 constexpr int ROW_W = 2 * KRY_M + 1;
-constexpr int KRY_ROWS = ROW_W + 10;
+constexpr int KRY_ROWS = ROW_W + 11;
 
 __global__ __launch_bounds__(DOT_THREADS) void k_mdot(int64_t N, const double* __restrict__ V, int64_t ld,
                                                        const double* __restrict__ w, double* __restrict__ part) {
@@ -104,7 +104,8 @@ __global__ void k_coupled_apply(int64_t N, int NR, const int32_t* __restrict__ r
   } else {
     e = zi;
   }
-  w[i] = (e - f[i] * zi) + d[i] * y[i];
+  if (f) e = e - f[i] * zi;
+  w[i] = e + d[i] * y[i];
 }
 
 // r = b - w
@@ -151,6 +152,103 @@ __global__ void k_flux_rhs(int64_t N, const double* __restrict__ sh, const doubl
   if (i < N) t[i] = sh[i] - s[i] * x[i];
 }
 
+// t = ((hOld - x) area) / dt: the eroded volume rate whose accumulation is the
+// total sediment flux Qt of the non-linear residual (nlSPL.py:104-107)
+__global__ void k_nled_qrhs(int64_t N, const double* __restrict__ hold, const double* __restrict__ x,
+                            const double* __restrict__ area, double dt, double* __restrict__ t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) t[i] = ((hold[i] - x[i]) * area[i]) * (1.0 / dt);
+}
+
+// One Newton linearisation of the non-linear transport-limited residual
+// (_form_residual_ed, nlSPL.py:82-125) at x:
+//   F = (x - hOld)(1 - f) + Kbr S^n - f dt max(Qt, 0) / area,
+// S the weighted positive slope to the lower receivers with w > 0 (local_spl,
+// functions.F90:1469-1502).  Writes the rows of the triangular part
+// (dd = n Kbr S^(n-1); off -dd c_k, diagonal 1 - f + dd sum c), rhs = -F and
+// the coupling d' = f dt / area where Qt > 0 (the derivative of the clipped
+// flux): J = T + diag(d') F^-1 diag(area / dt).  The preconditioning sweep
+// takes T without the -f of the coupled rows (fcoup = f there, 0 elsewhere),
+// applied beside it: J = (T + diag(fcoup)) - diag(fcoup) + diag(d') F^-1 ...,
+// as the stream-power rows precondition the linear solve (a sweep of 1 - f
+// on a coupled row would divide by as little as 0.01).
+__global__ void k_nled_jac(int64_t N, int NR, const int32_t* __restrict__ rcv, const double* __restrict__ wgt,
+                           const double* __restrict__ dist, const double* __restrict__ x,
+                           const double* __restrict__ hold, const double* __restrict__ kbr, double n,
+                           const double* __restrict__ f, const double* __restrict__ qt,
+                           const double* __restrict__ area, double dt, double* __restrict__ off,
+                           double* __restrict__ diag, double* __restrict__ rhs, double* __restrict__ dprime,
+                           double* __restrict__ fcoup) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double xi = x[i];
+  double S = 0.0, c0 = 0.0;
+  double ck[GOSPL_MAXW];
+  for (int k = 0; k < NR; ++k) {
+    ck[k] = 0.0;
+    const double w = wgt[(int64_t)k * N + i];
+    if (w > 0.0) {
+      const double xr = x[rcv[(int64_t)k * N + i]];
+      if (xi > xr) {
+        const double d = dist[(int64_t)k * N + i];
+        S = S + w * (xi - xr) / d;
+        ck[k] = -w / d;
+        c0 = c0 + w / d;
+      }
+    }
+  }
+  const double kb = kbr[i];
+  const double fi = f[i];
+  const double q = qt[i];
+  const double qp = q < 0.0 ? 0.0 : q;
+  double F = (xi - hold[i]) * (1.0 - fi);
+  F = F + kb * (S > 0.0 ? pow(S, n) : 0.0);
+  F = F - (fi * dt) * qp / area[i];
+  const double dd = S > 0.0 ? n * kb * pow(S, n - 1.0) : 0.0;
+  for (int k = 0; k < NR; ++k) off[(int64_t)k * N + i] = dd * ck[k];
+  // where the flux couples (Qt > 0) the -f of J's diagonal is applied beside
+  // the sweep; elsewhere the row is triangular and the sweep takes it whole
+  const bool coupled = q > 0.0;
+  diag[i] = (coupled ? 1.0 : 1.0 - fi) + dd * c0;
+  rhs[i] = -F;
+  dprime[i] = coupled ? fi * dt / area[i] : 0.0;
+  fcoup[i] = coupled ? fi : 0.0;
+}
+
+// x += d; keys of max |d| and max |x| (one atomic per workgroup)
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_newton_step(int64_t N, const double* __restrict__ d,
+                                                            double* __restrict__ x,
+                                                            unsigned long long* __restrict__ keys) {
+  __shared__ unsigned long long sh[2][GOSPL_BLOCK / 64];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long a = 0ull, b = 0ull;
+  if (i < N) {
+    const double dv = d[i];
+    const double xv = x[i] + dv;
+    x[i] = xv;
+    a = dkey(fabs(dv));
+    b = dkey(fabs(xv));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long ca = __shfl_xor(a, o), cb = __shfl_xor(b, o);
+    a = ca > a ? ca : a;
+    b = cb > b ? cb : b;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      a = sh[0][w] > a ? sh[0][w] : a;
+      b = sh[1][w] > b ? sh[1][w] : b;
+    }
+    atomicMax(keys, a);
+    atomicMax(keys + 1, b);
+  }
+}
+
 struct Coupled {
   int nE;
   const int32_t* rcvE;  // nullptr: E = I
@@ -173,18 +271,19 @@ struct Work {
   double* y;
   double* b;
   double* x;
-  double* vec[5];
+  double* vec[6];
   double* part;  // DOT_BLOCKS * (KRY_M + 1) partial sums (device)
   double* dsum;  // KRY_M + 1 sums (device)
   double* dcoef; // KRY_M + 1 coefficients (device)
   double* hsum;  // pinned: sums read back
   double* hcoef; // pinned: coefficients sent
+  unsigned long long* keys;  // two max keys (device)
 };
 
 int ensure_work(gospl_mesh* m, Work* wk) {
   const int64_t ld = m->N + 64;
   if (!m->kry_buf) GOSPL_CHECK(hipMalloc(&m->kry_buf, sizeof(double) * KRY_ROWS * ld));
-  if (!m->kry_d) GOSPL_CHECK(hipMalloc(&m->kry_d, sizeof(double) * (DOT_BLOCKS + 2) * (KRY_M + 2)));
+  if (!m->kry_d) GOSPL_CHECK(hipMalloc(&m->kry_d, sizeof(double) * ((DOT_BLOCKS + 2) * (KRY_M + 2) + 4)));
   if (!m->kry_h) GOSPL_CHECK(hipHostMalloc((void**)&m->kry_h, sizeof(double) * 2 * (KRY_M + 2), hipHostMallocDefault));
   wk->N = m->N;
   wk->ld = ld;
@@ -195,12 +294,13 @@ int ensure_work(gospl_mesh* m, Work* wk) {
   wk->y = wk->t + ld;
   wk->b = wk->y + ld;
   wk->x = wk->b + ld;
-  for (int k = 0; k < 5; ++k) wk->vec[k] = wk->x + (int64_t)(k + 1) * ld;
+  for (int k = 0; k < 6; ++k) wk->vec[k] = wk->x + (int64_t)(k + 1) * ld;
   wk->part = m->kry_d;
   wk->dsum = m->kry_d + (int64_t)DOT_BLOCKS * (KRY_M + 2);
   wk->dcoef = wk->dsum + (KRY_M + 2);
   wk->hsum = m->kry_h;
   wk->hcoef = m->kry_h + (KRY_M + 2);
+  wk->keys = reinterpret_cast<unsigned long long*>(wk->dcoef + (KRY_M + 2));
   return 0;
 }
 
@@ -255,11 +355,15 @@ int precond(gospl_mesh* m, const Coupled& c, const Work& wk, const double* v, do
 // at every restart.  Stops once ||b - S x||_2 <= rtol ||b||_2 or after maxit
 // iterations (then *relres reports what was reached).
 int gmres(gospl_mesh* m, const Coupled& c, const Work& wk, const double* b, double* x, double rtol, int maxit,
-          int* iters, double* relres, hipStream_t s) {
+          int* iters, double* relres, hipStream_t s, bool zero_guess = false, double atol = 0.0,
+          double* bnorm = nullptr) {
   const int64_t N = wk.N;
   double nb = 0.0;
   if (mdot(wk, b, 1, b, &nb, s)) return 1;
   nb = std::sqrt(nb);
+  if (bnorm) *bnorm = nb;
+  // stop at ||r|| <= tol_abs: rtol relative to ||b||, or the absolute floor atol
+  const double tol_abs = rtol * nb > atol ? rtol * nb : atol;
   *iters = 0;
   *relres = 0.0;
   if (nb == 0.0) {
@@ -272,14 +376,19 @@ int gmres(gospl_mesh* m, const Coupled& c, const Work& wk, const double* b, doub
   while (true) {
     // r = b - S x into V_0
     double* v0 = wk.V;
-    if (apply(m, c, wk, x, wk.w, s)) return 1;
-    k_sub<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, b, wk.w, v0);
-    GOSPL_CHECK(hipGetLastError());
+    if (zero_guess && it == 0) {
+      GOSPL_CHECK(hipMemsetAsync(x, 0, sizeof(double) * N, s));
+      GOSPL_CHECK(hipMemcpyAsync(v0, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    } else {
+      if (apply(m, c, wk, x, wk.w, s)) return 1;
+      k_sub<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, b, wk.w, v0);
+      GOSPL_CHECK(hipGetLastError());
+    }
     double beta = 0.0;
     if (mdot(wk, v0, 1, v0, &beta, s)) return 1;
     beta = std::sqrt(beta);
     *relres = beta / nb;
-    if (!(beta > rtol * nb) || it >= maxit) break;
+    if (!(beta > tol_abs) || it >= maxit) break;
     k_scale<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, v0, 1.0 / beta, v0);
     GOSPL_CHECK(hipGetLastError());
     std::fill(g.begin(), g.end(), 0.0);
@@ -316,7 +425,7 @@ int gmres(gospl_mesh* m, const Coupled& c, const Work& wk, const double* b, doub
       g[j] = cs[j] * g[j];
       for (int i = 0; i <= j; ++i) H[i * KRY_M + j] = hcol[i];
       k = j + 1;
-      if (!(std::fabs(g[j + 1]) > rtol * nb) || hn == 0.0) break;
+      if (!(std::fabs(g[j + 1]) > tol_abs) || hn == 0.0) break;
       k_scale<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, wk.w, 1.0 / hn, wk.V + (int64_t)(j + 1) * wk.ld);
       GOSPL_CHECK(hipGetLastError());
     }
@@ -388,6 +497,84 @@ int gospl_spl_dep(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const doub
     if (dag_solve_ext(m, 0, nrcv, rcvi, wgti, nullptr, wk.t, qs, &bad, s)) return 1;
   }
   return spl_update_launch(m, wk.x, lsink, dt, h, cumed, eb, eblocal, s);
+}
+
+int gospl_spl_nl_dep(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const double* wgti, const double* disti,
+                     const double* fal, const double* kfac, double K, double mexp, double nexp, double G, double dt,
+                     double sealevel, const double* lfill, const int8_t* lsink, double* h, double* cumed,
+                     double* eb, double* eblocal, double tol, int32_t maxit, int32_t* iters, int32_t* inner,
+                     double* step_rel, gospl_stream_t stream) {
+  if (!m || nrcv < 1 || nrcv > GOSPL_MAXW) {
+    set_error("gospl_spl_nl_dep: null handle or nrcv outside 1..12");
+    return 1;
+  }
+  if (!(G > 0.0) || !(nexp > 0.0) || !(dt > 0.0) || maxit < 1) {
+    set_error("gospl_spl_nl_dep: G, n and dt must be positive and maxit >= 1");
+    return 1;
+  }
+  hipStream_t s = stream;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  if (gospl_ensure_coef(m, nrcv)) return 1;
+  double* off = m->wcoef;
+  double* diag = m->wd[3];
+  Work wk;
+  if (ensure_work(m, &wk)) return 1;
+  double *f = wk.vec[0], *d = wk.vec[1], *sv = wk.vec[2], *sh = wk.vec[3], *kbr = wk.vec[4], *fcoup = wk.vec[5];
+  double* rhs = wk.b;
+  double* dprime = d;  // the coupling of each Newton step (d = f dt / area is not used here)
+  double* qt = wk.y;   // overwritten by the GMRES sweeps after k_nled_jac has read it
+  double* step = wk.t;
+  double* xs = wk.x;   // the Newton iterate
+  k_ed_prep<<<g, GOSPL_BLOCK, 0, s>>>(N, G, m->area, fal, sealevel, lfill, m->outlet, m->flat, dt, h, f, d, sv, sh);
+  GOSPL_CHECK(hipGetLastError());
+  if (nl_kbr_launch(m, nrcv, rcvi, h, fal, kfac, K, mexp, dt, sealevel, lfill, kbr, s)) return 1;
+  GOSPL_CHECK(hipMemcpyAsync(xs, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  // the Newton step: GMRES's solution vector (sh, the linear solve's right-hand side, is not used here)
+  double* dx = sh;
+  int it = 0, tot = 0;
+  double rel = INFINITY, fnorm0 = 0.0;
+  while (it < maxit) {
+    k_nled_qrhs<<<g, GOSPL_BLOCK, 0, s>>>(N, h, xs, m->area, dt, step);
+    GOSPL_CHECK(hipGetLastError());
+    int64_t bad = 0;
+    if (dag_solve_ext(m, 0, nrcv, rcvi, wgti, nullptr, step, qt, &bad, s)) return 1;
+    if (bad) {
+      set_error("gospl_spl_nl_dep: the flow sweep left %lld vertices on an undrained receiver cycle", (long long)bad);
+      return 1;
+    }
+    k_nled_jac<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rcvi, wgti, disti, xs, h, kbr, nexp, f, qt, m->area, dt, off, diag,
+                                         rhs, dprime, fcoup);
+    GOSPL_CHECK(hipGetLastError());
+    Coupled c{nrcv, rcvi, off, diag, nrcv, rcvi, wgti, fcoup, dprime, sv};
+    int git = 0;
+    double grr = 0.0, fn = 0.0;
+    // each linear solve to 1e-13 of its residual or of the first Newton
+    // residual, whichever is larger (near the root -F is round-off)
+    if (gmres(m, c, wk, rhs, dx, 1.0e-13, 400, &git, &grr, s, true, 1.0e-13 * fnorm0, &fn)) return 1;
+    if (it == 0) fnorm0 = fn;
+    tot += git;
+    GOSPL_CHECK(hipMemsetAsync(wk.keys, 0, sizeof(unsigned long long) * 2, s));
+    k_newton_step<<<g, GOSPL_BLOCK, 0, s>>>(N, dx, xs, wk.keys);
+    GOSPL_CHECK(hipGetLastError());
+    GOSPL_CHECK(hipMemcpyAsync(wk.hsum, wk.keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    unsigned long long hk[2];
+    std::memcpy(hk, wk.hsum, sizeof(hk));
+    ++it;
+    const double dmax = dkey_inv_host(hk[0]), xmax = dkey_inv_host(hk[1]);
+    rel = dmax / (xmax > 1.0 ? xmax : 1.0);
+    if (!(rel == rel)) break;
+    if (rel <= tol) break;
+  }
+  if (iters) *iters = it;
+  if (inner) *inner = tot;
+  if (step_rel) *step_rel = rel;
+  if (!(rel <= tol)) {
+    set_error("gospl_spl_nl_dep: Newton did not converge in %d steps (last step %.3g of max|h|)", it, rel);
+    return 1;
+  }
+  return spl_update_launch(m, xs, lsink, dt, h, cumed, eb, eblocal, s);
 }
 
 }  // extern "C"

@@ -31,7 +31,7 @@ per-step surface-process path uses, with goSPL's defaults
 
 Everything else in goSPL's input describes subsystems outside this path and
 is ignored.  Settings that would change what this path computes and that it
-does not implement raise ``NotImplementedError`` (transport-limited G > 0 with n != 1, a
+does not implement raise ``NotImplementedError`` (a
 sea-level curve, open or cyclic planar edges, marine deposition of river
 sediment); n != 1 selects the non-linear stream-power solver, as goSPL's
 runProcesses does; ``nodep: False`` runs continental sediment routing.  Relative file names are resolved
@@ -124,9 +124,6 @@ def read_input(filename):
         raise ValueError("spl.n = %g: the slope exponent must be positive" % n)
     if G < 0.0:
         raise ValueError("spl.G = %g: the deposition coefficient must be >= 0" % G)
-    if G > 0.0 and n != 1.0:
-        raise NotImplementedError("spl.G = %g with n = %g: the non-linear transport-limited solve is not on this "
-                                  "path (G > 0 needs n = 1)" % (G, n))
     hill = inp.get("diffusion") or {}
     cda = float(hill.get("hillslopeKa", 0.0))
     cdm = float(hill.get("hillslopeKm", 0.0))

@@ -167,8 +167,6 @@ class Model(SEDMesh, WriteMesh):
         self.fDepa = float(G)
         if self.fDepa < 0.0:
             raise ValueError("G (the deposition coefficient) must be >= 0")
-        if self.fDepa > 0.0 and self.spl_n != 1.0:
-            raise NotImplementedError("transport-limited erosion (G > 0) with n != 1 is not on this path")
         # transport-limited solve: GMRES to this relative residual, at most this many iterations
         self.ed_rtol, self.ed_maxit = 1.0e-13, 400
         self.ed_iters, self.ed_relres = 0, 0.0
@@ -672,11 +670,23 @@ class Model(SEDMesh, WriteMesh):
 
     # -------------------------------------------------------------- erosion
     def erodepSPLnl(self):
-        """nlSPL.erodepSPLnl (gospl/eroder/nlSPL.py:321-456), G = 0: Newton
-        steps of exact erosion-direction sweeps to max|step| <= 1e-13
-        max|h| (goSPL's SNES stops at 1e-6)."""
+        """nlSPL.erodepSPLnl (gospl/eroder/nlSPL.py:321-456): Newton steps of
+        exact erosion-direction sweeps (G = 0) or of GMRES solves of the
+        coupled step (G > 0, _solveNL_ed) to max|step| <= 1e-13 max|h|
+        (goSPL's SNES stops at 1e-6)."""
         it = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)
+        if self.fDepa > 0.0:
+            inner = ctypes.c_int32(0)
+            self._call("gospl_spl_nl_dep", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
+                       _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.spl_n,
+                       self.fDepa, self.dt, self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal),
+                       _ptr(self.cumED), _ptr(self.Eb), _ptr(self.EbLocal), self.nl_tol, self.nl_maxit,
+                       ctypes.byref(it), ctypes.byref(inner), ctypes.byref(rel), self.stream)
+            self.nl_iters = int(it.value)
+            self.ed_iters = int(inner.value)
+            Model.sweeps["spl"] += 2 * self.ed_iters
+            return
         self._call("gospl_spl_nl", self.H, self.flowDir, _ptr(self.rcvIDi), _ptr(self.wghtVali),
                    _ptr(self.distRcvi), _ptr(self.FAL), _ptr(self._kfac), self.K, self.spl_m, self.spl_n, self.dt,
                    self.sealevel, _ptr(self.lFill), _ptr(self.lsink), _ptr(self.hLocal), _ptr(self.cumED),

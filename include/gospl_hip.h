@@ -224,6 +224,26 @@ int gospl_spl_dep(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const d
                   double* relres, double* qs, gospl_stream_t stream);
 
 /*
+ * Non-linear transport-limited stream power (G > 0, n != 1): replaces
+ * nlSPL._solveNL_ed / _form_residual_ed and the update of erodepSPLnl
+ * (gospl/eroder/nlSPL.py:82-125, 255-320, 398-456).  The root of
+ *   F(h) = (h - hOld)(1 - fDep) + Kbr S(h)^n - fDep dt max(Qt(h), 0) / area,
+ *   Qt(h) = fMati^-1 ((hOld - h) area / dt),
+ * (Kbr, S as gospl_spl_nl; fDep as gospl_spl_dep) by Newton steps, each
+ * linear system J d = -F (the triangular slope part plus the flux coupling
+ * where Qt > 0) solved by the GMRES of gospl_spl_dep to a relative residual
+ * of 1e-13, until max|d| <= tol max(max|h|, 1) (goSPL's SNES: L-BFGS to
+ * rtol/atol 1e-6), at most maxit steps, failing otherwise.  *iters / *inner /
+ * *step_rel (HOST, may be NULL): Newton steps, GMRES iterations in all and
+ * the last relative step.  Then the state update of gospl_spl.
+ */
+int gospl_spl_nl_dep(gospl_mesh_t* mesh, int32_t nrcv, const int32_t* rcvi, const double* wgti,
+                     const double* disti, const double* fal, const double* kfac, double K, double m, double n,
+                     double G, double dt, double sealevel, const double* lfill, const int8_t* lsink, double* h,
+                     double* cumed, double* eb, double* eblocal, double tol, int32_t maxit, int32_t* iters,
+                     int32_t* inner, double* step_rel, gospl_stream_t stream);
+
+/*
  * Refined solves (the reduced-cost mode; no counterpart in goSPL, which
  * solves these systems with FGMRES + ILU(0) to rtol 1e-8, flowplex.py:341-408).
  * Iterative refinement: each pass takes the FP64 residual r = rhs - A x of the

@@ -244,6 +244,57 @@ def transport_fdep(G, area, PA, sea, outlets=None):
     return fdep
 
 
+def nl_transport_solve(rcv, wgt, dist, kbr, n, fdep, area, dt, hold, rcv_f=None, wgt_f=None, tol=1.0e-14,
+                       maxit=100):
+    """Non-linear transport-limited stream power (nlSPL._solveNL_ed and
+    _form_residual_ed, gospl/eroder/nlSPL.py:82-125, 255-320): the root of
+
+        F(h) = (h - hOld)(1 - fDep) + Kbr S(h)^n - fDep dt Qt(h) / area,
+        Qt(h) = max(fMati^-1 ((hOld - h) area / dt), 0),
+
+    S the weighted positive slope of local_spl (functions.F90:1469-1502:
+    receivers with w > 0 lower than the vertex, S clipped at 0), by Newton
+    steps whose linear systems are goSPL's two-field form solved by sparse LU:
+
+        [J_S + diag(1 - fDep)   -diag(fDep dt / area [Qt > 0])] [dh]   [-F]
+        [diag(area / dt)         fMati                        ] [dq] = [ 0]
+
+    (J_S the derivative of Kbr S^n, local_spl_coeff :1504-1542), until the
+    step is below tol max(max|h|, 1).  goSPL solves it with SNES (L-BFGS, rtol
+    / atol 1e-6).  Returns (h, Newton steps)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    N, K = rcv.shape
+    rcv_f = rcv if rcv_f is None else rcv_f
+    wgt_f = wgt if wgt_f is None else wgt_f
+    F = flow_matrix(rcv_f, wgt_f).tocsc()
+    Fs = spla.splu(F)
+    x = np.array(hold, np.float64)
+    rows = np.repeat(np.arange(N), K)
+    for it in range(1, maxit + 1):
+        Qt = Fs.solve(((hold - x) * area) * (1.0 / dt))
+        Qp = np.where(Qt < 0.0, 0.0, Qt)
+        on = (wgt > 0.0) & (x[:, None] > x[rcv])
+        wd = np.where(on, wgt / np.where(on, dist, 1.0), 0.0)
+        S = np.sum(wd * (x[:, None] - x[rcv]), axis=1)
+        S = np.where(S < 0.0, 0.0, S)
+        Sn = np.where(S > 0.0, np.power(np.where(S > 0.0, S, 1.0), n), 0.0)
+        res = (x - hold) * (1.0 - fdep)
+        res = res + kbr * Sn
+        res = res - (fdep * dt) * Qp / area
+        dd = np.where(S > 0.0, n * kbr * np.power(np.where(S > 0.0, S, 1.0), n - 1.0), 0.0)
+        T = sp.diags((1.0 - fdep) + dd * wd.sum(axis=1)) + sp.csr_matrix(
+            ((-dd[:, None] * wd).ravel(), (rows, np.asarray(rcv, np.int64).ravel())), shape=(N, N))
+        dprime = np.where(Qt > 0.0, fdep * dt / area, 0.0)
+        M = sp.bmat([[T, sp.diags(-dprime)], [sp.diags(area / dt), F]], format="csc")
+        step = spla.spsolve(M, np.concatenate([-res, np.zeros(N)]))[:N]
+        x = x + step
+        if np.abs(step).max() <= tol * max(np.abs(x).max(), 1.0):
+            return x, it
+    raise RuntimeError("non-linear transport-limited oracle: Newton did not converge in %d steps" % maxit)
+
+
 def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
     """Jacobi sweeps to the bitwise fixed point; raises when ``maxit`` sweeps
     do not reach it (goSPL's Richardson solve would stop at rtol 1e-8)."""
@@ -637,8 +688,8 @@ class OracleModel(SED):
 
     # ---------------- non-linear erosion (nlSPL.py) ----------------
     def erodepSPLnl(self):
-        """nlSPL._solveNL + erodepSPLnl (gospl/eroder/nlSPL.py:321-456) for
-        G = 0.  goSPL solves F(x) = x - hOld + Kbr S(x)^n = 0 with SNES
+        """nlSPL._solveNL + erodepSPLnl (gospl/eroder/nlSPL.py:321-456); G > 0
+        goes to nl_transport_solve.  For G = 0 goSPL solves F(x) = x - hOld + Kbr S(x)^n = 0 with SNES
         (rtol/atol 1e-6); S_i = sum over receivers with w > 0 and x_i > x_r of
         w (x_i - x_r) / d (local_spl, functions.F90:1469-1502).  F_i depends on
         x_i and its receivers only, so here each vertex is solved exactly in
@@ -658,7 +709,15 @@ class OracleModel(SED):
         n = self.spl_n
         x = hOld.copy()
         rcv, wgt, dist = self.rcvIDi, self.wghtVali, self.distRcvi
-        for i in np.argsort(hOld, kind="stable"):
+        if self.fDepa > 0.0:
+            # transport-limited (nlSPL._solveNL_ed, nlSPL.py:255-320)
+            self.fDep = transport_fdep(self.fDepa, self.larea, PA, self.seaID,
+                                       self.outletIDs if self.flatModel else None)
+            x, self.nl_iters = nl_transport_solve(rcv, wgt, dist, Kbr, n, self.fDep, self.larea, self.dt, hOld)
+            order = []
+        else:
+            order = np.argsort(hOld, kind="stable")
+        for i in order:
             kb = Kbr[i]
             if kb == 0.0:
                 continue

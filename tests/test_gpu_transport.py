@@ -104,3 +104,31 @@ def test_transport_limited_flux_block_through_c_abi():
                 _ptr(g._kfac), g.K, g.spl_m, 0.0, g.dt, g.sealevel, _ptr(g.lFill), _ptr(g.lsink), _ptr(h),
                 _ptr(cumed), _ptr(eb), _ptr(ebl), 1e-13, 400, None, None, None, g.stream)
     g.close()
+
+
+@pytest.mark.parametrize("kind,size,n,G", [("sphere", 5, 1.5, 1.0), ("planar", 40, 2.0, 2.0)])
+def test_nonlinear_transport_limited_steps_match_oracle(kind, size, n, G):
+    """n != 1 with G > 0 (nlSPL._solveNL_ed): Newton steps of GMRES solves
+    against the oracle's Newton steps of sparse-LU solves of the two-field
+    system, per step from identical state, 1e-9."""
+    from gospl_amd.model import Model
+
+    m, h0, p = case(kind, size)
+    print("non-linear transport-limited SPL: %s %d, n = %g, G = %g" % (kind, size, n, G), flush=True)
+    g = Model(m, h0, tend=1e12, n=n, G=G, **p)
+    o = oracle_model(m, h0, p, n=n, G=G)
+    g.initialise()
+    o.initialise()
+    for s in range(3):
+        _sync_oracle(m, g, o)
+        g.step()
+        o.step()
+        assert 1 <= g.nl_iters <= 30, g.nl_iters
+        assert g.ed_iters <= 40 * g.nl_iters, (g.ed_iters, g.nl_iters)
+        h = m.to_input_order(g.hLocal.cpu().numpy())
+        assert rel_err(h, o.h) < TOL, (s, rel_err(h, o.h))
+        ed = m.to_input_order(g.cumED.cpu().numpy())
+        assert np.abs(ed - o.cumED).max() <= TOL * np.abs(o.h).max(), s
+        print("  step %d: Newton steps %d (oracle %d), GMRES iterations %d, rel err %.2e" %
+              (s, g.nl_iters, o.nl_iters, g.ed_iters, rel_err(h, o.h)), flush=True)
+    g.close()

@@ -90,7 +90,6 @@ def test_sediment_routing_keys(tmp_path):
 
 @pytest.mark.parametrize("old,new,exc", [
     ("  d: 0.5\n", "  d: 0.5\n  n: -1.\n", ValueError),
-    ("  d: 0.5\n", "  d: 0.5\n  G: 1.\n  n: 2.\n", NotImplementedError),
     ("  d: 0.5\n", "  d: 0.5\n  G: -1.\n", ValueError),
     ("  position: -10.\n", "  position: -10.\n  curve: 'sea.csv'\n", NotImplementedError),
     ("  K: 1.e-5\n", "  m: 0.4\n", KeyError),

@@ -101,3 +101,59 @@ def test_fdep_floor_is_the_capped_ratio():
     for i in range(500):
         want = 0.0 if (pa[i] == 0.0 or i in set(sea.tolist())) else min(2.0 * area[i] / pa[i], 0.99)
         assert abs(f[i] - want) <= 1e-15 * max(want, 1.0)
+
+
+# ---------------------------------------------------------------------------
+# non-linear transport-limited solve (oracle.nl_transport_solve,
+# nlSPL._solveNL_ed, nlSPL.py:82-125, 255-320)
+# ---------------------------------------------------------------------------
+
+def test_nonlinear_three_vertex_chain_by_nested_roots():
+    from scipy.optimize import brentq
+
+    rcv = np.array([[0], [0], [1]], np.int32)
+    wgt = np.array([[0.0], [1.0], [1.0]])
+    dist = np.array([[1.0], [1.0], [2.0]])
+    kbr = np.array([0.0, 3.0, 0.01])
+    area = np.array([3.0, 2.0, 5.0])
+    dt = 7.0
+    f1 = 0.4
+    fdep = np.array([0.0, f1, 0.0])
+    hold = np.array([-10.0, 20.0, 50.0])
+    n = 2.0
+    x, its = orc.nl_transport_solve(rcv, wgt, dist, kbr, n, fdep, area, dt, hold)
+
+    def x2_of(x1):  # vertex 2: x2 - hOld2 + kbr2 ((x2 - x1) / d2)^n = 0, x2 in [x1, hOld2]
+        return brentq(lambda y: y - hold[2] + kbr[2] * (max(y - x1, 0.0) / dist[2, 0]) ** n, x1, hold[2],
+                      xtol=1e-15, rtol=1e-15)
+
+    def F1(x1):  # vertex 1 with the total flux leaving it (its own and vertex 2's eroded volume rate)
+        qt = area[1] * (hold[1] - x1) / dt + area[2] * (hold[2] - x2_of(x1)) / dt
+        return (x1 - hold[1]) * (1.0 - f1) + kbr[1] * (max(x1 - hold[0], 0.0) / dist[1, 0]) ** n \
+            - f1 * dt * max(qt, 0.0) / area[1]
+
+    x1 = brentq(F1, hold[0], hold[1], xtol=1e-15, rtol=1e-15)
+    assert abs(x[0] - hold[0]) < 1e-13
+    assert abs(x[1] - x1) < 1e-11 * abs(hold).max()
+    assert abs(x[2] - x2_of(x1)) < 1e-11 * abs(hold).max()
+    assert its <= 20
+
+
+def test_nonlinear_without_deposition_is_the_vertex_by_vertex_solve():
+    """G = 0: the Newton iteration with sparse-LU steps equals the oracle's
+    independent vertex-by-vertex root finding (erodepSPLnl)."""
+    m, h0, p = case("sphere", 5)
+    o = oracle_model(m, h0, p, n=1.5)
+    o.initialise()
+    o.step()
+    o.flowAccumulation()
+    hold = o.h.copy()
+    PA = o.FAL
+    dh = (hold[:, None] - hold[o.rcvIDi]).max(axis=1)
+    elim = np.divide(dh, dh + 1.0e-2, out=np.zeros_like(dh), where=dh != 0)
+    kbr = o.K * (o.rainVal ** o.coeffd) * (o.dt * (PA ** o.spl_m) * elim)
+    kbr[o.seaID] = 0.0
+    x, _ = orc.nl_transport_solve(o.rcvIDi, o.wghtVali, o.distRcvi, kbr, 1.5, np.zeros(o.lpoints), o.larea, o.dt,
+                                  hold)
+    o.erodepSPLnl()
+    assert np.abs(x - o.nl_x).max() <= 1e-12 * np.abs(hold).max()

@@ -70,6 +70,16 @@ SIGNATURES = {
                             ctypes.POINTER(I32), ctypes.POINTER(F64), P, P]),
     "gospl_spl_nl_dep": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, F64, F64, P, P, P, P, P, P, F64, I32,
                                ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(F64), P]),
+    "gospl_coast_distance": (I32, [P, P, P, F64, P, P, ctypes.POINTER(I64), P]),
+    "gospl_eps_fill": (I32, [P, P, F64, P, ctypes.POINTER(I32), P]),
+    "gospl_diffuse_fixed": (I32, [P, P, P, P, I32, ctypes.POINTER(I32), P]),
+    "gospl_marine_receivers": (I32, [P, P, P, P, P, P]),
+    "gospl_marine_prep": (I32, [P, P, P, P, P, F64, F64, F64, P, P, P]),
+    "gospl_marine_route": (I32, [P, P, P, P, P, P, P, ctypes.POINTER(I32), P]),
+    "gospl_marine_dep": (I32, [P, P, P, P, P, F64, F64, F64, P, F64, I32, ctypes.POINTER(I32),
+                               ctypes.POINTER(F64), P]),
+    "gospl_marine_thickness": (I32, [P, P, F64, P, P]),
+    "gospl_marine_apply": (I32, [P, P, F64, P, P, P, P]),
     "gospl_flowacc_adjoint": (I32, [P, I32, P, P, P, P, P]),
     "gospl_spl_adjoint": (I32, [P, I32, P, P, P, P, P, F64, F64, F64, F64, P, P, P, P, P, P, P, P, P]),
     "gospl_hillslope_frozen": (I32, [P, F64, F64, F64, P, I32, P, P, P]),

@@ -459,6 +459,12 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->kry_buf);
   dfree(m->kry_d);
   if (m->kry_h) (void)hipHostFree(m->kry_h);
+  dfree(m->sea_d);
+  dfree(m->sea_ctl);
+  dfree(m->sea_pts);
+  dfree(m->sea_i);
+  dfree(m->sea_cell);
+  if (m->sea_h) (void)hipHostFree(m->sea_h);
   dfree(m->seg_keys);
   dfree(m->seg_ctl);
   dfree(m->seg_aux);

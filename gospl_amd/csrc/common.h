@@ -110,6 +110,13 @@ struct gospl_mesh {
   double* kry_buf;            // coupled h / q solves: Krylov basis, its preconditioned copies, work vectors
   double* kry_h;              // pinned host block of their reductions
   double* kry_d;              // device block of their reductions
+  double* sea_d;              // marine routing: six N vectors
+  double* sea_ctl;            // its partial sums and control values
+  double* sea_pts;            // coastline points and their grid-sorted copy (3 doubles each)
+  int32_t* sea_i;             // coastline counts / offsets
+  int32_t* sea_cell;          // grid cell of each coastline point
+  int64_t sea_pcap;           // points the two hold
+  unsigned long long* sea_h;  // pinned host block of the marine read-backs
 };
 
 namespace gospl {

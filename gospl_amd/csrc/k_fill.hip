@@ -123,24 +123,26 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_min_key_seg(int64_t N, const do
   }
 }
 
-// state: 1 seed, 0 included, -1 excluded (edge_tile, functions.F90:2036-2080)
+// state: 1 seed, 0 included, -1 excluded (edge_tile, functions.F90:2036-2080).
+// raw (epsfill's base, functions.F90:2633-2697): elevations unshifted, seeds
+// the cells below the level next to one at or above it, no border seeds.
 __global__ void k_fill_init(int64_t N, int W, const int32_t* __restrict__ nbr,
                             const int32_t* __restrict__ deg, const int8_t* __restrict__ outlet,
                             int flat, const double* __restrict__ h, double level_in, FillLevel lv,
-                            double* __restrict__ dem, int32_t* __restrict__ state) {
+                            double* __restrict__ dem, int32_t* __restrict__ state, int raw) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double level = fill_level(level_in, lv, i);
-  const double z = h[i] - level;
+  const double z = raw ? h[i] : h[i] - level;
   dem[i] = z;
-  const bool border = flat && outlet[i];
+  const bool border = !raw && flat && outlet[i];
   int st = 0;
-  if (z < 0.0) {
+  if (raw ? z < level : z < 0.0) {
     bool up = false;
     const int dg = deg[i];
     for (int p = 0; p < W && p < dg; ++p) {
       int n = nbr[(int64_t)p * N + i];
-      if (n >= 0 && h[n] - level >= 0.0) { up = true; break; }
+      if (n >= 0 && (raw ? h[n] >= level : h[n] - level >= 0.0)) { up = true; break; }
     }
     st = up ? 1 : -1;
   } else if (border) {
@@ -281,21 +283,62 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_fill_relax_list(
 __global__ void k_fill_final(int64_t N, const double* __restrict__ dem,
                              const int32_t* __restrict__ state, const int32_t* __restrict__ root,
                              const unsigned long long* __restrict__ S, double level_in, FillLevel lv,
-                             double sl, double* __restrict__ lfill) {
+                             double sl, double* __restrict__ lfill, int8_t* __restrict__ reach) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double level = fill_level(level_in, lv, i);
   const double z = dem[i];
   double f = z;
+  bool reached = false;
   if (state[i] >= 0) {
     unsigned long long s = S[root[i]];
-    if (s != KEY_POS_INF && z > 0.0) {
+    reached = s != KEY_POS_INF;
+    if (reached && (reach ? z > level : z > 0.0)) {
       double sv = dkey_inv(s);
       f = sv > z ? sv : z;
     }
   }
+  if (reach) {
+    // raw mode (epsfill's base): unshifted, no marine reset
+    reach[i] = reached ? 1 : 0;
+    lfill[i] = f;
+    return;
+  }
   if (f < sl - level) f = z;
   lfill[i] = f + level;
+}
+
+// epsfill (functions.F90:2633-2697): the priority flood raises a reached
+// cell c at or above the level to the next double above the fill of the
+// first neighbour popped before it, the one with the least fill:
+//   F(c) = max(z_c, nextafter(min_n F(n), +inf)),
+// seeds and excluded cells keep z.  F is the unique fixed point of that map
+// (the queue pops fills in non-decreasing order); the minimax fill is below
+// it everywhere, so raising from it until no cell changes reaches it.
+__global__ void k_eps_relax(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                            const double* __restrict__ h, const int32_t* __restrict__ state,
+                            const int8_t* __restrict__ reach, double* __restrict__ F, int32_t* __restrict__ flag,
+                            int pass) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (state[i] != 0 || !reach[i]) return;
+  double mn = INFINITY;
+  const int dg = deg[i];
+  for (int p = 0; p < W && p < dg; ++p) {
+    const int n = nbr[(int64_t)p * N + i];
+    if (n >= 0) {
+      const double v = F[n];
+      mn = v < mn ? v : mn;
+    }
+  }
+  if (mn == INFINITY) return;
+  const double c = nextafter(mn, INFINITY);
+  const double z = h[i];
+  const double nf = z > c ? z : c;
+  if (nf > F[i]) {
+    F[i] = nf;
+    *flag = pass;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -653,7 +696,7 @@ using namespace gospl;
 extern "C" {
 
 static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel lv, double sealevel,
-                     double* lfill, hipStream_t s) {
+                     double* lfill, hipStream_t s, int8_t* reach = nullptr) {
   const int64_t N = m->N;
   double* dem = m->wd[0];
   int32_t* state = m->wi[3];
@@ -661,7 +704,7 @@ static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel l
   unsigned long long* S = m->wu[0];
   const int g = nblocks(N);
   k_fill_init<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->outlet, m->flat, h, level, lv, dem,
-                                         state);
+                                         state, reach ? 1 : 0);
   k_fill_descent<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, dem, state, root, S);
   GOSPL_CHECK(hipGetLastError());
   int it = 0;
@@ -687,8 +730,28 @@ static int fill_body(gospl_mesh_t* m, const double* h, double level, FillLevel l
                                                       bcount, f, pass);
       }, &it))
     return 1;
-  k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, lv, sealevel, lfill);
+  k_fill_final<<<g, GOSPL_BLOCK, 0, s>>>(N, dem, state, root, S, level, lv, sealevel, lfill, reach);
   GOSPL_CHECK(hipGetLastError());
+  return 0;
+}
+
+int gospl_eps_fill(gospl_mesh_t* m, const double* h, double level, double* fillz, int32_t* passes,
+                   gospl_stream_t s) {
+  if (!m) {
+    set_error("gospl_eps_fill: null handle");
+    return 1;
+  }
+  const int64_t N = m->N;
+  int8_t* reach = reinterpret_cast<int8_t*>(m->wi[7]);
+  if (fill_body(m, h, level, FillLevel{nullptr, 0, 0.0, nullptr}, -INFINITY, fillz, s, reach)) return 1;
+  const int32_t* state = m->wi[3];
+  const int g = nblocks(N);
+  int it = 0;
+  if (until_stable(m, s, (int)N + 16, 4, &m->iter_hint[3], [&](int32_t* f, int pass) {
+        k_eps_relax<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, h, state, reach, fillz, f, pass);
+      }, &it))
+    return 1;
+  if (passes) *passes = it;
   return 0;
 }
 

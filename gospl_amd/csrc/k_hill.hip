@@ -15,6 +15,8 @@
 #include <cstring>
 #include <vector>
 
+#include <utility>
+
 #include "common.h"
 
 namespace gospl {
@@ -529,9 +531,66 @@ __global__ void k_hill_coef_kd(int64_t N, int W, const int32_t* __restrict__ nbr
   out[i] = a > 0.0 ? 1.0 + s1 : 0.0;
 }
 
+// flat-mesh outlets: identity rows (hillslope.py:253-255)
+__global__ void k_hill_outlet_rows(int64_t N, const int8_t* __restrict__ outlet, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || !outlet[i]) return;
+  out[i] = 1.0;
+  for (int p = 1; p <= GOSPL_MAXW; ++p) out[(int64_t)p * N + i] = 0.0;
+}
+
 }  // namespace gospl
 
 using namespace gospl;
+
+extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const double* rhs, double* x, int32_t maxit,
+                                   int32_t* sweeps, gospl_stream_t s) {
+  using namespace gospl;
+  if (!m || maxit < 1) {
+    set_error("gospl_diffuse_fixed: null handle or maxit < 1");
+    return 1;
+  }
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  if (gospl_ensure_coef(m, 13)) return 1;
+  double* op = m->wcoef;  // (13, N): row 0 the diagonal, row 1 + p neighbour p
+  k_hill_coef_kd<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, kd, op);
+  if (m->flat) k_hill_outlet_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->outlet, op);
+  GOSPL_CHECK(hipGetLastError());
+  // Jacobi from x = rhs to the bitwise fixed point, the change measured every 16th sweep
+  double* xa = m->wd[1];
+  double* xb = m->wd[2];
+  GOSPL_CHECK(hipMemcpyAsync(xa, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  double* xin = xa;
+  double* xout = xb;
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
+  unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
+  int done = 0;
+  bool fixed = false;
+  while (!fixed && done < maxit) {
+    for (int k = 0; k < 15 && done < maxit - 1; ++k, ++done) {
+      k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, xin, xout);
+      std::swap(xin, xout);
+    }
+    GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
+    k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, xin, xout, keys);
+    GOSPL_CHECK(hipGetLastError());
+    std::swap(xin, xout);
+    ++done;
+    GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+    GOSPL_CHECK(hipStreamSynchronize(s));
+    const double chg = dkey_inv_host(hk[0]);
+    if (!(chg == chg)) break;
+    fixed = chg == 0.0;
+  }
+  if (!fixed) {
+    set_error("gospl_diffuse_fixed: Jacobi sweeps reached no fixed point within %d sweeps", (int)maxit);
+    return 1;
+  }
+  if (sweeps) *sweeps = done;
+  GOSPL_CHECK(hipMemcpyAsync(x, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
 
 extern "C" int gospl_hillslope_coeff(gospl_mesh_t* m, const double* kd, double* coeff,
                                      gospl_stream_t s) {

@@ -249,6 +249,42 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_newton_step(int64_t N, const do
   }
 }
 
+// Marine deposition coefficients (seaplex.py:262-293): fDepm = min(Gmar,
+// 0.99), 0 where h > sea level; d = fDepm dt / area, s = area / dt;
+// r0 = (1 - fDepm) h, r1 = vol / dt + (h area) / dt (kept in b and t).
+__global__ void k_mdep_prep(int64_t N, double Gm, double sl, double dt, const double* __restrict__ area,
+                            const double* __restrict__ h, const double* __restrict__ vol, double* __restrict__ f,
+                            double* __restrict__ d, double* __restrict__ s, double* __restrict__ r0,
+                            double* __restrict__ r1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double fd = Gm > 0.99 ? 0.99 : Gm;
+  const double hi = h[i];
+  if (hi > sl) fd = 0.0;
+  const double a = area[i];
+  f[i] = fd;
+  d[i] = fd * dt / a;
+  s[i] = a / dt;
+  r0[i] = (1.0 - fd) * hi;
+  r1[i] = vol[i] / dt + (hi * a) * (1.0 / dt);
+}
+
+// b = r0 + d y (y = F^-1 r1), in place of r0
+__global__ void k_mdep_rhs(int64_t N, const double* __restrict__ d, const double* __restrict__ y,
+                           double* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) b[i] = b[i] + d[i] * y[i];
+}
+
+// volDep = max((hnew - h) area, 0) (seaplex.py:394-399)
+__global__ void k_mdep_vol(int64_t N, const double* __restrict__ hn, const double* __restrict__ h,
+                           const double* __restrict__ area, double* __restrict__ vol) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double v = (hn[i] - h[i]) * area[i];
+  vol[i] = v < 0.0 ? 0.0 : v;
+}
+
 struct Coupled {
   int nE;
   const int32_t* rcvE;  // nullptr: E = I
@@ -575,6 +611,46 @@ int gospl_spl_nl_dep(gospl_mesh_t* m, int32_t nrcv, const int32_t* rcvi, const d
     return 1;
   }
   return spl_update_launch(m, xs, lsink, dt, h, cumed, eb, eblocal, s);
+}
+
+int gospl_marine_dep(gospl_mesh_t* m, const int32_t* rcv, const double* wgt, const double* h, const double* vol,
+                     double Gmar, double sealevel, double dt, double* voldep, double rtol, int32_t maxit,
+                     int32_t* iters, double* relres, gospl_stream_t stream) {
+  if (!m || !rcv || !wgt || !h || !vol || !voldep) {
+    set_error("gospl_marine_dep: null argument");
+    return 1;
+  }
+  if (!(Gmar > 0.0) || !(dt > 0.0) || maxit < 1) {
+    set_error("gospl_marine_dep: Gmar and dt must be positive and maxit >= 1");
+    return 1;
+  }
+  hipStream_t s = stream;
+  const int64_t N = m->N;
+  const int g = nblocks(N);
+  Work wk;
+  if (ensure_work(m, &wk)) return 1;
+  double *f = wk.vec[0], *d = wk.vec[1], *sv = wk.vec[2], *r1 = wk.vec[3];
+  k_mdep_prep<<<g, GOSPL_BLOCK, 0, s>>>(N, Gmar, sealevel, dt, m->area, h, vol, f, d, sv, wk.b, r1);
+  GOSPL_CHECK(hipGetLastError());
+  int64_t bad = 0;
+  if (dag_solve_ext(m, 0, GOSPL_MAXW, rcv, wgt, nullptr, r1, wk.y, &bad, s)) return 1;
+  if (bad) {
+    set_error("gospl_marine_dep: the routing sweep left %lld vertices on an undrained receiver cycle",
+              (long long)bad);
+    return 1;
+  }
+  k_mdep_rhs<<<g, GOSPL_BLOCK, 0, s>>>(N, d, wk.y, wk.b);
+  GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipMemcpyAsync(wk.x, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  Coupled c{0, nullptr, nullptr, nullptr, GOSPL_MAXW, rcv, wgt, f, d, sv};
+  int it = 0;
+  double rr = 0.0;
+  if (gmres(m, c, wk, wk.b, wk.x, rtol, maxit, &it, &rr, s)) return 1;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  k_mdep_vol<<<g, GOSPL_BLOCK, 0, s>>>(N, wk.x, h, m->area, voldep);
+  GOSPL_CHECK(hipGetLastError());
+  return 0;
 }
 
 }  // extern "C"

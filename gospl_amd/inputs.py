@@ -20,7 +20,8 @@ per-step surface-process path uses, with goSPL's defaults
   ``hillslopeKm``, ``oFill`` (-6000), and the lake-deposit keys
   ``nonlinKm`` (10), ``nlPitVolume`` (1e9), ``nlPitDepth`` (100), ``nlPitK``,
   ``pitInletBias`` (0.1), ``marineSolver``, ``picardSub`` (10),
-  ``picardIts`` (2);
+  ``picardIts`` (2), and the marine keys ``clinSlp`` (1e-6), ``offshore``
+  (1e7 m), ``Gmar`` (0);
 * ``sea`` (`_readSealevel`, :710-787): ``position`` (0);
 * ``climate`` (`_readRain`, :1352-1520): events ``{start, uniform | map:
   [file, key]}``;
@@ -32,9 +33,10 @@ per-step surface-process path uses, with goSPL's defaults
 Everything else in goSPL's input describes subsystems outside this path and
 is ignored.  Settings that would change what this path computes and that it
 does not implement raise ``NotImplementedError`` (a
-sea-level curve, open or cyclic planar edges, marine deposition of river
-sediment); n != 1 selects the non-linear stream-power solver, as goSPL's
-runProcesses does; ``nodep: False`` runs continental sediment routing.  Relative file names are resolved
+sea-level curve, open or cyclic planar edges); n != 1 selects the non-linear
+stream-power solver, as goSPL's runProcesses does; ``nodep: False`` runs
+continental sediment routing, and with ``seadepo: True`` the marine
+deposition of what reaches the sea.  Relative file names are resolved
 against the YAML file's folder, then the working directory, as goSPL opens
 them from the working directory.
 """
@@ -102,10 +104,6 @@ def read_input(filename):
         raise ValueError("domain.bc must be 4 characters from {o, f, w, c}; got %r" % dom.get("bc"))
     nodep = bool(dom.get("nodep", False))
     seadepo = bool(dom.get("seadepo", True))
-    if not nodep and seadepo:
-        raise NotImplementedError("domain.nodep is False with domain.seadepo True (goSPL's defaults): marine "
-                                  "deposition of river sediment (seaChange) is not on this path; set seadepo: "
-                                  "False to route and deposit sediment on the continents only, or nodep: True")
     tdict = inp.get("time")
     if tdict is None:
         raise KeyError("Key time is required in the input file!")
@@ -134,7 +132,8 @@ def read_input(filename):
                  nlPitK=float(hill["nlPitK"]) if hill.get("nlPitK") is not None else None,
                  pitInletBias=float(hill.get("pitInletBias", 0.10 if "diffusion" in inp else 0.50)),
                  marineSolver=str(hill.get("marineSolver", "ts")), picardSub=int(hill.get("picardSub", 10)),
-                 picardIts=int(hill.get("picardIts", 2)))
+                 picardIts=int(hill.get("picardIts", 2)), clinSlp=float(hill.get("clinSlp", 1.0e-6)),
+                 offshore=float(hill.get("offshore", 100.0e5)), Gmar=float(hill.get("Gmar", 0.0)))
     sea = inp.get("sea") or {}
     if sea.get("curve") is not None:
         raise NotImplementedError("sea.curve: a time-varying sea level is not on this path (sea.position only)")

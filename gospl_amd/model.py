@@ -33,6 +33,7 @@ import torch
 from . import _lib
 from .mesh import Mesh
 from .outmesh import WriteMesh
+from .sea import SEAMesh
 from .sed import SEDMesh
 
 MISSING_DATA_SENTINEL = -1.0e8   # gospl/tools/constants.py:34
@@ -104,7 +105,7 @@ class DeviceMesh:
             pass
 
 
-class Model(SEDMesh, WriteMesh):
+class Model(SEDMesh, WriteMesh, SEAMesh):
     """Per-step hot path of goSPL on one GPU (FP64 state).
 
     ``Model.sweeps`` counts, process-wide, the exact sweeps launched by every
@@ -137,7 +138,8 @@ class Model(SEDMesh, WriteMesh):
                  refine_rtol=1.0e-12, refine_maxpass=4, showlog=False, profile=False, schedule=None,
                  outputDir=None, makedir=False, tout=None, rstep=0, compression="gzip", finput=None,
                  nodep=True, seadepo=False, nlPitVolume=1.0e9, nlPitDepth=100.0, nlPitK=None, nonlinKm=10.0,
-                 pitInletBias=0.10, picardSub=10, picardIts=2, marineSolver="ts"):
+                 pitInletBias=0.10, picardSub=10, picardIts=2, marineSolver="ts", clinSlp=1.0e-6, offshore=100.0e5,
+                 Gmar=0.0):
         if isinstance(mesh, (str, os.PathLike)):
             # goSPL's constructor, Model(filename, verbose, showlog, profile)
             # (gospl/model.py:212-304): the YAML file and its npz mesh
@@ -254,10 +256,7 @@ class Model(SEDMesh, WriteMesh):
         self.native_flow = True
         self._fbuf = None
         self.vSedLocal = torch.zeros(N, **f64)
-        if not nodep and seadepo:
-            raise NotImplementedError("marine deposition of river sediment (seaChange, domain.seadepo) is not on "
-                                      "this path: set seadepo: False")
-        self.seaDepo = bool(seadepo)
+        self._init_sea(seadepo=seadepo, clinSlp=clinSlp, offshore=offshore, Gmar=Gmar)
         self._init_sed(nodep=nodep, nlPitVolume=nlPitVolume, nlPitDepth=nlPitDepth, nlPitK=nlPitK,
                        nonlinKm=nonlinKm, pitInletBias=pitInletBias, picardSub=picardSub, picardIts=picardIts,
                        marineSolver=marineSolver)
@@ -452,6 +451,7 @@ class Model(SEDMesh, WriteMesh):
                    _ptr(self.lFill), 1 if down else 0, _ptr(self.pitIDs), _ptr(self.flatDirs),
                    _ptr(self.rcvID), _ptr(self.distRcv), _ptr(self.wghtVal), _ptr(self.lsink),
                    self.stream)
+        self._snapshot_sea()
 
     def _solve_KSP(self, guess, matrix, vector1, vector2, fatal=False, seed=False):
         """goSPL's flow solve (flowplex.py:323-427) with its signature:
@@ -786,7 +786,10 @@ class Model(SEDMesh, WriteMesh):
         self.erodepSPL()
         if not self.nodep:
             self.flowAccumulation()
+            self._snapshot_sea()
             self.sedChange()
+            if self.seaDepo:
+                self.seaChange()
         self.getHillslope()
         if self.tNow < self.tEnd:
             self.applyForces()

@@ -622,6 +622,75 @@ int gospl_sed_scale(gospl_mesh_t* mesh, const int32_t* pitids, const int8_t* sel
 int gospl_sed_apply(gospl_mesh_t* mesh, const double* delta, double* h, double* cumed, gospl_stream_t stream);
 
 /*
+ * Marine deposition of river sediment: SEAMesh.seaChange
+ * (gospl/sed/seaplex.py:738-861) and its helpers.  All arrays on the device;
+ * xyz is (N, 3) row-major; rcv / wgt are (12, N).
+ *  - gospl_coast_distance: _distanceCoasts / _globalCoastsTree (:51-135).
+ *    The coastline points are VTK's contour of z at `level` on the triangle
+ *    mesh (one point per edge with one end below the level and the other at
+ *    or above it, x_lo + t (x_hi - x_lo), t = (level - z_lo) / (z_hi - z_lo));
+ *    dist = Euclidean distance of each vertex with mask != 0 (mask may be
+ *    NULL: all) to the nearest point (goSPL: a k-d tree query), 0 for the
+ *    others and everywhere when there is no point.  *npoints (HOST, may be
+ *    NULL): the point count.  Synchronises `stream`.
+ *  - gospl_eps_fill: epsfill (functions.F90:2633-2697): cells below `level`
+ *    keep h; a cell at or above it reached from a seed (a cell below the
+ *    level next to one at or above it) becomes max(h, the next double above
+ *    the least fill of its neighbours), as goSPL's priority flood leaves it;
+ *    unreached cells keep h.  *passes (HOST, may be NULL).  Synchronises.
+ *  - gospl_diffuse_fixed: the implicit linear diffusion (I + L(kd)) x = rhs of
+ *    hillSLP._buildDiffMat / _solveSmooth (hillslope.py:131-150, 235-257,
+ *    373-402) with per-vertex kd (sethillslopecoeff, identity rows at
+ *    flat-mesh outlets), Jacobi sweeps from x = rhs to their bitwise fixed
+ *    point (goSPL: Richardson to rtol 1e-8), at most maxit sweeps, failing
+ *    otherwise.  *sweeps (HOST, may be NULL).  Synchronises.
+ *  - gospl_marine_receivers: mfdrcvrs (functions.F90:1673-1785) as _matOcean
+ *    calls it (:198-219): rcv = the strictly lower neighbours on fillz in
+ *    neighbour order then the vertex itself, wgt = 1 / their count in single
+ *    precision as the Fortran's `1. / kk` (0 on the padding), sink = 1 where
+ *    there is none; flat-mesh outlets: no receivers, not sinks.
+ *  - gospl_marine_prep: shelf capacity cap = (clinoH - hl) area with
+ *    clinoH = sealevel - 1e-3 - cdist clinslp, hl where hl >= sealevel or
+ *    above clinoH, and the incoming volume vin = vsed dt on cells with
+ *    lfill <= sealevel, 0 elsewhere or where negative (:761-776).
+ *  - gospl_marine_route: _distOcean (:404-619): vdep = the deposited volume
+ *    of the vin routed through (rcv, wgt) against the capacities cap; the
+ *    passes, at most 5000, stop once the volume still moving is at most
+ *    max(1, 1e-6 of its initial total).  *passes (HOST, may be NULL).
+ *    Synchronises `stream` once per 16 passes.
+ *  - gospl_marine_dep: _depMarineSystem (:240-402), the coupled solve of the
+ *    routed volumes vol with Gmar > 0: goSPL's nested system with A00 =
+ *    I - diag(fDepm), A01 = -diag(fDepm dt / area), A10 = diag(area / dt),
+ *    A11 = (I - W)^T of (rcv, wgt) (fDepm = min(Gmar, 0.99), 0 where h >
+ *    sealevel; right-hand sides (1 - fDepm) h and vol / dt + h area / dt),
+ *    reduced to its elevation block and solved by the GMRES of gospl_spl_dep
+ *    (identity preconditioner); voldep = max((hnew - h) area, 0).
+ *  - gospl_marine_thickness: dh = vdep / area (0 where area is 0), values
+ *    below floor_m set to 0 (seaplex.py:814-818).
+ *  - gospl_marine_apply: cumed += t; h += t; eblocal += t / dt (:836-847).
+ */
+int gospl_coast_distance(gospl_mesh_t* mesh, const double* xyz, const double* z, double level,
+                         const int8_t* mask, double* dist, int64_t* npoints, gospl_stream_t stream);
+int gospl_eps_fill(gospl_mesh_t* mesh, const double* h, double level, double* fillz, int32_t* passes,
+                   gospl_stream_t stream);
+int gospl_diffuse_fixed(gospl_mesh_t* mesh, const double* kd, const double* rhs, double* x, int32_t maxit,
+                        int32_t* sweeps, gospl_stream_t stream);
+int gospl_marine_receivers(gospl_mesh_t* mesh, const double* fillz, int32_t* rcv, double* wgt, int8_t* sink,
+                           gospl_stream_t stream);
+int gospl_marine_prep(gospl_mesh_t* mesh, const double* hl, const double* cdist, const double* lfill,
+                      const double* vsed, double sealevel, double clinslp, double dt, double* cap, double* vin,
+                      gospl_stream_t stream);
+int gospl_marine_route(gospl_mesh_t* mesh, const int32_t* rcv, const double* wgt, const int8_t* sink,
+                       const double* cap, const double* vin, double* vdep, int32_t* passes, gospl_stream_t stream);
+int gospl_marine_dep(gospl_mesh_t* mesh, const int32_t* rcv, const double* wgt, const double* h, const double* vol,
+                     double Gmar, double sealevel, double dt, double* voldep, double rtol, int32_t maxit,
+                     int32_t* iters, double* relres, gospl_stream_t stream);
+int gospl_marine_thickness(gospl_mesh_t* mesh, const double* vdep, double floor_m, double* dh,
+                           gospl_stream_t stream);
+int gospl_marine_apply(gospl_mesh_t* mesh, const double* t, double dt, double* h, double* cumed, double* eblocal,
+                       gospl_stream_t stream);
+
+/*
  * Non-linear thickness diffusion of a deposit over one step: replaces
  * hillSLP._diffuseImplicit in its lagged-diffusivity form
  * (_diffuseImplicitPicard, gospl/sed/hillslope.py:925-1076; PETSc KSP

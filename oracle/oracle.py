@@ -16,6 +16,7 @@ mesh order:
 * ``erodepSPLnl`` (n != 1)     gospl/eroder/nlSPL.py:321-456
 * ``getHillslope``             gospl/sed/hillslope.py:90-293, 614-649
 * ``sedChange`` (nodep False)  gospl/sed/sedplex.py:103-1086 (oracle/sed.py)
+* ``seaChange`` (seadepo True) gospl/sed/seaplex.py:51-861 (oracle/sea.py)
 * ``applyForces``              gospl/mesher/unstructuredmesh.py:743-794, 903-972
 * ``runProcesses``             gospl/model.py:331-466
 
@@ -46,6 +47,7 @@ import subprocess
 
 import numpy as np
 
+from .sea import SEA
 from .sed import SED
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -311,7 +313,7 @@ def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
 # Model-level restatement
 # --------------------------------------------------------------------------
 
-class OracleModel(SED):
+class OracleModel(SED, SEA):
     """goSPL's hot-path state machine in FP64 on the CPU (one rank).
 
     ``geom`` holds input-order arrays: ``ngb`` (N,12) int32 (-1 pad), ``nnb``,
@@ -371,7 +373,9 @@ class OracleModel(SED):
         self.tmp = np.zeros(N)
         self.bL = self.rainVal * self.larea
         self.coords = None if geom.get("coords") is None else np.asarray(geom["coords"], np.float64)
+        seakw = {k: sedkw.pop(k) for k in ("seadepo", "clinSlp", "offshore", "Gmar") if k in sedkw}
         self._init_sed(nodep=nodep, **sedkw)
+        self._init_sea(**seakw)
 
     # ---------------- pit filling (pitfilling.py) ----------------
     def fillElevation(self, sed=False):
@@ -796,6 +800,8 @@ class OracleModel(SED):
         if not self.nodep:
             self.flowAccumulation()
             self.sedChange()
+            if self.seaDepo:
+                self.seaChange()
         self.getHillslope()
         if self.tNow < self.tEnd:
             self.applyForces()

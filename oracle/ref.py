@@ -239,3 +239,35 @@ def fctcoeff(h, kd):
     out = np.zeros(nb)
     L.fctcoeff_(_p(a[0]), _p(a[1]), _p(out), _i(nb))
     return out
+
+
+def globalngbhs(nt, cells):
+    """functions.F90:2539 — the global neighbour table epsfill reads
+    (cells (n, 3), 0-based)."""
+    L = lib()
+    c = np.asfortranarray(cells, dtype=np.int32)
+    L.globalngbhs_(_i(nt), _p(c), _i(len(c)))
+
+
+def epsfill(sl, elev):
+    """functions.F90:2633 — the priority flood with epsilon increments
+    (globalngbhs must have run on the same mesh)."""
+    L = lib()
+    nb = len(elev)
+    e = np.ascontiguousarray(elev, dtype=np.float64)
+    out = np.zeros(nb)
+    L.epsfill_(_d(sl), _p(e), _p(out), _i(nb))
+    return out
+
+
+def mfdrcvrs(nrcv, exp, elev, sl, gid):
+    """functions.F90:1673 — marine receivers: (rcv, dist, wgt) as (N, 12)."""
+    L = lib()
+    nb = len(elev)
+    e = np.ascontiguousarray(elev, dtype=np.float64)
+    g = np.ascontiguousarray(gid, dtype=np.int32)
+    rcv = np.zeros((nb, 12), dtype=np.int32, order="F")
+    dist = np.zeros((nb, 12), order="F")
+    wgt = np.zeros((nb, 12), order="F")
+    L.mfdrcvrs_(_i(nrcv), _d(exp), _p(e), _d(sl), _p(g), _p(rcv), _p(dist), _p(wgt), _i(nb))
+    return np.ascontiguousarray(rcv), np.ascontiguousarray(dist), np.ascontiguousarray(wgt)

@@ -1,0 +1,309 @@
+"""CPU FP64 restatement of goSPL's marine deposition of river sediment
+(SEAMesh.seaChange).
+
+TEST INFRASTRUCTURE ONLY: mixed into ``oracle.OracleModel``; the product
+(``gospl_amd``) never imports it.
+
+Restates, in input-mesh order and one rank (no stratigraphy, dual lithology
+or provenance, which are off on this path):
+
+* ``_distanceCoasts``   gospl/sed/seaplex.py:51-135: the coastline is VTK's
+  contour of the elevation at sea level on the triangle mesh (vtkContourFilter
+  with vtkTriangle's interpolation: a point on every edge whose ends straddle
+  the level, interpolated from the lower end), restated here; the distances
+  come from scipy's cKDTree as in goSPL;
+* ``_matOcean``         :137-238, with ``_hillSlope(smooth=2)``
+  (gospl/sed/hillslope.py:131-150, Kd = N area, N = 1 on land and 5 at sea)
+  solved to the Jacobi fixed point, ``epsfill`` (functions.F90:2633-2697) as a
+  priority flood and ``mfdrcvrs`` (:1673-1785) as goSPL calls it;
+* ``_distOcean``        :404-619 (without provenance), the push of each pass
+  summed per receiver in increasing input id;
+* ``_depMarineSystem``  :240-402, goSPL's nested system solved by sparse LU
+  (goSPL: TFQMR with a Schur fieldsplit to rtol 1e-8);
+* ``seaChange``         :738-861, with the marine diffusion of
+  ``_diffuseOcean`` (hillslope.py:739-786) by the Picard form of oracle/sed.py.
+
+Pins (tests/test_oracle_sea_pins.py): epsfill and mfdrcvrs against goSPL's
+own Fortran (oracle/_ref) on random surfaces; the contour points against
+their definition (each on a straddling edge at the level, one per edge) and
+the distances against brute force; the routing's volume balance (input =
+deposits + what left through outlets); the coupled solve's flux block balance
+and its Gmar -> 0 limit.
+"""
+
+from __future__ import annotations
+
+import heapq
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+BOUNDARY_FLOW_SENTINEL = -1.0e6   # gospl/tools/constants.py:90
+DEPOSIT_FLOOR = 1.0e-3            # gospl/tools/constants.py:58
+MARINE_SMOOTH_N_LAND = 1.0        # gospl/tools/constants.py:126
+MARINE_SMOOTH_N_SEA = 5.0         # gospl/tools/constants.py:127
+
+
+def contour_points(z, level, coords, ngb, nnb, with_edges=False):
+    """VTK's contour of z at `level` on the mesh edges: for each edge (a, b)
+    with z_a < level <= z_b the point x_a + t (x_b - x_a), t = (level - z_a)
+    / (z_b - z_a) (vtkTriangle::Contour interpolates from the lower end)."""
+    pts = []
+    edges = []
+    for a in range(len(z)):
+        za = z[a]
+        if not za < level:
+            continue
+        for p in range(nnb[a]):
+            b = ngb[a, p]
+            if b < 0 or not z[b] >= level:
+                continue
+            t = (level - za) / (z[b] - za)
+            pts.append(coords[a] + t * (coords[b] - coords[a]))
+            edges.append((a, b))
+    pts = np.array(pts, dtype=np.float64).reshape(-1, 3)
+    if with_edges:
+        return pts, np.array(edges, dtype=np.int64).reshape(-1, 2)
+    return pts
+
+
+def epsfill(sl, elev, ngb, nnb):
+    """epsfill (functions.F90:2633-2697): cells below sl are flagged; those
+    next to a cell at or above it seed a priority flood in which a popped
+    cell raises each unflagged neighbour to at least the next double above
+    its own fill."""
+    N = len(elev)
+    fz = np.array(elev, dtype=np.float64)
+    flag = fz < sl
+    pq = []
+    for i in range(N):
+        if fz[i] < sl:
+            for k in range(nnb[i]):
+                c = ngb[i, k]
+                if c >= 0 and fz[c] >= sl:
+                    heapq.heappush(pq, (fz[i], i))
+                    break
+    while pq:
+        _, i = heapq.heappop(pq)
+        h = np.nextafter(fz[i], np.inf)
+        for k in range(nnb[i]):
+            c = ngb[i, k]
+            if c >= 0 and not flag[c]:
+                flag[c] = True
+                if not fz[c] > h:
+                    fz[c] = h
+                heapq.heappush(pq, (fz[c], c))
+    return fz
+
+
+def marine_receivers(fz, ngb, nnb, elgt, outlets=None):
+    """mfdrcvrs (functions.F90:1673-1785) with 12 receivers and the -1e6
+    sentinel as sea level (seaplex.py:186-193): the lower neighbours (edge
+    length > 0, (z_k - z_n)^fexp / length > 0) in neighbour order, weight 1 /
+    their count
+    -- computed as the Fortran writes it, ``1. / kk``, a default-real (single
+    precision) quotient stored in a double -- and the vertex itself on the
+    remaining slots; flat-mesh outlets get no receivers (:196-198).  Returns
+    (rcv (N, 12), wgt (N, 12), sink (N,))."""
+    N = len(fz)
+    rcv = np.repeat(np.arange(N, dtype=np.int32)[:, None], 12, axis=1)
+    wgt = np.zeros((N, 12))
+    for i in range(N):
+        # val = (z_k - z_n)^fexp / L > 0 (fexp = 1.1 above the sentinel, 0.01 at
+        # it, :1709-1729): a drop whose power underflows does not count
+        fexp = 1.1 if fz[i] > BOUNDARY_FLOW_SENTINEL else 0.01
+        ids = []
+        for p in range(nnb[i]):
+            n = ngb[i, p]
+            if n < 0 or not elgt[i, p] > 0.0:
+                continue
+            d = fz[i] - fz[n]
+            with np.errstate(invalid="ignore"):
+                val = np.power(d, fexp) / elgt[i, p]
+            if val > 0.0:
+                ids.append(n)
+        for k, n in enumerate(ids):
+            rcv[i, k] = n
+            wgt[i, k] = np.float64(np.float32(1.0) / np.float32(len(ids)))
+    if outlets is not None and len(outlets):
+        rcv[outlets, :] = np.asarray(outlets)[:, None]
+        wgt[outlets, :] = 0.0
+    sink = np.all(rcv == np.arange(N)[:, None], axis=1)
+    if outlets is not None and len(outlets):
+        sink[outlets] = False
+    return rcv, wgt, sink
+
+
+def push_down(mv, rcv, wgt):
+    """dMat1.mult (seaplex.py:514): moved[r] = sum over the donors i with r
+    among their receivers of mv[i] wgt; each receiver's donors added in
+    increasing input id."""
+    N, K = rcv.shape
+    don = np.repeat(np.arange(N), K)
+    r = rcv.ravel()
+    w = wgt.ravel()
+    keep = (r != don) & (w != 0.0)
+    don, r, w = don[keep], r[keep], w[keep]
+    order = np.lexsort((don, r))
+    moved = np.zeros(N)
+    for r_, d_, w_ in zip(r[order], don[order], w[order]):
+        moved[r_] += mv[d_] * w_
+    return moved
+
+
+def marine_dep_solve(rcv, wgt, h, vol, Gmar, sealevel, area, dt):
+    """_depMarineSystem (seaplex.py:240-402): goSPL's nested system
+
+        [I - diag(fDepm)   -diag(fDepm dt / area)] [h']   [(1 - fDepm) h            ]
+        [diag(area / dt)    (I - W)^T            ] [q ] = [vol / dt + h area / dt ]
+
+    (fDepm = min(Gmar, 0.99), 0 where h > sea level; W the marine receivers)
+    solved by sparse LU.  Returns (h', q, volDep = max((h' - h) area, 0))."""
+    from .oracle import flow_matrix
+
+    N = len(h)
+    fDepm = np.full(N, Gmar)
+    fDepm[fDepm > 0.99] = 0.99
+    fDepm[h > sealevel] = 0.0
+    A00 = sp.identity(N) + sp.diags(-fDepm)
+    A01 = sp.diags(-fDepm * dt / area)
+    A10 = sp.diags(area / dt)
+    A11 = flow_matrix(rcv, wgt)
+    M = sp.bmat([[A00, A01], [A10, A11]], format="csc")
+    r0 = (1.0 - fDepm) * h
+    r1 = vol / dt + (h * area) * (1.0 / dt)
+    x = spla.spsolve(M, np.concatenate([r0, r1]))
+    volDep = (x[:N] - h) * area
+    volDep[volDep < 0] = 0.0
+    return x[:N], x[N:], volDep
+
+
+class SEA:
+    """goSPL's SEAMesh for the oracle model."""
+
+    def _init_sea(self, seadepo=False, clinSlp=1.0e-6, offshore=100.0e5, Gmar=0.0):
+        """The marine keys of goSPL's input (inputparser.py:546-597)."""
+        self.seaDepo = bool(seadepo)
+        self.clinSlp = max(1.0e-6, float(clinSlp))
+        self.offshore = float(offshore)
+        self.Gmar = float(Gmar)
+        self.route_passes = []
+
+    # ----------------------------------------------------------- coastline
+    def _distanceCoasts(self, data):
+        from scipy import spatial
+
+        self.coastDist = np.zeros(self.lpoints)
+        pts = contour_points(data, self.sealevel, self.coords, self.ngb, self.nnb)
+        self.coast_points = pts
+        if len(pts) == 0:
+            return
+        tree = spatial.cKDTree(pts, leafsize=10)
+        self.coastDist[self.seaID], _ = tree.query(self.coords[self.seaID, :], k=1)
+
+    # ------------------------------------------------------------ routing
+    def _smoothMarine(self):
+        """_hillSlope(smooth=2) (hillslope.py:131-150): the marine smoothing
+        of h, solved to the Jacobi fixed point."""
+        from .oracle import hillslope_solve, sethillslopecoeff
+
+        Cd = np.full(self.lpoints, MARINE_SMOOTH_N_LAND)
+        Cd[self.seaID] = MARINE_SMOOTH_N_SEA
+        coeff = sethillslopecoeff(Cd * self.larea, self.ngb, self.nnb, self.larea, self.elgt, self.vdist)
+        if self.flatModel:
+            coeff[self.outletIDs, 1:] = 0.0
+            coeff[self.outletIDs, 0] = 1.0
+        hs, _ = hillslope_solve(coeff, self.h.copy(), self.ngb, self.nnb)
+        return hs
+
+    def _matOcean(self):
+        hl = self.h.copy()
+        minh = hl.min() + 0.1
+        if not self.flatModel:
+            minh = max(minh, self.oFill)
+            hsmth = self._smoothMarine()
+            hsmth[self.coastDist > self.offshore] = BOUNDARY_FLOW_SENTINEL
+        else:
+            minh = max(minh, self.oFill)
+            hsmth = hl.copy()
+            hsmth[self.outletIDs] = BOUNDARY_FLOW_SENTINEL
+        self.hsmth = hsmth
+        fillz = epsfill(minh, hsmth, self.ngb, self.nnb)
+        if not self.flatModel:
+            far = self.coastDist > self.offshore
+            fillz[far] = hl[far]
+        self.fillz = fillz
+        self.marRcv, self.marWgt, self.is_sink = marine_receivers(
+            fillz, self.ngb, self.nnb, self.elgt, self.outletIDs if self.flatModel else None)
+
+    def _distOcean(self, sedflux):
+        marVol = self.maxDepQs.copy()
+        sinkVol = sedflux.copy()
+        vdep = np.zeros(self.lpoints)
+        sink = self.is_sink
+        if sink.any():
+            vdep[sink] += sinkVol[sink]
+            sinkVol[sink] = 0.0
+        tmp = sinkVol
+        sumExcess = tmp.sum()
+        threshold = max(1.0, 1.0e-6 * sumExcess)
+        step = 0
+        while sumExcess > threshold:
+            if step >= 5000:
+                break
+            sinkVol = push_down(tmp, self.marRcv, self.marWgt)
+            excess = sinkVol >= marVol
+            sinkVol[excess] -= marVol[excess]
+            vdep[excess] += marVol[excess]
+            marVol[excess] = 0.0
+            noexcess = np.invert(excess)
+            marVol[noexcess] -= sinkVol[noexcess]
+            vdep[noexcess] += sinkVol[noexcess]
+            vdep[self.outletIDs] = 0.0
+            sinkVol[noexcess] = 0.0
+            sinkVol[self.outletIDs] = 0.0
+            if sink.any():
+                vdep[sink] += sinkVol[sink]
+                sinkVol[sink] = 0.0
+            tmp = sinkVol
+            sumExcess = tmp.sum()
+            step += 1
+        self.route_passes.append(step)
+        if tmp.any():
+            vdep += tmp
+            vdep[self.outletIDs] = 0.0
+        return vdep
+
+    def _depMarineSystem(self, sedflux):
+        self.newH, self.marFlux, volDep = marine_dep_solve(self.marRcv, self.marWgt, self.h, sedflux, self.Gmar,
+                                                           self.sealevel, self.larea, self.dt)
+        return volDep
+
+    def seaChange(self):
+        """SEAMesh.seaChange (seaplex.py:738-861)."""
+        self.sinkIDs = self.lFill <= self.sealevel
+        hl = self.h.copy()
+        self._distanceCoasts(hl)
+        self.clinoH = self.sealevel - 1.0e-3 - self.coastDist * self.clinSlp
+        self.clinoH[hl >= self.sealevel] = hl[hl >= self.sealevel]
+        self.clinoH[self.clinoH < hl] = hl[self.clinoH < hl]
+        self.maxDepQs = (self.clinoH - hl) * self.larea
+        sedFlux = self.vSedLocal.copy() * self.dt
+        sedFlux[np.invert(self.sinkIDs)] = 0.0
+        sedFlux[sedFlux < 0] = 0.0
+        self.marineInput = sedFlux
+        self._matOcean()
+        marDep = self._distOcean(sedFlux)
+        if not self.flatModel and self.Gmar > 0.0:
+            vdep = self._depMarineSystem(marDep)
+            marDep = self._distOcean(vdep)
+        self.marDep = marDep
+        dh = np.divide(marDep, self.larea, out=np.zeros_like(self.larea), where=self.larea != 0)
+        dh[dh < DEPOSIT_FLOOR] = 0.0
+        self.marThick = dh
+        ndepo = self._diffuseImplicit(dh, self.seaID, self.nlK)
+        self.tmp = ndepo
+        self.cumED = self.cumED + ndepo
+        self.h = self.h + ndepo
+        self.EbLocal = self.EbLocal + ndepo / self.dt

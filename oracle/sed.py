@@ -78,7 +78,7 @@ class SED:
     """goSPL's SEDMesh glue for the oracle model."""
 
     def _init_sed(self, nodep=True, nlPitVolume=1.0e9, nlPitDepth=100.0, nlPitK=None, nonlinKm=10.0,
-                  pitInletBias=0.10, picardSub=10, picardIts=2, marineSolver="ts", seadepo=False):
+                  pitInletBias=0.10, picardSub=10, picardIts=2, marineSolver="ts"):
         self.nodep = bool(nodep)
         self.nlK = float(nonlinKm)
         self.nl_pit_volume = float(nlPitVolume)

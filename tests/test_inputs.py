@@ -69,19 +69,24 @@ def test_read_input_keys_and_defaults(tmp_path):
 
 
 def test_sediment_routing_keys(tmp_path):
-    """nodep False routes sediment; goSPL's default seadepo True (marine
-    deposition) is refused; the lake-deposit diffusion keys are read with
-    goSPL's defaults."""
+    """nodep False routes sediment, with goSPL's default seadepo True
+    (marine deposition) unless it is set False; the lake-deposit and marine
+    diffusion keys are read with goSPL's defaults."""
     fn, *_ = _write_case(tmp_path)
     text = fn.read_text().replace("  nodep: True\n", "")
     fn.write_text(text)
-    with pytest.raises(NotImplementedError, match="seadepo"):
-        inputs.read_input(str(fn))
+    inp = inputs.read_input(str(fn))
+    assert inp["nodep"] is False and inp["seadepo"] is True
     fn.write_text(text.replace("  bc: 'ffff'\n", "  bc: 'ffff'\n  seadepo: False\n"))
     inp = inputs.read_input(str(fn))
     assert inp["nodep"] is False and inp["seadepo"] is False
     assert inp["sed"] == dict(nonlinKm=10.0, nlPitVolume=1.0e9, nlPitDepth=100.0, nlPitK=None, pitInletBias=0.10,
-                              marineSolver="ts", picardSub=10, picardIts=2)
+                              marineSolver="ts", picardSub=10, picardIts=2, clinSlp=1.0e-6, offshore=100.0e5,
+                              Gmar=0.0)
+    fn.write_text(text.replace("  hillslopeKm: 0.2\n", "  hillslopeKm: 0.2\n  clinSlp: 1.e-5\n  Gmar: 0.5\n"
+                                                          "  offshore: 5.e5\n"))
+    inp = inputs.read_input(str(fn))
+    assert (inp["sed"]["clinSlp"], inp["sed"]["Gmar"], inp["sed"]["offshore"]) == (1.0e-5, 0.5, 5.0e5)
     fn.write_text(text.replace("  bc: 'ffff'\n", "  bc: 'ffff'\n  seadepo: False\n").replace(
         "  hillslopeKm: 0.2\n", "  hillslopeKm: 0.2\n  nlPitK: 3.\n  marineSolver: picard\n  picardSub: 4\n"))
     inp = inputs.read_input(str(fn))

@@ -557,18 +557,41 @@ extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const doub
   k_hill_coef_kd<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, kd, op);
   if (m->flat) k_hill_outlet_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->outlet, op);
   GOSPL_CHECK(hipGetLastError());
-  // Jacobi from x = rhs to the bitwise fixed point, the change measured every 16th sweep
+  // Jacobi from x = rhs until a sweep changes nothing (the change measured
+  // every 16th sweep) or until the error bound rho^k / (1 - rho) of the
+  // contraction rho = max (d - 1) / d is below 2^-53: on large meshes the
+  // last bits of a few rows can cycle for ever.  The cap is the least k with
+  // rho^k <= 2^-53 (1 - rho) (repeated products), rounded up to a multiple of
+  // 16, plus 64.
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
+  unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
+  GOSPL_CHECK(hipMemsetAsync(keys + 2, 0, sizeof(unsigned long long), s));
+  k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, op, keys + 2);
+  GOSPL_CHECK(hipGetLastError());
+  GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  GOSPL_CHECK(hipStreamSynchronize(s));
+  const double rho = hk[2] ? dkey_inv_host(hk[2]) : 0.0;
+  int limit = maxit;
+  if (rho < 1.0) {
+    const double tgt = 0x1p-53 * (1.0 - rho);
+    double pw = 1.0;
+    int k = 0;
+    while (pw > tgt && k < maxit) {
+      pw *= rho;
+      ++k;
+    }
+    const int cap = 16 * ((k + 15) / 16) + 64;
+    limit = cap < maxit ? cap : maxit;
+  }
   double* xa = m->wd[1];
   double* xb = m->wd[2];
   GOSPL_CHECK(hipMemcpyAsync(xa, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   double* xin = xa;
   double* xout = xb;
-  unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
-  unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
   int done = 0;
   bool fixed = false;
-  while (!fixed && done < maxit) {
-    for (int k = 0; k < 15 && done < maxit - 1; ++k, ++done) {
+  while (!fixed && done < limit) {
+    for (int k = 0; k < 15 && done < limit - 1; ++k, ++done) {
       k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, xin, xout);
       std::swap(xin, xout);
     }
@@ -583,8 +606,8 @@ extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const doub
     if (!(chg == chg)) break;
     fixed = chg == 0.0;
   }
-  if (!fixed) {
-    set_error("gospl_diffuse_fixed: Jacobi sweeps reached no fixed point within %d sweeps", (int)maxit);
+  if (!fixed && !(rho < 1.0 && done == limit)) {
+    set_error("gospl_diffuse_fixed: Jacobi sweeps did not converge within %d sweeps (contraction %.6g)", done, rho);
     return 1;
   }
   if (sweeps) *sweeps = done;

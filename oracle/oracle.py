@@ -297,6 +297,37 @@ def nl_transport_solve(rcv, wgt, dist, kbr, n, fdep, area, dt, hold, rcv_f=None,
     raise RuntimeError("non-linear transport-limited oracle: Newton did not converge in %d steps" % maxit)
 
 
+def jacobi_sweep_cap(diag):
+    """Sweeps after which the Jacobi error bound rho^k / (1 - rho) is below
+    2^-53 (rho = max over rows of (d - 1) / d): the least such k from
+    repeated products, rounded up to a multiple of 16, plus 64."""
+    d = np.asarray(diag, np.float64)
+    d = d[d > 0.0]
+    rho = max(float(((d - 1.0) / d).max()), 0.0) if len(d) else 0.0
+    if not rho < 1.0:
+        return None
+    tgt = 2.0 ** -53 * (1.0 - rho)
+    pw, k = 1.0, 0
+    while pw > tgt:
+        pw *= rho
+        k += 1
+    return 16 * ((k + 15) // 16) + 64
+
+
+def smooth_solve(coeff, rhs, ngb, nnb):
+    """Jacobi sweeps from x = rhs until one changes nothing or for
+    jacobi_sweep_cap sweeps (the last bits of a few rows can cycle on large
+    meshes).  Returns (x, sweeps)."""
+    cap = jacobi_sweep_cap(coeff[:, 0])
+    if cap is None:
+        raise RuntimeError("Jacobi sweeps do not contract")
+    N = len(rhs)
+    x = np.empty(N)
+    it = lib().or_hillslope_solve(N, _p(ngb), _p(nnb), _p(_c(coeff, np.float64)), _p(_c(rhs, np.float64)),
+                                  _p(x), int(cap))
+    return x, min(int(it), cap)
+
+
 def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):
     """Jacobi sweeps to the bitwise fixed point; raises when ``maxit`` sweeps
     do not reach it (goSPL's Richardson solve would stop at rtol 1e-8)."""

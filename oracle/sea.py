@@ -14,7 +14,7 @@ or provenance, which are off on this path):
   come from scipy's cKDTree as in goSPL;
 * ``_matOcean``         :137-238, with ``_hillSlope(smooth=2)``
   (gospl/sed/hillslope.py:131-150, Kd = N area, N = 1 on land and 5 at sea)
-  solved to the Jacobi fixed point, ``epsfill`` (functions.F90:2633-2697) as a
+  by Jacobi sweeps to their fixed point or contraction bound, ``epsfill`` (functions.F90:2633-2697) as a
   priority flood and ``mfdrcvrs`` (:1673-1785) as goSPL calls it;
 * ``_distOcean``        :404-619 (without provenance), the push of each pass
   summed per receiver in increasing input id;
@@ -205,8 +205,9 @@ class SEA:
     # ------------------------------------------------------------ routing
     def _smoothMarine(self):
         """_hillSlope(smooth=2) (hillslope.py:131-150): the marine smoothing
-        of h, solved to the Jacobi fixed point."""
-        from .oracle import hillslope_solve, sethillslopecoeff
+        of h by Jacobi sweeps to their fixed point or their contraction
+        bound (oracle.smooth_solve)."""
+        from .oracle import sethillslopecoeff, smooth_solve
 
         Cd = np.full(self.lpoints, MARINE_SMOOTH_N_LAND)
         Cd[self.seaID] = MARINE_SMOOTH_N_SEA
@@ -214,7 +215,7 @@ class SEA:
         if self.flatModel:
             coeff[self.outletIDs, 1:] = 0.0
             coeff[self.outletIDs, 0] = 1.0
-        hs, _ = hillslope_solve(coeff, self.h.copy(), self.ngb, self.nnb)
+        hs, self.smooth_sweeps = smooth_solve(coeff, self.h.copy(), self.ngb, self.nnb)
         return hs
 
     def _matOcean(self):

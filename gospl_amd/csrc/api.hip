@@ -339,6 +339,9 @@ static int mesh_alloc(gospl_mesh* m) {
   for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
+  GOSPL_CHECK(hipMalloc(&m->red_part, sizeof(unsigned long long) * 2 * (nblocks(N) + 64)));
+  GOSPL_CHECK(hipMalloc(&m->red_cnt, sizeof(unsigned int) * 64));
+  GOSPL_CHECK(hipMemset(m->red_cnt, 0, sizeof(unsigned int) * 64));
   GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
   // rocPRIM workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
   size_t bytes = 0, b2 = 0;
@@ -459,7 +462,11 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->kry_buf);
   dfree(m->kry_d);
   if (m->kry_h) (void)hipHostFree(m->kry_h);
+  dfree(m->red_part);
+  dfree(m->red_cnt);
   dfree(m->sea_d);
+  dfree(m->sea_don);
+  dfree(m->sea_nd);
   dfree(m->sea_ctl);
   dfree(m->sea_pts);
   dfree(m->sea_i);

@@ -117,6 +117,10 @@ struct gospl_mesh {
   int32_t* sea_cell;          // grid cell of each coastline point
   int64_t sea_pcap;           // points the two hold
   unsigned long long* sea_h;  // pinned host block of the marine read-backs
+  int32_t* sea_don;           // marine routing: donors of each vertex (W, N), by input id
+  int8_t* sea_nd;             // their count
+  unsigned long long* red_part;  // per-workgroup partial maxima of the convergence tests
+  unsigned int* red_cnt;         // their completion counter (0 between launches)
 };
 
 namespace gospl {

@@ -313,78 +313,82 @@ __global__ __launch_bounds__(256) void k_route_check(const double* __restrict__ 
   if (!(s > tot[2]) || ctl[1] >= ROUTE_MAX_PASSES) ctl[0] = 1;
 }
 
-// moved[r] = sum over the donors i of r of mv[i] w_i (dMat1.mult), donors in
-// increasing input id; a donor is a neighbour that has r among its receivers
-// (gospl_marine_receivers: a vertex's receivers come first, then itself)
-__global__ void k_route_push(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
-                             const int32_t* __restrict__ gid, const int32_t* __restrict__ rcv,
-                             const double* __restrict__ wgt, const double* __restrict__ mv,
-                             double* __restrict__ moved, const int32_t* __restrict__ ctl) {
-  if (ctl[0]) return;
+// The donors of each vertex r -- the neighbours that have r among their
+// receivers (gospl_marine_receivers: a vertex's receivers come first, then
+// itself) -- sorted by input id, and each vertex's uniform weight: built once
+// per routing call, read by every pass.
+__global__ void k_route_donors(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                               const int32_t* __restrict__ gid, const int32_t* __restrict__ rcv,
+                               const double* __restrict__ wgt, int32_t* __restrict__ don, int8_t* __restrict__ nd,
+                               double* __restrict__ wv) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
-  int don[GOSPL_MAXW];
-  double val[GOSPL_MAXW];
-  int nd = 0;
+  int dl[GOSPL_MAXW];
+  int n_d = 0;
   const int dg = deg[r];
   for (int p = 0; p < W && p < dg; ++p) {
     const int n = nbr[(int64_t)p * N + r];
     if (n < 0) continue;
-    // r is a receiver of n: find its slot (the receivers are n's lower neighbours)
     for (int k = 0; k < GOSPL_MAXW; ++k) {
       const int q = rcv[(int64_t)k * N + n];
       if (q == (int)n) break;  // past the last receiver
       if (q == (int)r) {
-        const double w = wgt[(int64_t)k * N + n];
         // insertion by input id
-        int j = nd++;
+        int j = n_d++;
         const int gn = gid[n];
-        while (j > 0 && gid[don[j - 1]] > gn) {
-          don[j] = don[j - 1];
-          val[j] = val[j - 1];
+        while (j > 0 && gid[dl[j - 1]] > gn) {
+          dl[j] = dl[j - 1];
           --j;
         }
-        don[j] = n;
-        val[j] = mv[n] * w;
+        dl[j] = n;
         break;
       }
     }
   }
-  double s = 0.0;
-  for (int j = 0; j < nd; ++j) s += val[j];
-  moved[r] = s;
+  for (int j = 0; j < n_d; ++j) don[(int64_t)j * N + r] = dl[j];
+  nd[r] = (int8_t)n_d;
+  wv[r] = wgt[r];  // slot 0: the weight of every receiver of r (0 without one)
 }
 
-// the capacity test of one pass (seaplex.py:535-570)
-__global__ void k_route_dep(int64_t N, const double* __restrict__ moved, const int8_t* __restrict__ sink,
-                            const int8_t* __restrict__ outlet, int flat, double* __restrict__ cap,
-                            double* __restrict__ dep, double* __restrict__ mv, const int32_t* __restrict__ ctl) {
+// One pass of _distOcean (seaplex.py:514-570): moved[r] = sum over r's donors
+// (in increasing input id) of mv_in[i] w_i (dMat1.mult), then the capacity
+// test; what keeps moving goes to mv_out.
+__global__ void k_route_pass(int64_t N, const int32_t* __restrict__ don, const int8_t* __restrict__ nd,
+                             const double* __restrict__ wv, const double* __restrict__ mv_in,
+                             double* __restrict__ mv_out, const int8_t* __restrict__ sink,
+                             const int8_t* __restrict__ outlet, int flat, double* __restrict__ cap,
+                             double* __restrict__ dep, const int32_t* __restrict__ ctl) {
   if (ctl[0]) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  double s = moved[i];
+  const int n_d = nd[i];
+  double s = 0.0;
+  for (int j = 0; j < n_d; ++j) {
+    const int d = don[(int64_t)j * N + i];
+    s += mv_in[d] * wv[d];
+  }
   double c = cap[i];
-  double d = dep[i];
+  double dp = dep[i];
   if (s >= c) {
     s = s - c;
-    d = d + c;
+    dp = dp + c;
     c = 0.0;
   } else {
     c = c - s;
-    d = d + s;
+    dp = dp + s;
     s = 0.0;
   }
   if (flat && outlet[i]) {
-    d = 0.0;
+    dp = 0.0;
     s = 0.0;
   }
   if (sink[i]) {
-    d = d + s;
+    dp = dp + s;
     s = 0.0;
   }
   cap[i] = c;
-  dep[i] = d;
-  mv[i] = s;
+  dep[i] = dp;
+  mv_out[i] = s;
 }
 
 // what still moves is deposited in place (seaplex.py:596-606)
@@ -560,30 +564,39 @@ int gospl_marine_route(gospl_mesh_t* m, const int32_t* rcv, const double* wgt, c
   if (ensure_sea(m, 0)) return 1;
   const int64_t ld = N + 64;
   double* cap = m->sea_d;
-  double* mv = m->sea_d + ld;
-  double* moved = m->sea_d + 2 * ld;
+  double* mvb[2] = {m->sea_d + ld, m->sea_d + 2 * ld};  // what moves, before / after a pass
+  double* wv = m->sea_d + 3 * ld;
   double* part = m->sea_ctl;
   int32_t* ctl = m->dflag + 24;        // done, passes
   double* tot = m->sea_ctl + SUM_BLOCKS;  // first sum, last sum, threshold
-  k_route_init<<<g, GOSPL_BLOCK, 0, s>>>(N, cap0, vin, sink, cap, mv, vdep);
-  k_psum<<<SUM_BLOCKS, 256, 0, s>>>(N, mv, part, nullptr);
+  if (!m->sea_don) {
+    GOSPL_CHECK(hipMalloc(&m->sea_don, sizeof(int32_t) * (int64_t)m->W * ld));
+    GOSPL_CHECK(hipMalloc(&m->sea_nd, ld));
+  }
+  k_route_donors<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->gid, rcv, wgt, m->sea_don, m->sea_nd, wv);
+  k_route_init<<<g, GOSPL_BLOCK, 0, s>>>(N, cap0, vin, sink, cap, mvb[0], vdep);
+  k_psum<<<SUM_BLOCKS, 256, 0, s>>>(N, mvb[0], part, nullptr);
   k_route_check<<<1, 256, 0, s>>>(part, ctl, tot, 1);
   GOSPL_CHECK(hipGetLastError());
   int32_t hc[2] = {0, 0};
+  int launched = 0;  // pass j reads mvb[j % 2]; passes past the end return at once
   for (;;) {
     GOSPL_CHECK(hipMemcpyAsync(m->sea_h, ctl, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
     GOSPL_CHECK(hipStreamSynchronize(s));
     std::memcpy(hc, m->sea_h, sizeof(hc));
     if (hc[0]) break;
-    for (int b = 0; b < ROUTE_BATCH; ++b) {
-      k_route_push<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->gid, rcv, wgt, mv, moved, ctl);
-      k_route_dep<<<g, GOSPL_BLOCK, 0, s>>>(N, moved, sink, m->outlet, m->flat, cap, vdep, mv, ctl);
-      k_psum<<<SUM_BLOCKS, 256, 0, s>>>(N, mv, part, ctl);
+    for (int b = 0; b < ROUTE_BATCH; ++b, ++launched) {
+      double* in = mvb[launched & 1];
+      double* out = mvb[(launched + 1) & 1];
+      k_route_pass<<<g, GOSPL_BLOCK, 0, s>>>(N, m->sea_don, m->sea_nd, wv, in, out, sink, m->outlet, m->flat, cap,
+                                             vdep, ctl);
+      k_psum<<<SUM_BLOCKS, 256, 0, s>>>(N, out, part, ctl);
       k_route_check<<<1, 256, 0, s>>>(part, ctl, tot, 0);
     }
     GOSPL_CHECK(hipGetLastError());
   }
-  k_route_end<<<g, GOSPL_BLOCK, 0, s>>>(N, mv, m->outlet, m->flat, vdep);
+  // the passes run: hc[1]; the last one wrote mvb[hc[1] % 2]
+  k_route_end<<<g, GOSPL_BLOCK, 0, s>>>(N, mvb[hc[1] & 1], m->outlet, m->flat, vdep);
   GOSPL_CHECK(hipGetLastError());
   if (passes) *passes = hc[1];
   return 0;

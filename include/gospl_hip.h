@@ -643,7 +643,7 @@ int gospl_sed_apply(gospl_mesh_t* mesh, const double* delta, double* h, double* 
  *    373-402) with per-vertex kd (sethillslopecoeff, identity rows at
  *    flat-mesh outlets), Jacobi sweeps from x = rhs until one changes
  *    nothing or the contraction bound rho^k / (1 - rho) (rho = max over rows
- *    of (diag - 1) / diag) is below 2^-53 (the least such k rounded up to a
+ *    of (diag - 1) / diag) is below 2^-33 (the least such k rounded up to a
  *    multiple of 16, plus 64), at most maxit sweeps (goSPL: Richardson to
  *    rtol 1e-8).  *sweeps (HOST, may be NULL).  Synchronises.
  *  - gospl_marine_receivers: mfdrcvrs (functions.F90:1673-1785) as _matOcean

@@ -299,14 +299,14 @@ def nl_transport_solve(rcv, wgt, dist, kbr, n, fdep, area, dt, hold, rcv_f=None,
 
 def jacobi_sweep_cap(diag):
     """Sweeps after which the Jacobi error bound rho^k / (1 - rho) is below
-    2^-53 (rho = max over rows of (d - 1) / d): the least such k from
+    2^-33 (rho = max over rows of (d - 1) / d): the least such k from
     repeated products, rounded up to a multiple of 16, plus 64."""
     d = np.asarray(diag, np.float64)
     d = d[d > 0.0]
     rho = max(float(((d - 1.0) / d).max()), 0.0) if len(d) else 0.0
     if not rho < 1.0:
         return None
-    tgt = 2.0 ** -53 * (1.0 - rho)
+    tgt = 2.0 ** -33 * (1.0 - rho)
     pw, k = 1.0, 0
     while pw > tgt:
         pw *= rho

@@ -49,23 +49,19 @@ def contour_points(z, level, coords, ngb, nnb, with_edges=False):
     """VTK's contour of z at `level` on the mesh edges: for each edge (a, b)
     with z_a < level <= z_b the point x_a + t (x_b - x_a), t = (level - z_a)
     / (z_b - z_a) (vtkTriangle::Contour interpolates from the lower end)."""
-    pts = []
-    edges = []
-    for a in range(len(z)):
-        za = z[a]
-        if not za < level:
-            continue
-        for p in range(nnb[a]):
-            b = ngb[a, p]
-            if b < 0 or not z[b] >= level:
-                continue
-            t = (level - za) / (z[b] - za)
-            pts.append(coords[a] + t * (coords[b] - coords[a]))
-            edges.append((a, b))
-    pts = np.array(pts, dtype=np.float64).reshape(-1, 3)
+    z = np.asarray(z, np.float64)
+    slot = np.arange(ngb.shape[1])[None, :] < np.asarray(nnb)[:, None]
+    a, p = np.nonzero(slot & (z[:, None] < level))
+    b = ngb[a, p]
+    keep = (b >= 0)
+    a, b = a[keep], b[keep]
+    keep = z[b] >= level
+    a, b = a[keep], b[keep]
+    t = (level - z[a]) / (z[b] - z[a])
+    pts = coords[a] + t[:, None] * (coords[b] - coords[a])
     if with_edges:
-        return pts, np.array(edges, dtype=np.int64).reshape(-1, 2)
-    return pts
+        return pts.reshape(-1, 3), np.stack([a, b], axis=1).astype(np.int64).reshape(-1, 2)
+    return pts.reshape(-1, 3)
 
 
 def epsfill(sl, elev, ngb, nnb):

@@ -117,3 +117,61 @@ def test_fullsize_reproducible(full):
         g.dm.close()
     for a, b in zip(*res):
         assert np.array_equal(a, b)
+
+
+def test_fullsize_reference_default_step(full):
+    """goSPL's default physics (nodep False, seadepo True: flow, erosion,
+    flow again, continental sediment routing, marine deposition, hillslope)
+    on the 2.6M sphere: two runs give the same bits; the marine routing puts
+    every volume that reaches the sea somewhere on the closed sphere (up to
+    the single-precision weights of goSPL's mfdrcvrs, ~6e-8 per pass); the
+    marine diffusion smooths elevation plus deposit, which keeps their volume,
+    and clips what it takes from the old surface at zero, so the deposit it
+    returns is non-negative and holds at least the routed volume; the
+    coastline distances are the nearest of the contour points (spot check
+    against brute force); the transport-limited solve converges."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    m, h0, p = full
+    res = []
+    for rep in range(2):
+        g = Model(m, h0, tend=1e12, nodep=False, seadepo=True, marineSolver="picard", G=1.0, **p)
+        g.initialise()
+        for s in range(2):
+            g.step()
+            torch.cuda.synchronize()
+            st = g.sea_stats
+            print("[fullsize default] rep %d step %d: coast points %d, routing passes %s, GMRES %d (%.1e)" %
+                  (rep, s, st["coast_points"], st["route_passes"], g.ed_iters, g.ed_relres), flush=True)
+            assert g.ed_relres <= 1e-13
+            sb = g._seabuf
+            vin = float(sb["vin"].sum())
+            vdep = float(sb["vdep"].sum())
+            assert vin > 0.0
+            assert abs(vdep - vin) <= 1.2e-7 * st["route_passes"][-1] * vin, (vdep, vin)
+            area = g.dm.area
+            dvol = float((sb["dh"] * area).sum())
+            nvol = float((sb["ndepo"] * area).sum())
+            assert bool((sb["ndepo"] >= 0.0).all())
+            assert nvol >= dvol * (1.0 - 1e-6) and nvol <= dvol * (1.0 + 1e-3), (nvol, dvol)
+            for t in (g.hLocal, g.cumED, g.EbLocal, g.FAL):
+                assert bool(torch.isfinite(t).all())
+        # coastline distances of a few marine vertices against brute force
+        from oracle import sea
+
+        hin = m.to_input_order(g.hLocal.cpu().numpy())
+        coords_in = m.to_input_order(m.coords)
+        g._distanceCoasts(g.hLocal)
+        torch.cuda.synchronize()
+        d = m.to_input_order(g.coastDist.cpu().numpy())
+        pts = sea.contour_points(hin, g.sealevel, coords_in, m.extras["ngb_input"], m.extras["nnb_input"])
+        seam = m.to_input_order(g._seaFlow.cpu().numpy()).astype(bool)
+        pick = np.nonzero(seam)[0][:: max(1, int(seam.sum()) // 200)]
+        bf = np.sqrt(((coords_in[pick, None, :] - pts[None, :, :]) ** 2).sum(axis=2)).min(axis=1)
+        assert np.array_equal(d[pick], bf) or np.abs(d[pick] - bf).max() <= 1e-9 * bf.max()
+        res.append([t.cpu().numpy() for t in (g.hLocal, g.FAL, g.cumED)])
+        g.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)

@@ -862,6 +862,8 @@ struct ChainWave {
   }
 };
 
+#define CHAIN_AHEAD 3  // dependents' record lines loaded ahead per solved vertex
+
 template <int MODE>
 __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_t* __restrict__ list,
                                                        const int32_t* __restrict__ nlist,
@@ -883,6 +885,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
   int64_t ci = -1;            // global-queue entry this lane waits for
   int retry = 0;              // passes this lane waited on a row or a queue entry
   int idle = 0;               // consecutive idle passes of the wavefront
+  int pf[CHAIN_AHEAD] = {};   // first words of dependents' records, loaded ahead
   bool act = true;            // counted in ctl[CQ_ACTIVE]
   bool poller = false;        // counted in ctl[CQ_IDLERS]
   unsigned long long* qword = reinterpret_cast<unsigned long long*>(&ctl[CQ_QUEUE]);
@@ -1023,6 +1026,16 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
           }
         }
       }
+      // One word of each dependent's record, loaded beside the decrements
+      // (unconditionally, the lane's own line standing in): the lane whose
+      // decrement releases a dependent reads that record next and finds the
+      // line on its way to this XCD's L2.  Consumed at the end of the pass.
+      {
+        const int64_t own = (int64_t)(v >= 0 ? v : 0) * REC_W;
+#pragma unroll
+        for (int k = 0; k < CHAIN_AHEAD; ++k)
+          pf[k] = *reinterpret_cast<const int*>(a.rec + (k < 3 && dep[k] ? (int64_t)r[k] * REC_W : own));
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k) W.offer(dep[k] && old[k] == 1, r[k]);
     } else {
@@ -1079,6 +1092,12 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
           }
         }
       }
+      {
+        const int64_t own = (int64_t)(v >= 0 ? v : 0) * REC_W;
+#pragma unroll
+        for (int c = 0; c < CHAIN_AHEAD; ++c)
+          pf[c] = *reinterpret_cast<const int*>(a.rec + (c < nd ? (int64_t)j[c] * REC_W : own));
+      }
 #pragma unroll
       for (int c = 0; c < REC_NTGT; ++c) W.offer(c < nd && old[c] == 1, j[c]);
       // rows longer than the record (more than 14 donor entries)
@@ -1093,6 +1112,14 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
         }
         W.offer(rl, jj);
       }
+    }
+    // the words loaded ahead: a record's first word is its row length, far
+    // below 2^30
+    {
+      int any = 0;
+#pragma unroll
+      for (int k = 0; k < CHAIN_AHEAD; ++k) any |= pf[k];
+      if (any & 0x40000000) st_agent_i(&ctl[CQ_RETRIES], any);
     }
     // share work with idle wavefronts once the pool is long
     if (W.np >= W.share_at + 64) W.spill();

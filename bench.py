@@ -69,9 +69,9 @@ ALG_BYTES = {
 METRIC = "steps/s, 2.6M-vertex sphere: FP32+refine vs FP64 plain at ≤1e-6 rel err"
 STEP_BYTES = 564  # compulsory bytes per vertex and step (sum over the phases)
 # the sweep kernels between the library's timing events, per entry point
-SWEEP_KERNELS = {"gospl_flowacc": ("k_dag_level0<0> + k_chain<0>", 0),
-                 "gospl_flow_accumulation": ("k_dag_level0<0> + k_chain<0>", 0),
-                 "gospl_spl": ("k_dag_level0<1> + k_chain<1>", 1),
+SWEEP_KERNELS = {"gospl_flowacc": ("k_rec_build0 (with level 0) + k_chain<0>", 0),
+                 "gospl_flow_accumulation": ("k_rec_build0 (with level 0) + k_chain<0>", 0),
+                 "gospl_spl": ("k_rec_build1 (with level 0) + k_chain<1>", 1),
                  "gospl_flowacc_refine": ("k_rx_level0<0> + k_rx_chain<0>", 0),
                  "gospl_spl_refine": ("k_rx_level0<1> + k_rx_chain<1>", 1)}
 # HBM traffic per sweep (level 0 + dataflow kernel) on the level-9 workload,
@@ -466,7 +466,7 @@ def rocpd_summary(db, first, count, mode, N):
     for name, n, tot, avg in rows[:30]:
         out.append("| `%s` | %d | %.2f | %.1f |" % (name.split("(")[0].replace("void ", "")[:70], n, tot / 1e6,
                                                   avg / 1e3))
-    key = {0: ("k_dag_level0<0, true>", "k_chain<0>"), 1: ("k_dag_level0<1, true>", "k_chain<1>")}[mode]
+    key = {0: ("k_rec_build0", "k_chain<0>"), 1: ("k_rec_build1", "k_chain<1>")}[mode]
     lv = [r[0] for r in cur.execute("select duration from kernels where name like ? order by start",
                                     ("%gospl::" + key[0] + "%",))]
     ch = [r[0] for r in cur.execute("select duration from kernels where name like ? order by start",

@@ -388,13 +388,15 @@ __global__ void k_rev_nbr(int64_t N, int W, const int32_t* __restrict__ nbr,
 // from the donor side, one atomicOr per dependency (the mask is zeroed
 // before).
 // The position of i in the list of its receiver k is kept in rpos[k * N + i]
-// (-1: no dependency) for the record builds.
+// (-1: no dependency) for the record builds, and, when dmask is given, the
+// receivers i depends on in the stream-power sweep (bit k: a receiver other
+// than i with a non-zero coefficient).
 // The first workgroup also clears the sweep's small counters: zero[0..nz).
 __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__ nbr,
                            const int32_t* __restrict__ deg, const int8_t* __restrict__ rev,
                            const int32_t* __restrict__ rcv, const double* __restrict__ coef,
                            unsigned long long* __restrict__ rowmask, int8_t* __restrict__ rpos,
-                           ZeroList zl) {
+                           int8_t* __restrict__ dmask, ZeroList zl) {
   if (blockIdx.x == 0)
     for (int k = 0; k < zl.n; ++k)
       for (int t = threadIdx.x; t < zl.len[k]; t += blockDim.x) zl.p[k][t] = 0;
@@ -405,10 +407,13 @@ __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__
   const int dg = deg[i];
 #pragma unroll
   for (int q = 0; q < GOSPL_MAXW; ++q) nb[q] = (q < W && q < dg) ? nbr[(int64_t)q * N + i] : -1;
+  int dm = 0;
   for (int k = 0; k < NR; ++k) {
     const int r = rcv[(int64_t)k * N + i];
+    const double ck = coef[(int64_t)k * N + i];
     int p = -1;
-    if (r != (int)i && r >= 0 && coef[(int64_t)k * N + i] != 0.0) {
+    if (k < 3 && r != (int)i && ck != 0.0) dm |= 1 << k;
+    if (r != (int)i && r >= 0 && ck != 0.0) {
       int q = -1;
 #pragma unroll
       for (int t = GOSPL_MAXW - 1; t >= 0; --t)
@@ -418,6 +423,20 @@ __global__ void k_rec_mask(int64_t N, int W, int NR, const int32_t* __restrict__
     }
     rpos[(int64_t)k * N + i] = (int8_t)p;
   }
+  if (dmask) dmask[i] = (int8_t)dm;
+}
+
+// Appends i to the start list of the sweep when `start` (one reservation per
+// wavefront).  Reached by all lanes of the wavefront.
+__device__ __forceinline__ void start_append(bool start, int i, int32_t* __restrict__ fb,
+                                             int32_t* __restrict__ cnt) {
+  const unsigned long long m = __ballot(start);
+  if (m == 0ull) return;
+  const int lane = __lane_id();
+  int base = 0;
+  if (lane == 0) base = atomicAdd(cnt, __popcll(m));
+  base = __shfl(base, 0);
+  if (start) fb[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
 }
 
 // Records of a workgroup staged in LDS (9 double2 per record: the padding
@@ -444,19 +463,25 @@ __device__ __forceinline__ double rec_empty() { return __longlong_as_double(REC_
 __device__ __forceinline__ bool is_empty(double d) { return __double_as_longlong(d) == REC_EMPTY_BITS; }
 
 // Whole flow-accumulation record of vertex i, written at once.  The slot of
-// (i, receiver k) is the rank of its bit in the receiver's row mask.  Row
-// entries start empty.  pend[i] = row length.
-__global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
-                             const int32_t* __restrict__ rcv, const double* __restrict__ coef,
-                             const double* __restrict__ rhs,
+// (i, receiver k) is the rank of its bit in the receiver's row mask.  Level 0
+// of the sweep is done here: the entry of a donor without donors of its own
+// (its value is its b) is its push w * b, made by the row's owner, and the
+// other entries start empty; pend[i] = the donors still to come.  A vertex
+// without donors is solved (x = b); one whose donors all are is appended to
+// the start list fb (count *cnt).
+__global__ void k_rec_build0(int64_t N, int NR, const int32_t* __restrict__ nbr,
+                             const int8_t* __restrict__ rpos, const int32_t* __restrict__ rcv,
+                             const double* __restrict__ coef, const double* __restrict__ rhs,
                              const unsigned long long* __restrict__ rowmask, double* __restrict__ rec,
                              double* __restrict__ oacc, int32_t* __restrict__ pend,
-                             int32_t* __restrict__ nonnbr) {
+                             int32_t* __restrict__ nonnbr, double* __restrict__ x, int32_t* __restrict__ fb,
+                             int32_t* __restrict__ cnt) {
   __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
   const bool valid = i < N;
   double2 out[8];
+  bool start = false;
   if (valid) {
   int r[3], sl[3];
   double w[3];
@@ -478,38 +503,68 @@ __global__ void k_rec_build0(int64_t N, int NR, const int8_t* __restrict__ rpos,
     sl[k] = c < REC_NACC ? (int)(rk * (int64_t)REC_W + REC_ACC + c)
                          : (int)(-2 - ((int64_t)(c - REC_NACC) * N + rk));
   }
-  const int nd = __popcll(rowmask[i]);
-  pend[i] = nd;
+  const unsigned long long mi = rowmask[i];
+  const int nd = __popcll(mi);
   const double e = rec_empty();
+  double ent[REC_NACC];
+#pragma unroll
+  for (int c = 0; c < REC_NACC; ++c) ent[c] = e;
+  int npend = 0;
+  unsigned long long mk = mi;
+  for (int c = 0; mk != 0ull; ++c) {
+    const int b = __ffsll((long long)mk) - 1;
+    mk &= mk - 1ull;
+    const int p = b / NR, k = b - p * NR;
+    const int j = nbr[(int64_t)p * N + i];
+    double val = e;
+    if (rowmask[j] == 0ull) val = coef[(int64_t)k * N + j] * rhs[j];
+    else ++npend;
+    if (c < REC_NACC) {
+#pragma unroll
+      for (int q = 0; q < REC_NACC; ++q)
+        if (q == c) ent[q] = val;
+    } else {
+      oacc[(int64_t)(c - REC_NACC) * N + i] = val;
+    }
+  }
+  pend[i] = npend;
+  if (nd == 0) x[i] = rhs[i];
+  start = nd > 0 && npend == 0;
   out[0] = make_double2(pack32(nd, 0), rhs[i]);
   out[1] = make_double2(w[0], w[1]);
   out[2] = make_double2(w[2], pack32(r[0], r[1]));
   out[3] = make_double2(pack32(r[2], sl[0]), pack32(sl[1], sl[2]));
-  out[4] = make_double2(e, e);
-  out[5] = make_double2(e, e);
-  out[6] = make_double2(e, e);
-  out[7] = make_double2(e, e);
-  for (int c = REC_NACC; c < nd; ++c) oacc[(int64_t)(c - REC_NACC) * N + i] = e;
+  out[4] = make_double2(ent[0], ent[1]);
+  out[5] = make_double2(ent[2], ent[3]);
+  out[6] = make_double2(ent[4], ent[5]);
+  out[7] = make_double2(ent[6], ent[7]);
   }
+  start_append(start, (int)i, fb, cnt);
   rec_store_staged(stage, out, valid, rec, N);
 }
 
 // Whole stream-power record of vertex i, written at once: its dependency
-// mask, coefficients, its receivers' values (empty until pushed) and its row
-// (the donors' receiver-value slots, in row order, from its row mask;
-// entries past the record go to `odon`).
-// pend[i] = number of receivers i depends on.
+// mask, coefficients, its receivers' values and its row (the donors'
+// receiver-value slots, in row order, from its row mask; entries past the
+// record go to `odon`).  Level 0 of the sweep is done here: the value of a
+// receiver that depends on nothing (dmask 0: h / diagonal) is made by i, the
+// others start empty until pushed; pend[i] = the receivers still to come.  A
+// vertex that depends on nothing is solved; one whose receivers all are is
+// appended to the start list fb (count *cnt).
 __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
                              const int8_t* __restrict__ rpos, const int32_t* __restrict__ rcv,
                              const double* __restrict__ coef, const double* __restrict__ diag,
                              const double* __restrict__ rhs, const unsigned long long* __restrict__ rowmask,
                              double* __restrict__ rec, int32_t* __restrict__ odon,
-                             int32_t* __restrict__ pend, int32_t* __restrict__ nonnbr) {
+                             int32_t* __restrict__ pend, int32_t* __restrict__ nonnbr,
+                             const int8_t* __restrict__ dmask, double* __restrict__ x, int32_t* __restrict__ fb,
+                             int32_t* __restrict__ cnt) {
   __shared__ double2 stage[GOSPL_BLOCK][REC_STAGE];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if ((int64_t)blockIdx.x * blockDim.x >= N) return;  // whole workgroup past the mesh
   const bool valid = i < N;
   double2 out[8];
+  bool start = false;
   if (valid) {
   int mask = 0;
   double o[3] = {0.0, 0.0, 0.0};
@@ -541,17 +596,30 @@ __global__ void k_rec_build1(int64_t N, int NR, const int32_t* __restrict__ nbr,
       odon[(int64_t)(c - REC_NTGT) * N + i] = tj;
     }
   }
-  pend[i] = __popc(mask);
   const double e = rec_empty();
+  double xr[3] = {e, e, e};
+  int npend = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if ((mask >> k) & 1) {
+      const int r = rcv[(int64_t)k * N + i];
+      if (r >= 0 && r < N && dmask[r] == 0) xr[k] = rhs[r] / diag[r];
+      else ++npend;
+    }
+  }
+  pend[i] = npend;
+  if (mask == 0) x[i] = rhs[i] / diag[i];
+  start = mask != 0 && npend == 0;
   out[0] = make_double2(pack32(nd, mask), rhs[i]);
   out[1] = make_double2(diag[i], o[0]);
   out[2] = make_double2(o[1], o[2]);
-  out[3] = make_double2(e, e);
-  out[4] = make_double2(e, pack32(t[0], t[1]));
+  out[3] = make_double2(xr[0], xr[1]);
+  out[4] = make_double2(xr[2], pack32(t[0], t[1]));
   out[5] = make_double2(pack32(t[2], t[3]), pack32(t[4], t[5]));
   out[6] = make_double2(pack32(t[6], t[7]), pack32(t[8], t[9]));
   out[7] = make_double2(pack32(t[10], t[11]), pack32(t[12], t[13]));
   }
+  start_append(start, (int)i, fb, cnt);
   rec_store_staged(stage, out, valid, rec, N);
 }
 
@@ -1576,14 +1644,17 @@ static int dag_issue(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
     unsigned long long* rowmask = m->wu[0];
     GOSPL_CHECK(hipMemsetAsync(rowmask, 0, sizeof(unsigned long long) * N, s));
     int8_t* rpos = reinterpret_cast<int8_t*>(m->wdag[4]);  // 3 N bytes
+    int8_t* dmask = mode == 1 ? rpos + 3 * N : nullptr;     // the fourth N bytes
     k_rec_mask<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, nrcv, m->nbr, m->deg, m->rev, rcv, coef, rowmask, rpos,
-                                         zl);
+                                         dmask, zl);
+    // the sweep proper: the builds do level 0, their releases are the start list
+    if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
     if (mode == 0)
-      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc, pend,
-                                              dev + 3);
+      k_rec_build0<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rpos, rcv, coef, rhs, rowmask, m->wrec, m->wacc,
+                                              pend, dev + 3, x, fb, counts + 1);
     else
       k_rec_build1<<<g, GOSPL_BLOCK, 0, s>>>(N, nrcv, m->nbr, rpos, rcv, coef, diag, rhs, rowmask, m->wrec,
-                                              m->wdon, pend, dev + 3);
+                                              m->wdon, pend, dev + 3, dmask, x, fb, counts + 1);
   } else {
     // CSR donor rows
     if (ensure_donors(m, (int64_t)nrcv * N)) return 1;
@@ -1619,12 +1690,8 @@ static int dag_issue(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
   const void* kern = rec ? (mode == 0 ? (const void*)k_dag_level<0, true> : (const void*)k_dag_level<1, true>)
                         : (mode == 0 ? (const void*)k_dag_level<0, false> : (const void*)k_dag_level<1, false>);
   int32_t* gq = m->chain_gq;
-  if (m->timing) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
+  if (m->timing && !rec) GOSPL_CHECK(hipEventRecord(m->tev[0], s));
   if (chain) {
-    // level 0 on the full grid; its releases are the start list
-    if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-    else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-    GOSPL_CHECK(hipGetLastError());
     if (mode == 0)
       k_chain<0><<<m->chain_blocks[0], CHAIN_THREADS, 0, s>>>(a, fb, counts + 1, gq, gq_cap, m->chain_ctl,
                                                            m->chain_pool, m->chain_share, m->chain_pollers, m->chain_spin, vtrace);
@@ -1633,12 +1700,9 @@ static int dag_issue(gospl_mesh* m, int mode, int nrcv, const int32_t* rcv, cons
                                                            m->chain_pool, m->chain_share, m->chain_pollers, m->chain_spin, vtrace);
     GOSPL_CHECK(hipGetLastError());
   } else {
-    // level by level: level 0 on the full grid, the persistent level kernel,
-    // then the one-workgroup tail
-    if (rec) {
-      if (mode == 0) k_dag_level0<0, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-      else k_dag_level0<1, true><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
-    } else {
+    // level by level: level 0 (done by the record builds), the persistent
+    // level kernel, then the one-workgroup tail
+    if (!rec) {
       if (mode == 0) k_dag_level0<0, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
       else k_dag_level0<1, false><<<g, GOSPL_BLOCK, 0, s>>>(a, fb, counts);
     }

@@ -930,7 +930,8 @@ struct ChainWave {
   }
 };
 
-#define CHAIN_AHEAD 3  // dependents' record lines loaded ahead per solved vertex
+#define CHAIN_AHEAD 3   // dependents' record lines loaded ahead per solved vertex (flow)
+#define CHAIN_AHEAD1 6  // (stream power: the first donors of the row)
 
 template <int MODE>
 __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_t* __restrict__ list,
@@ -953,7 +954,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
   int64_t ci = -1;            // global-queue entry this lane waits for
   int retry = 0;              // passes this lane waited on a row or a queue entry
   int idle = 0;               // consecutive idle passes of the wavefront
-  int pf[CHAIN_AHEAD] = {};   // first words of dependents' records, loaded ahead
+  int pf[CHAIN_AHEAD1] = {};  // first words of dependents' records, loaded ahead
   bool act = true;            // counted in ctl[CQ_ACTIVE]
   bool poller = false;        // counted in ctl[CQ_IDLERS]
   unsigned long long* qword = reinterpret_cast<unsigned long long*>(&ctl[CQ_QUEUE]);
@@ -1163,7 +1164,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
       {
         const int64_t own = (int64_t)(v >= 0 ? v : 0) * REC_W;
 #pragma unroll
-        for (int c = 0; c < CHAIN_AHEAD; ++c)
+        for (int c = 0; c < CHAIN_AHEAD1; ++c)
           pf[c] = *reinterpret_cast<const int*>(a.rec + (c < nd ? (int64_t)j[c] * REC_W : own));
       }
 #pragma unroll
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_chain(DagArgs a, const int32_
     {
       int any = 0;
 #pragma unroll
-      for (int k = 0; k < CHAIN_AHEAD; ++k) any |= pf[k];
+      for (int k = 0; k < (MODE == 0 ? CHAIN_AHEAD : CHAIN_AHEAD1); ++k) any |= pf[k];
       if (any & 0x40000000) st_agent_i(&ctl[CQ_RETRIES], any);
     }
     // share work with idle wavefronts once the pool is long

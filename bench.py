@@ -74,7 +74,7 @@ SWEEP_KERNELS = {"gospl_flowacc": ("k_rec_build0 (with level 0) + k_chain<0>", 0
                  "gospl_spl": ("k_rec_build1 (with level 0) + k_chain<1>", 1),
                  "gospl_flowacc_refine": ("k_rx_level0<0> + k_rx_chain<0>", 0),
                  "gospl_spl_refine": ("k_rx_level0<1> + k_rx_chain<1>", 1)}
-# HBM traffic per sweep (level 0 + dataflow kernel) on the level-9 workload,
+# HBM traffic per sweep (record build with level 0 + dataflow kernel) on the level-9 workload,
 # from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch, mean over the passes' sweeps), each scaled by what the counter
 # reads for a 256 MiB device copy (`--copy-probe 256`): FETCH_SIZE reads 0.5x
@@ -82,8 +82,8 @@ SWEEP_KERNELS = {"gospl_flowacc": ("k_rec_build0 (with level 0) + k_chain<0>", 0
 # in profiles/*_pmc_traffic.md.
 PMC_SCALE = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
 PMC_KIB = {
-    "gospl_flowacc": {"FETCH_SIZE": 213481.8 + 50784.1, "WRITE_SIZE": 15297.1 + 139938.3},
-    "gospl_spl": {"FETCH_SIZE": 196818.8 + 46179.1, "WRITE_SIZE": 17258.1 + 132079.6},
+    "gospl_flowacc": {"FETCH_SIZE": 92162.2 + 124407.3, "WRITE_SIZE": 352432.5 + 140066.2},
+    "gospl_spl": {"FETCH_SIZE": 96833.1 + 111452.7, "WRITE_SIZE": 353008.5 + 132284.4},
 }
 PMC_KIB["gospl_flow_accumulation"] = PMC_KIB["gospl_flowacc"]
 

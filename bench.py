@@ -243,6 +243,43 @@ def run_leg(m, h0, p, dev, name, steps, warmup):
                 sweep=dict(fa_ms=sw[0].value, fa_calls=sw[1].value, spl_ms=sw[2].value, spl_calls=sw[3].value))
 
 
+def default_physics_leg(m, h0, p, dev, steps, warmup, G):
+    """goSPL's default step on the same workload (informational, not `value`):
+    nodep False and seadepo True -- flow, erosion, flow again, continental
+    sediment routing, marine deposition, hillslope -- with the erosion
+    transport-limited when G > 0.  W warm-up steps, K timed steps, then K
+    profiled steps (HIP events around every entry point)."""
+    import torch
+
+    from gospl_amd.model import Model
+
+    g = Model(m, h0, tend=1e12, device=dev, nodep=False, seadepo=True, marineSolver="picard", G=G, **p)
+    g.initialise()
+    for _ in range(warmup):
+        g.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    g.enable_profiling(True)
+    for _ in range(steps):
+        g.step()
+    ph = g.phase_times()
+    g.enable_profiling(False)
+    st = dict(g.sea_stats)
+    it = int(getattr(g, "ed_iters", 0) or 0)
+    g.close()
+    out = dict(G=G, steps=steps, warmup=warmup, steps_per_s=steps / elapsed, ms_per_step=1e3 * elapsed / steps,
+               phases_ms_per_step={k: v[0] / steps for k, v in sorted(ph.items(), key=lambda x: -x[1][0])},
+               last_step={"coast_points": st.get("coast_points"), "marine_smoothing_sweeps": st.get("smooth_sweeps"),
+                          "marine_routing_passes": st.get("route_passes"),
+                          "transport_limited_gmres_iterations": it})
+    log("[bench] default physics (G=%g): %.1f ms per step" % (G, out["ms_per_step"]))
+    return out
+
+
 def dominant_kernel(leg, N, level):
     """The entry point with the most time among the per-vertex phases, and its
     sweep kernels' duration per launch (HIP events inside the library)."""
@@ -503,6 +540,9 @@ def main():
                     help="vertices per input of the leg's finite-difference check")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="timed steps of the 144-member ensemble leg; 0 skips it")
+    ap.add_argument("--default-physics-steps", type=int, default=5,
+                    help="timed steps of goSPL's default step (sediment routing and marine deposition on), "
+                         "with G = 0 and G = 1; 0 skips it")
     ap.add_argument("--surface", default=None, metavar="NPY",
                     help="initial surface (input order) saved by --save-surface instead of the SH field "
                          "(keeps the SH construction's kernels out of a counter run)")
@@ -594,6 +634,9 @@ def main():
     ens = None
     if args.ensemble_steps > 0:
         ens = ensemble_leg(dev, args.ensemble_steps, 2, args.seed)
+    dphys = None
+    if args.default_physics_steps > 0:
+        dphys = [default_physics_leg(m, h0, p, dev, args.default_physics_steps, 2, G) for G in (0.0, 1.0)]
 
     leg_cfg = {}
     for name, lg in legs.items():
@@ -619,6 +662,8 @@ def main():
         cfg["adjoint"] = adj
     if ens is not None:
         cfg["ensemble"] = ens
+    if dphys is not None:
+        cfg["default_physics"] = dphys
     dtype = {"refined": "f64 (DAG sweeps, state, residuals) + f32 (hillslope corrections)",
              "refined_dag": "f64 (state, residuals) + i64 fixed-point (DAG corrections) + f32 (hillslope)",
              "fp64": "f64"}[main_leg]

@@ -133,6 +133,31 @@ __global__ void k_hill_sweep(int64_t N, int W, const int32_t* __restrict__ nbr,
   xout[i] = s / d;
 }
 
+// One step of the Chebyshev semi-iteration over that sweep:
+// xout = omega (sweep(xcur) - xprev) + xprev, the sweep in k_hill_sweep's
+// operation order.
+__global__ void k_hill_cheb(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+                            const double* __restrict__ coef, const double* __restrict__ diag,
+                            const double* __restrict__ rhs, const double* __restrict__ xcur,
+                            const double* __restrict__ xprev, double omega, double* __restrict__ xout) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double d = diag[i];
+  const double r = rhs[i];
+  double v = r;
+  if (d != 0.0) {
+    double s = r;
+    const int dg = deg[i];
+    for (int p = 0; p < W && p < dg; ++p) {
+      const double c = coef[(int64_t)p * N + i];
+      if (c != 0.0) s = s - c * xcur[nbr[(int64_t)p * N + i]];
+    }
+    v = s / d;
+  }
+  const double xp = xprev[i];
+  xout[i] = omega * (v - xp) + xp;
+}
+
 // Two maxima (order-preserving keys; a and b valid in thread 0) of every
 // workgroup reach keys[0..1] through per-workgroup partials: the last
 // workgroup to finish (a counter it resets) reduces them.  One same-address
@@ -613,13 +638,15 @@ extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const doub
   k_hill_coef_kd<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, kd, op);
   if (m->flat) k_hill_outlet_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->outlet, op);
   GOSPL_CHECK(hipGetLastError());
-  // Jacobi from x = rhs until a sweep changes nothing (the change measured
-  // every 16th sweep) or until the error bound rho^k / (1 - rho) of the
-  // contraction rho = max (d - 1) / d is below 2^-33 (~1.2e-10, below the
-  // rtol 1e-8 of goSPL's Richardson solve; the last bits of a few rows can
-  // cycle for ever on large meshes).  The cap is the least k with rho^k <=
-  // 2^-33 (1 - rho) (repeated products), rounded up to a multiple of 16, plus
-  // 64.
+  // Chebyshev semi-iteration over the Jacobi sweep from x0 = rhs.  rho = max
+  // over rows of (d - 1) / d bounds the spectrum of the sweep's iteration
+  // matrix, which is real (the operator is symmetric up to the row scaling
+  // by the cell areas).  The count is the least k with 2 sigma^k <= 2^-49
+  // (1 - rho), sigma = rho / (1 + sqrt(1 - rho^2)): the error bound in the
+  // norm that symmetrises the sweep, with 2^-16 of room for that norm against
+  // the max norm, under the 2^-33 of the plain sweeps' bound rho^k / (1 -
+  // rho) (goSPL: Richardson to rtol 1e-8).  Weights omega_1 = 1 / (1 - rho^2
+  // / 2), omega_k = 1 / (1 - rho^2 omega_{k-1} / 4).
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
   unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
   GOSPL_CHECK(hipMemsetAsync(keys + 2, 0, sizeof(unsigned long long), s));
@@ -628,47 +655,43 @@ extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const doub
   GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
   const double rho = hk[2] ? dkey_inv_host(hk[2]) : 0.0;
-  int limit = maxit;
-  if (rho < 1.0) {
-    const double tgt = 0x1p-33 * (1.0 - rho);
-    double pw = 1.0;
-    int k = 0;
-    while (pw > tgt && k < maxit) {
-      pw *= rho;
-      ++k;
-    }
-    const int cap = 16 * ((k + 15) / 16) + 64;
-    limit = cap < maxit ? cap : maxit;
-  }
-  double* xa = m->wd[1];
-  double* xb = m->wd[2];
-  GOSPL_CHECK(hipMemcpyAsync(xa, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-  double* xin = xa;
-  double* xout = xb;
-  int done = 0;
-  bool fixed = false;
-  while (!fixed && done < limit) {
-    for (int k = 0; k < 15 && done < limit - 1; ++k, ++done) {
-      k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, xin, xout);
-      std::swap(xin, xout);
-    }
-    GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
-    k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, xin, xout, keys, m->red_part, m->red_cnt);
-    GOSPL_CHECK(hipGetLastError());
-    std::swap(xin, xout);
-    ++done;
-    GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
-    GOSPL_CHECK(hipStreamSynchronize(s));
-    const double chg = dkey_inv_host(hk[0]);
-    if (!(chg == chg)) break;
-    fixed = chg == 0.0;
-  }
-  if (!fixed && !(rho < 1.0 && done == limit)) {
-    set_error("gospl_diffuse_fixed: Jacobi sweeps did not converge within %d sweeps (contraction %.6g)", done, rho);
+  if (!(rho < 1.0)) {
+    set_error("gospl_diffuse_fixed: the Jacobi sweep does not contract (rho %.6g)", rho);
     return 1;
   }
-  if (sweeps) *sweeps = done;
-  GOSPL_CHECK(hipMemcpyAsync(x, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  const double r2 = rho * rho;
+  const double sig = rho / (1.0 + std::sqrt(1.0 - r2));
+  const double tgt = 0x1p-49 * (1.0 - rho);
+  double pw = 2.0;
+  int nsw = 0;
+  while (pw > tgt && nsw < maxit) {
+    pw *= sig;
+    ++nsw;
+  }
+  if (pw > tgt) {
+    set_error("gospl_diffuse_fixed: %d sweeps do not reach the error bound (contraction %.6g)", maxit, rho);
+    return 1;
+  }
+  // x1 = sweep(rhs); then three buffers in turn (rhs is never written)
+  double* buf[3] = {m->wd[1], m->wd[2], x};
+  const double* prev = rhs;
+  int cur = 0;
+  k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, rhs, buf[0]);
+  double omega = 1.0;
+  int prev_i = -1;
+  for (int k = 1; k < nsw; ++k) {
+    omega = k == 1 ? 1.0 / (1.0 - 0.5 * r2) : 1.0 / (1.0 - 0.25 * r2 * omega);
+    int nxt = 0;
+    while (nxt == cur || nxt == prev_i) ++nxt;
+    k_hill_cheb<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, op + N, op, rhs, buf[cur], prev, omega,
+                                          buf[nxt]);
+    prev = buf[cur];
+    prev_i = cur;
+    cur = nxt;
+  }
+  GOSPL_CHECK(hipGetLastError());
+  if (buf[cur] != x) GOSPL_CHECK(hipMemcpyAsync(x, buf[cur], sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  if (sweeps) *sweeps = nsw;
   return 0;
 }
 

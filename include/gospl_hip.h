@@ -641,11 +641,13 @@ int gospl_sed_apply(gospl_mesh_t* mesh, const double* delta, double* h, double* 
  *  - gospl_diffuse_fixed: the implicit linear diffusion (I + L(kd)) x = rhs of
  *    hillSLP._buildDiffMat / _solveSmooth (hillslope.py:131-150, 235-257,
  *    373-402) with per-vertex kd (sethillslopecoeff, identity rows at
- *    flat-mesh outlets), Jacobi sweeps from x = rhs until one changes
- *    nothing or the contraction bound rho^k / (1 - rho) (rho = max over rows
- *    of (diag - 1) / diag) is below 2^-33 (the least such k rounded up to a
- *    multiple of 16, plus 64), at most maxit sweeps (goSPL: Richardson to
- *    rtol 1e-8).  *sweeps (HOST, may be NULL).  Synchronises.
+ *    flat-mesh outlets), by the Chebyshev semi-iteration over the Jacobi
+ *    sweep from x0 = rhs: x1 = sweep(x0), x_{k+1} = w_k (sweep(x_k) -
+ *    x_{k-1}) + x_{k-1}, w_1 = 1 / (1 - rho^2 / 2), w_k = 1 / (1 - rho^2
+ *    w_{k-1} / 4), rho = max over rows of (diag - 1) / diag; the count is
+ *    the least k with 2 sigma^k <= 2^-49 (1 - rho), sigma = rho / (1 +
+ *    sqrt(1 - rho^2)), at most maxit (goSPL: Richardson to rtol 1e-8).  x
+ *    must not alias rhs or kd.  *sweeps (HOST, may be NULL).  Synchronises.
  *  - gospl_marine_receivers: mfdrcvrs (functions.F90:1673-1785) as _matOcean
  *    calls it (:198-219): rcv = the strictly lower neighbours on fillz in
  *    neighbour order then the vertex itself, wgt = 1 / their count in single

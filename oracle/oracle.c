@@ -561,6 +561,39 @@ int64_t or_spl_solve(int64_t nb, int K, const int32_t* rcv, const double* off,
 /* col p+1 the entry for neighbour p.  Rows with a zero diagonal are kept at */
 /* rhs.  Returns the number of sweeps.                                        */
 /* ------------------------------------------------------------------------ */
+/* Chebyshev semi-iteration over the same Jacobi sweep: x1 = sweep(x0 = rhs),
+ * then x_{k+1} = omega[k] (sweep(x_k) - x_{k-1}) + x_{k-1} for k = 1 ..
+ * nsweep - 1, each row's sweep in the operation order of or_hillslope_solve. */
+void or_cheb_smooth(int64_t nb, const int32_t* ngb, const int32_t* nnb, const double* coeff,
+                    const double* rhs, double* x, int nsweep, const double* omega) {
+  double* xp = (double*)malloc((size_t)nb * sizeof(double));
+  double* y = (double*)malloc((size_t)nb * sizeof(double));
+  for (int64_t i = 0; i < nb; ++i) xp[i] = rhs[i];
+  for (int k = 0; k < nsweep; ++k) {
+    const double* xc = k == 0 ? xp : x;
+    for (int64_t i = 0; i < nb; ++i) {
+      double d = coeff[i * 13];
+      double v;
+      if (d == 0.0) {
+        v = rhs[i];
+      } else {
+        double s = rhs[i];
+        for (int p = 0; p < nnb[i]; ++p) {
+          double c = coeff[i * 13 + p + 1];
+          if (c != 0.0) s -= c * xc[ngb[i * MAXNB + p]];
+        }
+        v = s / d;
+      }
+      y[i] = k == 0 ? v : omega[k] * (v - xp[i]) + xp[i];
+    }
+    if (k > 0)
+      for (int64_t i = 0; i < nb; ++i) xp[i] = x[i];
+    for (int64_t i = 0; i < nb; ++i) x[i] = y[i];
+  }
+  free(xp);
+  free(y);
+}
+
 int or_hillslope_solve(int64_t nb, const int32_t* ngb, const int32_t* nnb,
                        const double* coeff, const double* rhs, double* x, int maxit) {
   double* y = (double*)malloc((size_t)nb * sizeof(double));

@@ -42,6 +42,7 @@ tolerance.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -83,6 +84,8 @@ def lib():
         L.or_spl_solve.restype = I64
         L.or_hillslope_solve.argtypes = [I64, P, P, P, P, P, I32]
         L.or_hillslope_solve.restype = I32
+        L.or_cheb_smooth.argtypes = [I64, P, P, P, P, P, I32, P]
+        L.or_cheb_smooth.restype = None
         _lib = L
     return _lib
 
@@ -297,35 +300,48 @@ def nl_transport_solve(rcv, wgt, dist, kbr, n, fdep, area, dt, hold, rcv_f=None,
     raise RuntimeError("non-linear transport-limited oracle: Newton did not converge in %d steps" % maxit)
 
 
-def jacobi_sweep_cap(diag):
-    """Sweeps after which the Jacobi error bound rho^k / (1 - rho) is below
-    2^-33 (rho = max over rows of (d - 1) / d): the least such k from
-    repeated products, rounded up to a multiple of 16, plus 64."""
+def chebyshev_plan(diag, maxit=1000000):
+    """Sweeps and weights of the Chebyshev semi-iteration over the Jacobi
+    sweep of a diffusion operator with diagonal `diag`.  rho = max over rows
+    of (d - 1) / d bounds the spectrum of the Jacobi iteration matrix (real:
+    it is similar to a symmetric matrix); the count is the least k with
+    2 sigma^k <= 2^-49 (1 - rho), sigma = rho / (1 + sqrt(1 - rho^2)) (the
+    error bound 2 sigma^k in the norm that symmetrises the sweep, with 2^-16
+    of room for that norm against the max norm, under the 2^-33 of the
+    Jacobi bound rho^k / (1 - rho)); omega[0] = 1 (a plain sweep), omega[1] =
+    1 / (1 - rho^2 / 2), omega[k] = 1 / (1 - rho^2 omega[k-1] / 4).  Returns
+    (rho, omega) or None when the sweep does not contract."""
     d = np.asarray(diag, np.float64)
     d = d[d > 0.0]
     rho = max(float(((d - 1.0) / d).max()), 0.0) if len(d) else 0.0
     if not rho < 1.0:
         return None
-    tgt = 2.0 ** -33 * (1.0 - rho)
-    pw, k = 1.0, 0
-    while pw > tgt:
-        pw *= rho
+    r2 = rho * rho
+    sig = rho / (1.0 + math.sqrt(1.0 - r2))
+    tgt = 2.0 ** -49 * (1.0 - rho)
+    pw, k = 2.0, 0
+    while pw > tgt and k < maxit:
+        pw *= sig
         k += 1
-    return 16 * ((k + 15) // 16) + 64
+    om = [1.0]
+    for j in range(1, k):
+        om.append(1.0 / (1.0 - 0.5 * r2) if j == 1 else 1.0 / (1.0 - 0.25 * r2 * om[-1]))
+    return rho, np.array(om, np.float64)
 
 
 def smooth_solve(coeff, rhs, ngb, nnb):
-    """Jacobi sweeps from x = rhs until one changes nothing or for
-    jacobi_sweep_cap sweeps (the last bits of a few rows can cycle on large
-    meshes).  Returns (x, sweeps)."""
-    cap = jacobi_sweep_cap(coeff[:, 0])
-    if cap is None:
+    """The marine smoothing: Chebyshev semi-iteration over the Jacobi sweep
+    of or_hillslope_solve from x = rhs, for the count of chebyshev_plan.
+    Returns (x, sweeps)."""
+    plan = chebyshev_plan(coeff[:, 0])
+    if plan is None:
         raise RuntimeError("Jacobi sweeps do not contract")
+    _, om = plan
     N = len(rhs)
     x = np.empty(N)
-    it = lib().or_hillslope_solve(N, _p(ngb), _p(nnb), _p(_c(coeff, np.float64)), _p(_c(rhs, np.float64)),
-                                  _p(x), int(cap))
-    return x, min(int(it), cap)
+    lib().or_cheb_smooth(N, _p(ngb), _p(nnb), _p(_c(coeff, np.float64)), _p(_c(rhs, np.float64)), _p(x),
+                         len(om), _p(om))
+    return x, len(om)
 
 
 def hillslope_solve(coeff, rhs, ngb, nnb, maxit=1000000):

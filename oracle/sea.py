@@ -14,7 +14,7 @@ or provenance, which are off on this path):
   come from scipy's cKDTree as in goSPL;
 * ``_matOcean``         :137-238, with ``_hillSlope(smooth=2)``
   (gospl/sed/hillslope.py:131-150, Kd = N area, N = 1 on land and 5 at sea)
-  by Jacobi sweeps to their fixed point or contraction bound, ``epsfill`` (functions.F90:2633-2697) as a
+  by the Chebyshev semi-iteration over the Jacobi sweep, ``epsfill`` (functions.F90:2633-2697) as a
   priority flood and ``mfdrcvrs`` (:1673-1785) as goSPL calls it;
 * ``_distOcean``        :404-619 (without provenance), the push of each pass
   summed per receiver in increasing input id;
@@ -201,8 +201,8 @@ class SEA:
     # ------------------------------------------------------------ routing
     def _smoothMarine(self):
         """_hillSlope(smooth=2) (hillslope.py:131-150): the marine smoothing
-        of h by Jacobi sweeps to their fixed point or their contraction
-        bound (oracle.smooth_solve)."""
+        of h by the Chebyshev semi-iteration over the Jacobi sweep
+        (oracle.smooth_solve)."""
         from .oracle import sethillslopecoeff, smooth_solve
 
         Cd = np.full(self.lpoints, MARINE_SMOOTH_N_LAND)

@@ -7,9 +7,10 @@ oracle is handed the GPU's state after the continental sediment routing of
 each step (elevation, cumulative erosion/deposition, sediment load, filled
 surface), then both run seaChange.  Bars: coastline distances, the smoothed
 and epsilon-filled marine surfaces, the marine receivers and weights and the
-routed volumes bit for bit (the same operations in the same order: Jacobi
-sweeps to their fixed point or the same contraction-bound count, the epsilon
-flood's unique fixed point, sums per receiver in increasing input id); the coupled Gmar > 0 solve, the marine
+routed volumes bit for bit (the same operations in the same order: the same
+count of Chebyshev steps over the Jacobi sweep with the same weights, the
+epsilon flood's unique fixed point, sums per receiver in increasing input
+id); the coupled Gmar > 0 solve, the marine
 diffusion, elevation, cumulative erosion/deposition and rate within 1e-9
 relative (GMRES / Jacobi against sparse direct solves).  Cases: the closed
 level-5 sphere with Gmar = 0 and Gmar = 0.5, and a plane whose sea level

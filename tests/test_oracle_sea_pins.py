@@ -11,7 +11,9 @@ SEAMesh.seaChange, gospl/sed/seaplex.py:51-861) -- CPU only.
   leaves through an outlet, no vertex outside the sinks takes more than its
   shelf capacity;
 * the coupled marine deposition on a hand-solved chain (each sea vertex keeps
-  fDepm of what reaches it) and its flux block's balance on a sphere.
+  fDepm of what reaches it) and its flux block's balance on a sphere;
+* the marine smoothing (Chebyshev semi-iteration) against a sparse direct
+  solve and against plain Jacobi sweeps run to their fixed point.
 """
 
 from __future__ import annotations
@@ -157,3 +159,40 @@ def test_coupled_marine_flux_balance_on_sphere():
     left = np.sum(q * (1.0 - passed)) * o.dt
     assert abs(vol.sum() - kept - left) <= 1e-9 * vol.sum()
     assert vd.sum() > 0
+
+
+@pytest.mark.parametrize("scale", [1.0, 40.0])
+def test_marine_smoothing_against_direct_solve(scale):
+    """The Chebyshev smoothing of the oracle (oracle.smooth_solve) against a
+    sparse direct solve of the same system (I + L(kd)) x = h and against
+    plain Jacobi sweeps run to their fixed point; scale 40 makes the sweep's
+    contraction close to 1 (several hundred Jacobi sweeps)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from oracle import oracle as orc
+
+    o = _sea_state("sphere", 5)
+    N = o.lpoints
+    Cd = np.full(N, sea.MARINE_SMOOTH_N_LAND)
+    Cd[o.seaID] = sea.MARINE_SMOOTH_N_SEA
+    coeff = orc.sethillslopecoeff(scale * Cd * o.larea, o.ngb, o.nnb, o.larea, o.elgt, o.vdist)
+    h = o.h.copy()
+    x, nsw = orc.smooth_solve(coeff, h, o.ngb, o.nnb)
+    rho, om = orc.chebyshev_plan(coeff[:, 0])
+    assert nsw == len(om) and 0.0 < rho < 1.0
+    rows, cols, vals = [np.arange(N)], [np.arange(N)], [coeff[:, 0]]
+    for p in range(o.ngb.shape[1]):
+        ok = (p < o.nnb) & (o.ngb[:, p] >= 0) & (coeff[:, p + 1] != 0.0)
+        rows.append(np.nonzero(ok)[0])
+        cols.append(o.ngb[ok, p])
+        vals.append(coeff[ok, p + 1])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    xd = spla.spsolve(A.tocsc(), h)
+    scl = np.abs(h).max()
+    assert np.abs(x - xd).max() <= 1e-11 * scl, np.abs(x - xd).max() / scl
+    xj, it = orc.hillslope_solve(coeff, h, o.ngb, o.nnb, maxit=200000)
+    assert np.abs(x - xj).max() <= 1e-11 * scl
+    # the plain sweeps need several times more passes than the semi-iteration
+    if scale > 1.0:
+        assert it > 3 * nsw, (it, nsw)

@@ -340,8 +340,6 @@ static int mesh_alloc(gospl_mesh* m) {
   GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
   GOSPL_CHECK(hipMalloc(&m->red_part, sizeof(unsigned long long) * 2 * (nblocks(N) + 64)));
-  GOSPL_CHECK(hipMalloc(&m->red_cnt, sizeof(unsigned int) * 64));
-  GOSPL_CHECK(hipMemset(m->red_cnt, 0, sizeof(unsigned int) * 64));
   GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
   // rocPRIM workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
   size_t bytes = 0, b2 = 0;
@@ -463,7 +461,6 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->kry_d);
   if (m->kry_h) (void)hipHostFree(m->kry_h);
   dfree(m->red_part);
-  dfree(m->red_cnt);
   dfree(m->sea_d);
   dfree(m->sea_don);
   dfree(m->sea_nd);

@@ -120,7 +120,6 @@ struct gospl_mesh {
   int32_t* sea_don;           // marine routing: donors of each vertex (W, N), by input id
   int8_t* sea_nd;             // their count
   unsigned long long* red_part;  // per-workgroup partial maxima of the convergence tests
-  unsigned int* red_cnt;         // their completion counter (0 between launches)
 };
 
 namespace gospl {

@@ -159,26 +159,23 @@ __global__ void k_hill_cheb(int64_t N, int W, const int32_t* __restrict__ nbr, c
 }
 
 // Two maxima (order-preserving keys; a and b valid in thread 0) of every
-// workgroup reach keys[0..1] through per-workgroup partials: the last
-// workgroup to finish (a counter it resets) reduces them.  One same-address
-// atomic per workgroup serialises at the L2 (~200 us over 2.6M rows).
-__device__ void max2_finish(unsigned long long a, unsigned long long b, unsigned long long* keys,
-                            unsigned long long* part, unsigned int* cnt) {
-  __shared__ bool last;
-  __shared__ unsigned long long sm[2][GOSPL_BLOCK / 64];
+// workgroup, as per-workgroup partials; k_max2_finish folds them into
+// keys[0..1] in the next launch.  (A last-workgroup reduction through one
+// completion counter took ~25 ns per workgroup on the counter's same-address
+// atomic, ~250 us over 2.6M rows.)
+__device__ __forceinline__ void max2_part(unsigned long long a, unsigned long long b, unsigned long long* part) {
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    last = atomicAdd(cnt, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+__global__ __launch_bounds__(1024) void k_max2_finish(int nblk, const unsigned long long* __restrict__ part,
+                                                      unsigned long long* __restrict__ keys) {
+  __shared__ unsigned long long sm[2][1024 / 64];
   unsigned long long ma = 0ull, mb = 0ull;
-  for (unsigned int k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-    const unsigned long long x = __hip_atomic_load(part + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long y = __hip_atomic_load(part + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+    const unsigned long long x = part[2 * k], y = part[2 * k + 1];
     ma = x > ma ? x : ma;
     mb = y > mb ? y : mb;
   }
@@ -199,7 +196,6 @@ __device__ void max2_finish(unsigned long long a, unsigned long long b, unsigned
     }
     keys[0] = ma > keys[0] ? ma : keys[0];
     keys[1] = mb > keys[1] ? mb : keys[1];
-    *cnt = 0u;
   }
 }
 
@@ -210,8 +206,7 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_hill_sweep_chk(int64_t N, int W
                                  const int32_t* __restrict__ deg, const double* __restrict__ coef,
                                  const double* __restrict__ diag, const double* __restrict__ rhs,
                                  const double* __restrict__ xin, double* __restrict__ xout,
-                                 unsigned long long* __restrict__ keys, unsigned long long* __restrict__ part,
-                                 unsigned int* __restrict__ cnt) {
+                                 unsigned long long* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double chg = 0.0, mag = 0.0;
   if (i < N) {
@@ -247,15 +242,13 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_hill_sweep_chk(int64_t N, int W
       mag = fmax(mag, sh[1][w]);
     }
   }
-  max2_finish(dkey(chg), dkey(mag), keys, part, cnt);
+  max2_part(dkey(chg), dkey(mag), part);
 }
 
 // max over the rows of the Jacobi contraction (diag - 1) / diag = s / (1 + s)
 // into key[0] (left as it is when no row contracts: 0 after the caller's clear)
 __global__ __launch_bounds__(GOSPL_BLOCK) void k_hill_rho(int64_t N, const double* __restrict__ diag,
-                                                         unsigned long long* __restrict__ key,
-                                                         unsigned long long* __restrict__ part,
-                                                         unsigned int* __restrict__ cnt) {
+                                                         unsigned long long* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double r = 0.0;
   if (i < N) {
@@ -268,9 +261,9 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_hill_rho(int64_t N, const doubl
   __syncthreads();
   if (threadIdx.x == 0)
     for (int w = 1; w < GOSPL_BLOCK / 64; ++w) r = fmax(r, sh[w]);
-  // a key of 0 for a workgroup without a contracting row: key[0] keeps the
-  // caller's 0 (key[1] is only compared with 0 and kept)
-  max2_finish(r > 0.0 ? dkey(r) : 0ull, 0ull, key, part, cnt);
+  // a key of 0 for a workgroup without a contracting row: the key keeps the
+  // caller's 0 (the second key is only compared with 0 and kept)
+  max2_part(r > 0.0 ? dkey(r) : 0ull, 0ull, part);
 }
 
 // Rows from given sea flags (the frozen operator of the adjoint and its
@@ -650,7 +643,8 @@ extern "C" int gospl_diffuse_fixed(gospl_mesh_t* m, const double* kd, const doub
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
   unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
   GOSPL_CHECK(hipMemsetAsync(keys + 2, 0, sizeof(unsigned long long), s));
-  k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, op, keys + 2, m->red_part, m->red_cnt);
+  k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, op, m->red_part);
+  k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys + 2);
   GOSPL_CHECK(hipGetLastError());
   GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   GOSPL_CHECK(hipStreamSynchronize(s));
@@ -765,7 +759,8 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
     // measuring (one read-back per chunk)
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(m->dflag + 48);
     GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 3, s));
-    k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, keys + 2, m->red_part, m->red_cnt);
+    k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, m->red_part);
+    k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys + 2);
     GOSPL_CHECK(hipGetLastError());
     unsigned long long* hk = reinterpret_cast<unsigned long long*>(m->hflag + 48);
     GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -781,7 +776,8 @@ extern "C" int gospl_hillslope(gospl_mesh_t* m, double cda, double cdm, double d
         xout = (xout == xa) ? xb : xa;
       }
       GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
-      k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout, keys, m->red_part, m->red_cnt);
+      k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, h, xin, xout, m->red_part);
+      k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys);
       GOSPL_CHECK(hipGetLastError());
       xin = xout;
       xout = (xout == xa) ? xb : xa;
@@ -1112,49 +1108,57 @@ extern "C" int gospl_nl_diffusion(gospl_mesh_t* m, const double* dh, const int8_
       k_nld_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, h, zb, mask, cd0,
                                            min_diff, dexp, dts, coef, diag);
       GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 3, s));
-      k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, keys + 2, m->red_part, m->red_cnt);
+      k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, m->red_part);
+      k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys + 2);
+      // the first Jacobi sweep from the current iterate, with its largest
+      // change and magnitude
+      k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, h, xa, m->red_part);
+      k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys);
       GOSPL_CHECK(hipGetLastError());
-      GOSPL_CHECK(hipMemcpyAsync(hk + 2, keys + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
       GOSPL_CHECK(hipStreamSynchronize(s));
       const double rho = hk[2] ? dkey_inv_host(hk[2]) : 0.0;
-      const double factor = rho < 1.0 ? rho / (1.0 - rho) : 1.0e300;
-      // sweeps that bring the contraction bound below 2^-60 before the first
-      // measured sweep, then chunks of 32
-      int first = 2;
-      if (rho > 0.0) {
-        const double n = std::ceil(-60.0 * std::log(2.0) / std::log(rho));
-        first = n > 62.0 ? 62 : (n < 2.0 ? 2 : (int)n);
-      }
-      const double* xin = h;
-      double* xout = xa;
-      int64_t done = 0;
-      bool conv = false;
-      int chunk = first;
-      while (!conv && done < GOSPL_HILL_MAX_SWEEPS) {
-        for (int k = 0; k < chunk - 1; ++k) {
-          k_hill_sweep<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, xin, xout);
-          xin = xout;
-          xout = (xout == xa) ? xb : xa;
-        }
-        GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 2, s));
-        k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, xin, xout, keys, m->red_part, m->red_cnt);
-        GOSPL_CHECK(hipGetLastError());
-        xin = xout;
-        xout = (xout == xa) ? xb : xa;
-        done += chunk;
-        GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
-        GOSPL_CHECK(hipStreamSynchronize(s));
-        const double chg = dkey_inv_host(hk[0]), mag = dkey_inv_host(hk[1]);
-        if (!(chg == chg) || !(mag == mag)) break;
-        conv = chg * factor <= 0x1p-52 * mag;
-        chunk = 32;
-      }
-      if (!conv) {
-        set_error("gospl_nl_diffusion: Jacobi sweeps did not converge (contraction %.6g)", rho);
+      const double chg = dkey_inv_host(hk[0]), mag = dkey_inv_host(hk[1]);
+      if (!(rho < 1.0) || !(chg == chg) || !(mag == mag)) {
+        set_error("gospl_nl_diffusion: Jacobi sweeps do not converge (contraction %.6g)", rho);
         return 1;
       }
-      total += done;
-      GOSPL_CHECK(hipMemcpyAsync(h, xin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+      // Chebyshev semi-iteration over the sweep (the operator is symmetric up
+      // to the row scaling by the cell areas, so the sweep's spectrum is real
+      // and within [-rho, rho]): x_{k+1} = w_k (sweep(x_k) - x_{k-1}) +
+      // x_{k-1}, w_1 = 1 / (1 - rho^2 / 2), w_k = 1 / (1 - rho^2 w_{k-1} / 4),
+      // for the least count whose error bound 2 sigma^k ||x0 - x*||, sigma =
+      // rho / (1 + sqrt(1 - rho^2)), ||x0 - x*|| <= chg / (1 - rho), with
+      // 2^16 of room for the norm that symmetrises the sweep against the max
+      // norm, is at most 2^-52 of the solution's magnitude.
+      const double r2 = rho * rho;
+      const double sig = rho / (1.0 + std::sqrt(1.0 - r2));
+      const double tgt = 0x1p-52 * mag;
+      double pw = 0x1p17 * chg / (1.0 - rho) * sig;
+      int nsw = 1;
+      while (pw > tgt && nsw < GOSPL_HILL_MAX_SWEEPS) {
+        pw *= sig;
+        ++nsw;
+      }
+      if (pw > tgt) {
+        set_error("gospl_nl_diffusion: %d sweeps do not reach the error bound (contraction %.6g)",
+                  (int)GOSPL_HILL_MAX_SWEEPS, rho);
+        return 1;
+      }
+      double* buf[3] = {h, xa, xb};
+      int prev = 0, cur = 1;
+      double omega = 1.0;
+      for (int k = 1; k < nsw; ++k) {
+        omega = k == 1 ? 1.0 / (1.0 - 0.5 * r2) : 1.0 / (1.0 - 0.25 * r2 * omega);
+        const int nxt = 3 - prev - cur;
+        k_hill_cheb<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, buf[cur], buf[prev], omega,
+                                              buf[nxt]);
+        prev = cur;
+        cur = nxt;
+      }
+      GOSPL_CHECK(hipGetLastError());
+      total += nsw;
+      if (cur != 0) GOSPL_CHECK(hipMemcpyAsync(h, buf[cur], sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     }
   }
   k_nld_out<<<g, GOSPL_BLOCK, 0, s>>>(N, h, zb, ndepo);

@@ -468,6 +468,7 @@ int gospl_mesh_destroy(gospl_mesh_t* m) {
   dfree(m->sea_pts);
   dfree(m->sea_i);
   dfree(m->sea_cell);
+  dfree(m->sea_bvh);
   if (m->sea_h) (void)hipHostFree(m->sea_h);
   dfree(m->seg_keys);
   dfree(m->seg_ctl);

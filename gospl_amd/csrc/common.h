@@ -116,6 +116,8 @@ struct gospl_mesh {
   int32_t* sea_i;             // coastline counts / offsets
   int32_t* sea_cell;          // grid cell of each coastline point
   int64_t sea_pcap;           // points the two hold
+  double* sea_bvh;            // bounding boxes of the sorted points' buckets (6 doubles per tree node)
+  int64_t sea_bvh_cap;        // nodes it holds
   unsigned long long* sea_h;  // pinned host block of the marine read-backs
   int32_t* sea_don;           // marine routing: donors of each vertex (W, N), by input id
   int8_t* sea_nd;             // their count

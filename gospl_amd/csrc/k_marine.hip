@@ -8,8 +8,10 @@
 //    the lower end, x = x_lo + t (x_hi - x_lo), t = (level - z_lo) /
 //    (z_hi - z_lo).  Each selected vertex gets the Euclidean distance to the
 //    nearest coastline point (goSPL queries a k-d tree): here an exact search
-//    over a uniform grid of the points, shells of cells around the query's
-//    cell until the next shell cannot hold a closer point.
+//    of a bounding-box tree over the points (counting-sorted by the Morton
+//    code of a uniform grid's cells, buckets of 8 consecutive points as
+//    leaves, a complete binary tree above them), nearer child first,
+//    pruning boxes that cannot hold a closer point.
 //  * gospl_marine_receivers: mfdrcvrs (functions.F90:1673-1785) as _matOcean
 //    calls it (12 receivers, sea level at the -1e6 sentinel): every lower
 //    neighbour on the filled surface whose drop to the power 1.1 over the
@@ -119,14 +121,26 @@ __device__ __forceinline__ int grid_axis(double x, double lo, double cs, int n) 
   return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
 
-__global__ void k_cst_cell(int64_t M, const double* __restrict__ pts, Grid g, int32_t* __restrict__ cell,
+// bits of x spread to every third position
+__device__ __forceinline__ unsigned int spread3(unsigned int x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+// Sort key of each point: the Morton code of its grid cell, the cell
+// coordinates shifted right by `sh` so that the keys stay below 2^(3 bits).
+__global__ void k_cst_cell(int64_t M, const double* __restrict__ pts, Grid g, int sh, int32_t* __restrict__ cell,
                            int32_t* __restrict__ cnt) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M) return;
-  const int cx = grid_axis(pts[3 * k], g.lo[0], g.cs, g.n[0]);
-  const int cy = grid_axis(pts[3 * k + 1], g.lo[1], g.cs, g.n[1]);
-  const int cz = grid_axis(pts[3 * k + 2], g.lo[2], g.cs, g.n[2]);
-  const int c = cx + g.n[0] * (cy + g.n[1] * cz);
+  const int cx = grid_axis(pts[3 * k], g.lo[0], g.cs, g.n[0]) >> sh;
+  const int cy = grid_axis(pts[3 * k + 1], g.lo[1], g.cs, g.n[1]) >> sh;
+  const int cz = grid_axis(pts[3 * k + 2], g.lo[2], g.cs, g.n[2]) >> sh;
+  const int c = (int)(spread3((unsigned)cx) | (spread3((unsigned)cy) << 1) | (spread3((unsigned)cz) << 2));
   cell[k] = c;
   atomicAdd(cnt + c, 1);
 }
@@ -141,9 +155,61 @@ __global__ void k_cst_scatter(int64_t M, const double* __restrict__ pts, const i
   for (int j = 0; j < 3; ++j) sorted[3 * (int64_t)slot + j] = pts[3 * k + j];
 }
 
-// nearest point of the grid for each selected vertex (0 elsewhere)
-__global__ void k_cst_query(int64_t N, const double* __restrict__ xyz, const int8_t* __restrict__ mask, Grid g,
-                            const int32_t* __restrict__ start, const double* __restrict__ sorted,
+#define CST_LEAF 8  // sorted points per leaf box
+
+// Leaf boxes: leaf j holds sorted points [8 j, 8 j + 8) (fewer in the last);
+// leaves past the points are empty boxes (lo = +inf, hi = -inf).  Tree in
+// heap order: node 1 the root, children 2 i and 2 i + 1, leaves at P .. 2 P - 1
+// (P a power of two); each node 6 doubles, lo xyz then hi xyz.
+__global__ void k_cst_leaves(int64_t M, int64_t P, const double* __restrict__ sorted, double* __restrict__ box) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  const int64_t a = j * CST_LEAF, b = a + CST_LEAF < M ? a + CST_LEAF : M;
+  for (int64_t k = a; k < b; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const double x = sorted[3 * k + d];
+      lo[d] = x < lo[d] ? x : lo[d];
+      hi[d] = x > hi[d] ? x : hi[d];
+    }
+  double* o = box + 6 * (P + j);
+  for (int d = 0; d < 3; ++d) {
+    o[d] = lo[d];
+    o[3 + d] = hi[d];
+  }
+}
+
+// Inner boxes, level by level up to the root, in one workgroup.
+__global__ __launch_bounds__(1024) void k_cst_inner(int64_t P, double* __restrict__ box) {
+  for (int64_t w = P >> 1; w >= 1; w >>= 1) {
+    for (int64_t i = w + threadIdx.x; i < 2 * w; i += blockDim.x) {
+      const double* l = box + 6 * (2 * i);
+      const double* r = box + 6 * (2 * i + 1);
+      double* o = box + 6 * i;
+      for (int d = 0; d < 3; ++d) {
+        o[d] = l[d] < r[d] ? l[d] : r[d];
+        o[3 + d] = l[3 + d] > r[3 + d] ? l[3 + d] : r[3 + d];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Lower bound of the squared distance from q to a box: each axis's gap, in
+// the distance's own order (rounding is monotone, so the bound never
+// exceeds a computed distance to a point inside); +inf for an empty box.
+__device__ __forceinline__ double box_lb(const double* b, double qx, double qy, double qz) {
+  if (!(b[0] <= b[3])) return INFINITY;
+  const double gx = qx < b[0] ? b[0] - qx : (qx > b[3] ? qx - b[3] : 0.0);
+  const double gy = qy < b[1] ? b[1] - qy : (qy > b[4] ? qy - b[4] : 0.0);
+  const double gz = qz < b[2] ? b[2] - qz : (qz > b[5] ? qz - b[5] : 0.0);
+  return (gx * gx + gy * gy) + gz * gz;
+}
+
+// nearest point for each selected vertex (0 elsewhere): depth-first, the
+// nearer child first, boxes no nearer than the best point so far skipped
+__global__ void k_cst_query(int64_t N, const double* __restrict__ xyz, const int8_t* __restrict__ mask, int64_t M,
+                            int64_t P, const double* __restrict__ box, const double* __restrict__ sorted,
                             double* __restrict__ dist) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -152,43 +218,31 @@ __global__ void k_cst_query(int64_t N, const double* __restrict__ xyz, const int
     return;
   }
   const double qx = xyz[3 * i], qy = xyz[3 * i + 1], qz = xyz[3 * i + 2];
-  const int cx = grid_axis(qx, g.lo[0], g.cs, g.n[0]);
-  const int cy = grid_axis(qy, g.lo[1], g.cs, g.n[1]);
-  const int cz = grid_axis(qz, g.lo[2], g.cs, g.n[2]);
-  int rmax = g.n[0] > g.n[1] ? g.n[0] : g.n[1];
-  rmax = rmax > g.n[2] ? rmax : g.n[2];
   double best = INFINITY;
-  for (int r = 0; r <= rmax; ++r) {
-    // a point in shell r (Chebyshev cell distance r from the query's cell,
-    // the query projected into the grid's box) is at least (r - 1) cs away;
-    // half a cell is kept back for the rounding of the cell assignments
-    if (r >= 2) {
-      const double lb = ((double)r - 1.5) * g.cs;
-      if (lb * lb >= best) break;
-    }
-    for (int dz = -r; dz <= r; ++dz) {
-      const int z = cz + dz;
-      if (z < 0 || z >= g.n[2]) continue;
-      for (int dy = -r; dy <= r; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= g.n[1]) continue;
-        const bool face = (dz == -r || dz == r || dy == -r || dy == r);
-        const int step = (face || r == 0) ? 1 : 2 * r;
-        for (int dx = -r; dx <= r; dx += step) {
-          const int x = cx + dx;
-          if (x < 0 || x >= g.n[0]) continue;
-          const int c = x + g.n[0] * (y + g.n[1] * z);
-          const int e = start[c + 1];
-          for (int k = start[c]; k < e; ++k) {
-            const double ex = qx - sorted[3 * (int64_t)k];
-            const double ey = qy - sorted[3 * (int64_t)k + 1];
-            const double ez = qz - sorted[3 * (int64_t)k + 2];
-            const double d2 = (ex * ex + ey * ey) + ez * ez;
-            best = d2 < best ? d2 : best;
-          }
-        }
+  int64_t stack[64];
+  int sp = 0;
+  stack[sp++] = 1;
+  while (sp > 0) {
+    const int64_t nd = stack[--sp];
+    if (box_lb(box + 6 * nd, qx, qy, qz) >= best) continue;
+    if (nd >= P) {
+      const int64_t a = (nd - P) * CST_LEAF, b = a + CST_LEAF < M ? a + CST_LEAF : M;
+      for (int64_t k = a; k < b; ++k) {
+        const double ex = qx - sorted[3 * k];
+        const double ey = qy - sorted[3 * k + 1];
+        const double ez = qz - sorted[3 * k + 2];
+        const double d2 = (ex * ex + ey * ey) + ez * ez;
+        best = d2 < best ? d2 : best;
       }
+      continue;
     }
+    const double la = box_lb(box + 6 * (2 * nd), qx, qy, qz);
+    const double lb = box_lb(box + 6 * (2 * nd + 1), qx, qy, qz);
+    // the farther child below the nearer one (depth <= 62: two per level)
+    const int64_t nf = la <= lb ? 2 * nd + 1 : 2 * nd, nn = la <= lb ? 2 * nd : 2 * nd + 1;
+    const double lf = la <= lb ? lb : la, ln = la <= lb ? la : lb;
+    if (lf < best && sp < 64) stack[sp++] = nf;
+    if (ln < best && sp < 64) stack[sp++] = nn;
   }
   dist[i] = sqrt(best);
 }
@@ -512,17 +566,37 @@ int gospl_coast_distance(gospl_mesh_t* m, const double* xyz, const double* z, do
     int n = (int)std::floor((hi - g.lo[j]) / g.cs) + 1;
     g.n[j] = n < 1 ? 1 : (n > G + 1 ? G + 1 : n);
   }
-  const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  // cell counts reuse cnt (ncell + 1 <= N + 1 entries), their starts off
-  GOSPL_CHECK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (ncell + 1), s));
-  k_cst_cell<<<nblocks(M), GOSPL_BLOCK, 0, s>>>(M, pts, g, cellid, cnt);
+  // sort keys: Morton codes of the cells, the cell coordinates cut to b bits
+  // each so that the 2^(3 b) keys fit the N + 64 counts
+  int gbits = 0;
+  while ((1 << gbits) < g.n[0] || (1 << gbits) < g.n[1] || (1 << gbits) < g.n[2]) ++gbits;
+  int b = gbits < 10 ? gbits : 10;
+  while (b > 0 && (int64_t(1) << (3 * b)) + 1 > N + 64) --b;
+  const int sh = gbits - b;
+  const int64_t nkey = int64_t(1) << (3 * b);
+  GOSPL_CHECK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nkey + 1), s));
+  k_cst_cell<<<nblocks(M), GOSPL_BLOCK, 0, s>>>(M, pts, g, sh, cellid, cnt);
   GOSPL_CHECK(hipGetLastError());
   bytes = m->cub_tmp_bytes;
-  GOSPL_CHECK(rocprim::exclusive_scan(m->cub_tmp, bytes, cnt, off, 0, (size_t)(ncell + 1), rocprim::plus<int32_t>(),
+  GOSPL_CHECK(rocprim::exclusive_scan(m->cub_tmp, bytes, cnt, off, 0, (size_t)(nkey + 1), rocprim::plus<int32_t>(),
                                       s));
-  GOSPL_CHECK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (ncell + 1), s));
+  GOSPL_CHECK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nkey + 1), s));
   k_cst_scatter<<<nblocks(M), GOSPL_BLOCK, 0, s>>>(M, pts, cellid, off, cnt, sorted);
-  k_cst_query<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, xyz, mask, g, off, sorted, dist);
+  // the tree: leaves of CST_LEAF consecutive sorted points, P leaves (a power
+  // of two), nodes 1 .. 2 P - 1
+  const int64_t nleaf = (M + CST_LEAF - 1) / CST_LEAF;
+  int64_t P = 1;
+  while (P < nleaf) P <<= 1;
+  if (2 * P > m->sea_bvh_cap) {
+    if (m->sea_bvh) GOSPL_CHECK(hipFree(m->sea_bvh));
+    m->sea_bvh = nullptr;
+    m->sea_bvh_cap = 0;
+    GOSPL_CHECK(hipMalloc(&m->sea_bvh, sizeof(double) * 6 * 2 * P));
+    m->sea_bvh_cap = 2 * P;
+  }
+  k_cst_leaves<<<nblocks(P), GOSPL_BLOCK, 0, s>>>(M, P, sorted, m->sea_bvh);
+  if (P > 1) k_cst_inner<<<1, 1024, 0, s>>>(P, m->sea_bvh);
+  k_cst_query<<<nblocks(N), GOSPL_BLOCK, 0, s>>>(N, xyz, mask, M, P, m->sea_bvh, sorted, dist);
   GOSPL_CHECK(hipGetLastError());
   return 0;
 }

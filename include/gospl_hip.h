@@ -702,10 +702,12 @@ int gospl_marine_apply(gospl_mesh_t* mesh, const double* t, double dt, double* h
  * h = bed + dh; nsub backward-Euler sub-steps of dt / nsub, each with npic
  * Picard updates solving (I + dt_sub L(Cd(h))) h' = h_start with Cd =
  * min_diff + cd0 (1 - exp(-dexp max(h - bed, 0 below 0.1))) on mask cells,
- * 0 elsewhere; ndepo = max(h - bed, 0).  Each linear solve runs to
- * convergence (goSPL's Richardson stops at rtol 1e-8); *sweeps (HOST, may
- * be NULL) receives the Jacobi sweeps used.  Synchronises `stream` once per
- * linear solve.
+ * 0 elsewhere; ndepo = max(h - bed, 0).  Each linear solve is the
+ * Chebyshev semi-iteration over the Jacobi sweep, for the count at which
+ * its error bound (from the first sweep's change and the contraction bound
+ * rho = max (diag - 1) / diag) is at most 2^-52 of the solution's magnitude
+ * (goSPL's Richardson stops at rtol 1e-8); *sweeps (HOST, may be NULL)
+ * receives the sweeps used.  Synchronises `stream` once per linear solve.
  */
 int gospl_nl_diffusion(gospl_mesh_t* mesh, const double* dh, const int8_t* mask, double cd0, double min_diff,
                        double dexp, double dt, int32_t nsub, int32_t npic, const double* bed, double* ndepo,

@@ -153,3 +153,61 @@ def test_coast_distance_through_c_abi():
     g._call("gospl_coast_distance", g.H, _ptr(xyz), _ptr(z), 0.0, None, _ptr(out), ctypes.byref(n), g.stream)
     assert n.value == 0 and float(out.abs().max()) == 0.0
     g.close()
+
+
+@pytest.mark.parametrize("scale", [1.0, 40.0])
+def test_diffuse_fixed_matches_oracle_bitwise(scale):
+    """gospl_diffuse_fixed (the marine smoothing: Chebyshev semi-iteration
+    over the Jacobi sweep) through the C ABI against the oracle's
+    smooth_solve on the level-5 sphere, bit for bit, with the marine Kd and
+    with it scaled by 40 (contraction ~0.9987: ~800 steps)."""
+    from gospl_amd.model import Model, _ptr
+    from oracle import oracle as orc
+    from oracle import sea
+
+    m, h0, p = case("sphere", 5)
+    g = Model(m, h0, tend=1e12, **p)
+    o = oracle_model(m, h0, p)
+    rng = np.random.default_rng(21)
+    h_in = m.to_input_order(h0) + rng.uniform(-50.0, 50.0, m.N)
+    Cd = np.where(h_in < 0.0, sea.MARINE_SMOOTH_N_SEA, sea.MARINE_SMOOTH_N_LAND)
+    kd_in = scale * Cd * o.larea
+    coeff = orc.sethillslopecoeff(kd_in, o.ngb, o.nnb, o.larea, o.elgt, o.vdist)
+    want, nsw = orc.smooth_solve(coeff, h_in.copy(), o.ngb, o.nnb)
+    kd = g._dev(m.to_device_order(kd_in))
+    rhs = g._dev(m.to_device_order(h_in))
+    out = torch.zeros(m.N, dtype=torch.float64, device=g.device)
+    sw = ctypes.c_int32(0)
+    g._call("gospl_diffuse_fixed", g.H, _ptr(kd), _ptr(rhs), _ptr(out), 1000000, ctypes.byref(sw), g.stream)
+    got = m.to_input_order(out.cpu().numpy())
+    print("diffuse_fixed scale %g: %d steps (oracle %d)" % (scale, sw.value, nsw), flush=True)
+    assert sw.value == nsw
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    g.close()
+
+
+def test_coast_distance_single_island():
+    """A coastline of fewer points than one leaf of the search tree: one
+    vertex above the level, all others below."""
+    from scipy import spatial
+
+    from gospl_amd.model import Model, _ptr
+    from oracle import sea
+
+    m, h0, p = case("sphere", 5)
+    g = Model(m, h0, tend=1e12, **p)
+    o = oracle_model(m, h0, p)
+    xyz = torch.as_tensor(np.ascontiguousarray(m.coords, np.float64)).to(g.device)
+    z_in = np.full(m.N, -1.0)
+    z_in[123] = 1.0
+    pts = sea.contour_points(z_in, 0.0, o.coords, o.ngb, o.nnb)
+    assert 0 < len(pts) < 8
+    want = spatial.cKDTree(pts).query(o.coords, k=1)[0]
+    z = g._dev(m.to_device_order(z_in))
+    out = torch.zeros(m.N, dtype=torch.float64, device=g.device)
+    n = ctypes.c_int64(0)
+    g._call("gospl_coast_distance", g.H, _ptr(xyz), _ptr(z), 0.0, None, _ptr(out), ctypes.byref(n), g.stream)
+    got = m.to_input_order(out.cpu().numpy())
+    assert n.value == len(pts)
+    assert np.array_equal(got, want), np.abs(got - want).max()
+    g.close()

@@ -89,7 +89,30 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_pit_reduce_params(
   const int b = start[p], e = end[p];
   double v[1] = {0.0};
   double hm = -INFINITY, dh = -INFINITY;
-  for (int i = b + (int)threadIdx.x; i < e; i += GOSPL_BLOCK) {
+  // eight cells at a time, loads first, then the plain loop's additions in
+  // its order
+  constexpr int U = 8;
+  int i = b + (int)threadIdx.x;
+  for (; i + (U - 1) * GOSPL_BLOCK < e; i += U * GOSPL_BLOCK) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = (int)(keys[i + u * GOSPL_BLOCK] & 0xffffffffull);
+    double f[U], h[U], a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      f[u] = fill[c[u]];
+      h[u] = hl[c[u]];
+      a[u] = area[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double hc = h[u] < sl ? sl : h[u];
+      v[0] = v[0] + (f[u] - hc) * a[u];
+      hm = fmax(hm, f[u]);
+      dh = fmax(dh, f[u] - hc);
+    }
+  }
+  for (; i < e; i += GOSPL_BLOCK) {
     const int c = (int)(keys[i] & 0xffffffffull);
     const double f = fill[c];
     const double h = hl[c];
@@ -138,7 +161,27 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_pit_reduce_vol(
 #pragma unroll
   for (int k = 0; k < 4; ++k) l[k] = lvl[k * cap + p];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = b + (int)threadIdx.x; i < e; i += GOSPL_BLOCK) {
+  constexpr int U = 8;
+  int i = b + (int)threadIdx.x;
+  for (; i + (U - 1) * GOSPL_BLOCK < e; i += U * GOSPL_BLOCK) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = (int)(keys[i + u * GOSPL_BLOCK] & 0xffffffffull);
+    double hh[U], aa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      hh[u] = hl[c[u]];
+      aa[u] = area[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double h = hh[u] < sl ? sl : hh[u];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (l[k] > h) v[k] = v[k] + (l[k] - h) * aa[u];
+    }
+  }
+  for (; i < e; i += GOSPL_BLOCK) {
     const int c = (int)(keys[i] & 0xffffffffull);
     double h = hl[c];
     if (h < sl) h = sl;
@@ -164,7 +207,28 @@ __global__ __launch_bounds__(GOSPL_BLOCK) void k_pit_reduce_inflow(
   const int p = blockIdx.x;
   const int b = start[p], e = end[p];
   double v[1] = {0.0};
-  for (int i = b + (int)threadIdx.x; i < e; i += GOSPL_BLOCK) {
+  // each thread's cells in increasing position, eight at a time with all
+  // their loads issued before the additions (the order of the sum is that of
+  // the plain strided loop; a depression of a million cells is otherwise a
+  // chain of dependent gathers on one CU)
+  constexpr int U = 8;
+  int i = b + (int)threadIdx.x;
+  for (; i + (U - 1) * GOSPL_BLOCK < e; i += U * GOSPL_BLOCK) {
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = (int)(keys[i + u * GOSPL_BLOCK] & 0xffffffffull);
+    int8_t f[U];
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      f[u] = lsink[c[u]];
+      x[u] = fa[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (f[u]) v[0] = v[0] + x[u];
+  }
+  for (; i < e; i += GOSPL_BLOCK) {
     const int c = (int)(keys[i] & 0xffffffffull);
     if (lsink[c]) v[0] = v[0] + fa[c];
   }

@@ -63,7 +63,7 @@ struct gospl_mesh {
   int8_t* rev;                // (W, N) position of i in the neighbour list of its neighbour q
   int64_t wdon_cap;
   long long* trace_buf;       // per-level trace (GOSPL_DAG_TRACE)
-  int iter_hint[4];           // passes the relaxation loops needed last time
+  int iter_hint[6];           // passes the relaxation loops needed last time (one slot per loop and mode)
   // cells grouped by depression (sorted (pit id << 32 | cell) keys + runs)
   unsigned long long* pg_keys[2];
   int32_t* pg_seg;            // start (pg_cap + 64) then end (pg_cap + 64)

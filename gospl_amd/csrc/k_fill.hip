@@ -794,7 +794,7 @@ namespace gospl {
 static int pit_info_body(gospl_mesh_t* m, const double* hl, double* lfill, double label_lvl, double sealevel,
                          double clip_lvl, int32_t* pitids, int32_t* flatdirs, int32_t* ptdir, int8_t* lspill,
                          int64_t cap, int32_t* spill, double* pitparams, double* filled_lvl,
-                         double* filled_vol, int64_t* npits_out, gospl_stream_t s) {
+                         double* filled_vol, int64_t* npits_out, gospl_stream_t s, int* ptdir_hint) {
   const int64_t N = m->N;
   const int g = nblocks(N);
   int32_t* parent = m->wi[3];
@@ -857,7 +857,7 @@ static int pit_info_body(gospl_mesh_t* m, const double* hl, double* lfill, doubl
   GOSPL_CHECK(hipGetLastError());
   if (npits > 0 && npc > 0) {
     int it = 0;
-    if (until_stable(m, s, npc + 32, 16, &m->iter_hint[2], [&](int32_t* f, int pass) {
+    if (until_stable(m, s, npc + 32, 16, ptdir_hint, [&](int32_t* f, int pass) {
           k_ptdir_relax<<<(int)((npc + PTDIR_BLOCK - 1) / PTDIR_BLOCK), PTDIR_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, lfill, ncells,
                                                              cells, ptdir, f, pass);
         }, &it))
@@ -880,7 +880,7 @@ int gospl_pit_info(gospl_mesh_t* m, const double* hl, double* lfill, double seal
                    int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
                    int64_t* npits_out, gospl_stream_t s) {
   return pit_info_body(m, hl, lfill, sealevel, sealevel, sealevel, pitids, flatdirs, ptdir, lspill, cap, spill,
-                       pitparams, filled_lvl, filled_vol, npits_out, s);
+                       pitparams, filled_lvl, filled_vol, npits_out, s, &m->iter_hint[2]);
 }
 
 int gospl_pit_info_sed(gospl_mesh_t* m, const double* hl, double* lfill, double level, double sealevel,
@@ -888,7 +888,7 @@ int gospl_pit_info_sed(gospl_mesh_t* m, const double* hl, double* lfill, double 
                        int32_t* spill, double* pitparams, double* filled_lvl, double* filled_vol,
                        int64_t* npits_out, gospl_stream_t s) {
   return pit_info_body(m, hl, lfill, level, sealevel, -INFINITY, pitids, flatdirs, ptdir, lspill, cap, spill,
-                       pitparams, filled_lvl, filled_vol, npits_out, s);
+                       pitparams, filled_lvl, filled_vol, npits_out, s, &m->iter_hint[4]);
 }
 
 }  // extern "C"

@@ -339,7 +339,7 @@ static int mesh_alloc(gospl_mesh* m) {
   for (int k = 0; k < 4; ++k) GOSPL_CHECK(hipMalloc(&m->wd[k], sizeof(double) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->counts, sizeof(int32_t) * (N + 64)));
   GOSPL_CHECK(hipMalloc(&m->dflag, sizeof(int32_t) * 64));
-  GOSPL_CHECK(hipMalloc(&m->red_part, sizeof(unsigned long long) * 2 * (nblocks(N) + 64)));
+  GOSPL_CHECK(hipMalloc(&m->red_part, sizeof(unsigned long long) * 3 * (nblocks(N) + 64)));
   GOSPL_CHECK(hipHostMalloc((void**)&m->hflag, sizeof(int32_t) * 64, hipHostMallocDefault));
   // rocPRIM workspace: scans of N + 1 ints, selects over N ids, sorts of N 64-bit keys
   size_t bytes = 0, b2 = 0;

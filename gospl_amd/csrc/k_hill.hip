@@ -1028,37 +1028,111 @@ __device__ __forceinline__ double nld_cd(int64_t i, const double* __restrict__ h
   return mind + cd0 * (1.0 - exp(-dexp * dh));
 }
 
-__global__ void k_nld_rows(int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
-                           const double* __restrict__ area, const double* __restrict__ elgt,
-                           const double* __restrict__ vdist, const double* __restrict__ h,
-                           const double* __restrict__ zb, const int8_t* __restrict__ mask, double cd0,
-                           double mind, double dexp, double dts, double* __restrict__ coef,
-                           double* __restrict__ diag) {
+// The jacobiancoeff rows (Kp = 0) of the current iterate h, with what the
+// linear solve needs first: the row's contraction (d - 1) / d and the first Jacobi sweep from x0 = h with the
+// row's own coefficients (k_hill_sweep_chk's operation order), xout = x1;
+// per-workgroup maxima of |x1 - x0|, |x1| and the contraction into
+// part[3 b + 0..2] for k_max3_finish.
+__global__ __launch_bounds__(GOSPL_BLOCK) void k_nld_rows_first(
+    int64_t N, int W, const int32_t* __restrict__ nbr, const int32_t* __restrict__ deg,
+    const double* __restrict__ area, const double* __restrict__ elgt, const double* __restrict__ vdist,
+    const double* __restrict__ h, const double* __restrict__ zb, const int8_t* __restrict__ mask, double cd0,
+    double mind, double dexp, double dts, const double* __restrict__ rhs, double* __restrict__ coef,
+    double* __restrict__ diag, double* __restrict__ xout, unsigned long long* __restrict__ part) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  double s = 0.0;
-  const double a = area[i];
-  const double ck = nld_cd(i, h, zb, mask, cd0, mind, dexp);
-  const int dg = deg[i];
-  for (int p = 0; p < W; ++p) {
-    double off = 0.0;
-    if (a > 0.0 && p < dg) {
-      const int64_t e = (int64_t)p * N + i;
-      const double vd = vdist[e];
-      if (vd > 0.0) {
+  double chg = 0.0, mag = 0.0, rho = 0.0;
+  if (i < N) {
+    double s = 0.0;
+    const double a = area[i];
+    const double ck = nld_cd(i, h, zb, mask, cd0, mind, dexp);
+    const int dg = deg[i];
+    double off[GOSPL_MAXW];
+    int nb[GOSPL_MAXW];
+#pragma unroll
+    for (int p = 0; p < GOSPL_MAXW; ++p) {
+      off[p] = 0.0;
+      nb[p] = 0;
+      if (p >= W) continue;
+      if (a > 0.0 && p < dg) {
+        const int64_t e = (int64_t)p * N + i;
+        const double vd = vdist[e];
         const int n = nbr[e];
-        const double cn = nld_cd(n, h, zb, mask, cd0, mind, dexp);
-        if (ck > 0.0 && cn > 0.0) {
-          const double c = 0.5 * (ck + cn) / a;
-          const double v = c * vd / elgt[e];
-          s = s + v;
-          off = dts * (-v);
+        nb[p] = n;
+        if (vd > 0.0) {
+          const double cn = nld_cd(n, h, zb, mask, cd0, mind, dexp);
+          if (ck > 0.0 && cn > 0.0) {
+            const double c = 0.5 * (ck + cn) / a;
+            const double v = c * vd / elgt[e];
+            s = s + v;
+            off[p] = dts * (-v);
+          }
         }
       }
+      coef[(int64_t)p * N + i] = off[p];
     }
-    coef[(int64_t)p * N + i] = off;
+    const double d = dts * s + 1.0;
+    diag[i] = d;
+    if (d > 0.0) rho = (d - 1.0) / d;
+    const double r = rhs[i];
+    double x = r;
+    if (d != 0.0) {
+      double t = r;
+#pragma unroll
+      for (int p = 0; p < GOSPL_MAXW; ++p)
+        if (p < W && p < dg && off[p] != 0.0) t = t - off[p] * h[nb[p]];
+      x = t / d;
+    }
+    xout[i] = x;
+    chg = fabs(x - h[i]);
+    mag = fabs(x);
   }
-  diag[i] = dts * s + 1.0;
+  for (int o = 32; o > 0; o >>= 1) {
+    chg = fmax(chg, __shfl_xor(chg, o, 64));
+    mag = fmax(mag, __shfl_xor(mag, o, 64));
+    rho = fmax(rho, __shfl_xor(rho, o, 64));
+  }
+  __shared__ double sh[3][GOSPL_BLOCK / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = chg;
+    sh[1][threadIdx.x >> 6] = mag;
+    sh[2][threadIdx.x >> 6] = rho;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < GOSPL_BLOCK / 64; ++w) {
+      chg = fmax(chg, sh[0][w]);
+      mag = fmax(mag, sh[1][w]);
+      rho = fmax(rho, sh[2][w]);
+    }
+    part[3 * blockIdx.x] = dkey(chg);
+    part[3 * blockIdx.x + 1] = dkey(mag);
+    part[3 * blockIdx.x + 2] = rho > 0.0 ? dkey(rho) : 0ull;
+  }
+}
+
+// The three maxima of k_nld_rows_first's workgroups into keys[0..2].
+__global__ __launch_bounds__(1024) void k_max3_finish(int nblk, const unsigned long long* __restrict__ part,
+                                                      unsigned long long* __restrict__ keys) {
+  __shared__ unsigned long long sm[3][1024 / 64];
+  unsigned long long m3[3] = {0ull, 0ull, 0ull};
+  for (int k = threadIdx.x; k < nblk; k += blockDim.x)
+    for (int j = 0; j < 3; ++j) {
+      const unsigned long long x = part[3 * k + j];
+      m3[j] = x > m3[j] ? x : m3[j];
+    }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int j = 0; j < 3; ++j) {
+      const unsigned long long x = __shfl_xor(m3[j], o);
+      m3[j] = x > m3[j] ? x : m3[j];
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int j = 0; j < 3; ++j) sm[j][threadIdx.x >> 6] = m3[j];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 3; ++j) {
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m3[j] = sm[j][w] > m3[j] ? sm[j][w] : m3[j];
+      keys[j] = m3[j] > keys[j] ? m3[j] : keys[j];
+    }
 }
 
 __global__ void k_nld_init(int64_t N, const double* __restrict__ dh, const double* __restrict__ zb,
@@ -1105,15 +1179,12 @@ extern "C" int gospl_nl_diffusion(gospl_mesh_t* m, const double* dh, const int8_
   for (int sub = 0; sub < nsub; ++sub) {
     GOSPL_CHECK(hipMemcpyAsync(hs, h, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     for (int it = 0; it < npic; ++it) {
-      k_nld_rows<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, h, zb, mask, cd0,
-                                           min_diff, dexp, dts, coef, diag);
+      // the rows, their contraction and the first Jacobi sweep from the
+      // current iterate (its largest change and magnitude) in one pass
       GOSPL_CHECK(hipMemsetAsync(keys, 0, sizeof(unsigned long long) * 3, s));
-      k_hill_rho<<<g, GOSPL_BLOCK, 0, s>>>(N, diag, m->red_part);
-      k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys + 2);
-      // the first Jacobi sweep from the current iterate, with its largest
-      // change and magnitude
-      k_hill_sweep_chk<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, coef, diag, hs, h, xa, m->red_part);
-      k_max2_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys);
+      k_nld_rows_first<<<g, GOSPL_BLOCK, 0, s>>>(N, m->W, m->nbr, m->deg, m->area, m->elgt, m->vdist, h, zb, mask,
+                                                 cd0, min_diff, dexp, dts, hs, coef, diag, xa, m->red_part);
+      k_max3_finish<<<1, 1024, 0, s>>>(g, m->red_part, keys);
       GOSPL_CHECK(hipGetLastError());
       GOSPL_CHECK(hipMemcpyAsync(hk, keys, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, s));
       GOSPL_CHECK(hipStreamSynchronize(s));
